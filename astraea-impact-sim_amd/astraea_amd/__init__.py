@@ -1,0 +1,370 @@
+"""ctypes binding of libastraea_hip.so, the MI355X (gfx950) per-step physics library.
+
+The C ABI (include/astraea_hip.h) replaces the reference's CUDA launch layer
+(engine/sph_gravity.cuh:61-92) under `astraea::Simulation` (engine/simulation.hpp).
+This module mirrors it for Python callers: `Context` owns one device context,
+moves per-particle fields as numpy arrays and runs the stage entry points or
+whole KDK steps.  `scenario` builds the benchmark bodies on the host.
+
+There is no CPU fallback: if the shared library is missing this module raises
+on import.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libastraea_hip.so"
+
+if not LIB_PATH.exists():
+    raise ImportError(
+        f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+        "or `make -C astraea-impact-sim_amd/csrc`"
+    )
+_lib = C.CDLL(str(LIB_PATH))
+
+# ---- constants mirrored from include/astraea_hip.h ----
+OK = 0
+PREC_FP32 = 0
+PREC_FP64_ACCUM = 1
+GROUP_SIZE = 64
+SEGMENT_ALIGN = 256
+STACK_DEPTH = 192
+ENERGY_STRIDE = 18
+EN = dict(mass=0, kinetic=1, potential=2, internal=3, mom_x=4, mom_y=5, mom_z=6, ang_x=7, ang_y=8, ang_z=9,
+          radiated=10, rad_power_w=11, max_temp=12, max_pressure=13, com_x=14, com_y=15, com_z=16, sim_time=17)
+STAGES = ["dt", "kick1", "drift", "morton", "sort", "permute_tree", "moments", "neighbours", "pass1", "gravity",
+          "pass2", "kick2"]
+
+_FIELD_NAMES = [
+    "POS_X", "POS_Y", "POS_Z", "VEL_X", "VEL_Y", "VEL_Z", "MASS", "H", "H0", "RHO", "U",
+    "SXX", "SYY", "SZZ", "SXY", "SXZ", "SYZ", "DAMAGE", "DAMAGE_CBRT", "EPS_FAIL",
+    "MATERIAL", "BODY", "TRACER", "ORIGINAL_ID",
+    "POS64_X", "POS64_Y", "POS64_Z", "VEL64_X", "VEL64_Y", "VEL64_Z",
+    "ACC_X", "ACC_Y", "ACC_Z", "ACC_GRAV_X", "ACC_GRAV_Y", "ACC_GRAV_Z", "PHI",
+    "DRHO_DT", "DU_DT", "PRESSURE", "SOUND_SPEED", "TEMP_K",
+    "SHEPARD", "SURF_FACTOR", "DIV_V", "CURL_X", "CURL_Y", "CURL_Z",
+    "DS_XX", "DS_YY", "DS_ZZ", "DS_XY", "DS_XZ", "DS_YZ",
+    "DDAMAGE_CBRT_DT", "RAD_POWER_W", "PHASE", "NGB_COUNT",
+]
+FIELD = {name.lower(): k for k, name in enumerate(_FIELD_NAMES)}
+_U8 = {"material", "body", "tracer", "phase"}
+_U32 = {"original_id", "ngb_count"}
+_F64 = {"pos64_x", "pos64_y", "pos64_z", "vel64_x", "vel64_y", "vel64_z"}
+
+
+def field_dtype(name: str):
+    if name in _U8:
+        return np.uint8
+    if name in _U32:
+        return np.uint32
+    if name in _F64:
+        return np.float64
+    return np.float32
+
+
+class Params(C.Structure):
+    """astraea_params (include/astraea_hip.h)."""
+
+    _fields_ = [
+        ("theta", C.c_float),
+        ("soft_factor", C.c_float),
+        ("cfl_factor", C.c_float),
+        ("gamma_hhe", C.c_float),
+        ("cooling_min_temp_k", C.c_float),
+        ("strength_mask", C.c_uint32),
+        ("precision", C.c_uint32),
+        ("reserved", C.c_uint32 * 5),
+    ]
+
+
+MAX_LAYERS = 4
+
+
+class BodySpec(C.Structure):
+    """astraea_body_spec (include/astraea_hip.h)."""
+
+    _fields_ = [
+        ("radius_m", C.c_double),
+        ("num_layers", C.c_uint32),
+        ("layer_material", C.c_uint8 * MAX_LAYERS),
+        ("layer_mass_frac", C.c_double * MAX_LAYERS),
+        ("layer_density_kgm3", C.c_double * MAX_LAYERS),
+        ("h_over_spacing", C.c_double),
+        ("jitter", C.c_double),
+        ("surface_temp_k", C.c_double),
+        ("strength_flaws", C.c_uint32),
+        ("weibull_m", C.c_double),
+        ("weibull_k", C.c_double),
+        ("seed", C.c_uint64),
+    ]
+
+
+class BodyOut(C.Structure):
+    _fields_ = [(n, C.POINTER(C.c_float)) for n in ("x", "y", "z", "m", "h", "rho", "u", "eps_fail")] + [
+        ("mat", C.POINTER(C.c_uint8))
+    ]
+
+
+_P = C.c_void_p
+_sig = {
+    "astraea_last_error": (C.c_char_p, []),
+    "astraea_abi_version": (C.c_int, []),
+    "astraea_field_size": (C.c_size_t, [C.c_int]),
+    "astraea_default_params": (None, [C.POINTER(Params)]),
+    "astraea_create": (C.c_int, [C.POINTER(_P), C.c_int, C.c_uint32, C.c_uint32]),
+    "astraea_destroy": (C.c_int, [_P]),
+    "astraea_stream": (_P, [_P]),
+    "astraea_set_params": (C.c_int, [_P, C.POINTER(Params)]),
+    "astraea_get_params": (C.c_int, [_P, C.POINTER(Params)]),
+    "astraea_set_layout": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "astraea_num_particles": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
+    "astraea_upload_field": (C.c_int, [_P, C.c_int, _P]),
+    "astraea_download_field": (C.c_int, [_P, C.c_int, _P]),
+    "astraea_clear_stats": (C.c_int, [_P]),
+    "astraea_sort_and_build_bvh": (C.c_int, [_P]),
+    "astraea_sph_pass1": (C.c_int, [_P]),
+    "astraea_evaluate_eos": (C.c_int, [_P]),
+    "astraea_gravity": (C.c_int, [_P]),
+    "astraea_sph_pass2": (C.c_int, [_P]),
+    "astraea_reduce_dt": (C.c_int, [_P, C.c_float, C.POINTER(C.c_float)]),
+    "astraea_kick_half": (C.c_int, [_P, C.c_int]),
+    "astraea_drift": (C.c_int, [_P]),
+    "astraea_evaluate_initial_forces": (C.c_int, [_P]),
+    "astraea_step": (C.c_int, [_P, C.c_float, C.c_uint32, C.c_int]),
+    "astraea_synchronize": (C.c_int, [_P]),
+    "astraea_get_dt": (C.c_int, [_P, C.POINTER(C.c_float)]),
+    "astraea_energy": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "astraea_counters": (C.c_int, [_P, C.POINTER(C.c_uint64), C.c_int]),
+    "astraea_download_tree": (C.c_int, [_P, C.c_uint32] + [_P] * 9),
+    "astraea_enable_stage_timing": (C.c_int, [_P, C.c_int]),
+    "astraea_stage_times": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "astraea_default_body_spec": (None, [C.POINTER(BodySpec)]),
+    "astraea_build_body": (C.c_int, [C.POINTER(BodySpec), C.c_uint32, C.POINTER(BodyOut), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]),
+    "astraea_place_two_bodies": (C.c_int, [C.c_uint32, C.c_uint32] + [_P] * 7 +
+                                 [C.c_double] * 5 + [C.POINTER(C.c_double)]),
+    "astraea_splitmix64": (C.c_uint64, [C.c_uint64]),
+}
+for _name, (_res, _args) in _sig.items():
+    _fn = getattr(_lib, _name)
+    _fn.restype = _res
+    _fn.argtypes = _args
+
+EXPORTED_SYMBOLS = sorted(_sig)
+
+
+class AstraeaError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"astraea error {code}: {msg}")
+        self.code = code
+
+
+def _check(rc: int):
+    if rc != OK:
+        raise AstraeaError(rc, _lib.astraea_last_error().decode())
+
+
+def default_params(**overrides) -> Params:
+    p = Params()
+    _lib.astraea_default_params(C.byref(p))
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """One device context (astraea_create)."""
+
+    def __init__(self, capacity: int, max_segments: int = 1, device: int = 0):
+        h = C.c_void_p()
+        _check(_lib.astraea_create(C.byref(h), device, capacity, max_segments))
+        self._h = h
+        self.counts: list[int] = []
+
+    def close(self):
+        if self._h:
+            _lib.astraea_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self) -> int:
+        return _lib.astraea_stream(self._h) or 0
+
+    @property
+    def n(self) -> int:
+        return int(sum(self.counts))
+
+    def set_params(self, p: Params):
+        _check(_lib.astraea_set_params(self._h, C.byref(p)))
+
+    def params(self) -> Params:
+        p = Params()
+        _check(_lib.astraea_get_params(self._h, C.byref(p)))
+        return p
+
+    def set_layout(self, counts):
+        arr = (C.c_uint32 * len(counts))(*[int(x) for x in counts])
+        _check(_lib.astraea_set_layout(self._h, len(counts), arr))
+        self.counts = [int(x) for x in counts]
+
+    def upload(self, name: str, values):
+        a = np.ascontiguousarray(values, dtype=field_dtype(name))
+        if a.shape != (self.n,):
+            raise ValueError(f"{name}: expected {self.n} values, got {a.shape}")
+        _check(_lib.astraea_upload_field(self._h, FIELD[name], _ptr(a)))
+
+    def download(self, name: str) -> np.ndarray:
+        a = np.empty(self.n, dtype=field_dtype(name))
+        _check(_lib.astraea_download_field(self._h, FIELD[name], _ptr(a)))
+        return a
+
+    def load_state(self, state: dict):
+        for k, v in state.items():
+            self.upload(k, v)
+
+    # stage entry points (reference launch layer)
+    def clear_stats(self):
+        _check(_lib.astraea_clear_stats(self._h))
+
+    def sort_and_build(self):
+        _check(_lib.astraea_sort_and_build_bvh(self._h))
+
+    def pass1(self):
+        _check(_lib.astraea_sph_pass1(self._h))
+
+    def eos(self):
+        _check(_lib.astraea_evaluate_eos(self._h))
+
+    def gravity(self):
+        _check(_lib.astraea_gravity(self._h))
+
+    def pass2(self):
+        _check(_lib.astraea_sph_pass2(self._h))
+
+    def reduce_dt(self, max_dt: float) -> np.ndarray:
+        out = (C.c_float * len(self.counts))()
+        _check(_lib.astraea_reduce_dt(self._h, max_dt, out))
+        return np.array(out[:], dtype=np.float32)
+
+    def kick_half(self, clamp_thermal: bool = False):
+        _check(_lib.astraea_kick_half(self._h, int(clamp_thermal)))
+
+    def drift(self):
+        _check(_lib.astraea_drift(self._h))
+
+    def initial_forces(self):
+        _check(_lib.astraea_evaluate_initial_forces(self._h))
+
+    def step(self, max_dt: float, steps: int = 1, graph: bool = False):
+        _check(_lib.astraea_step(self._h, max_dt, steps, int(graph)))
+
+    def synchronize(self):
+        _check(_lib.astraea_synchronize(self._h))
+
+    def dt(self) -> np.ndarray:
+        out = (C.c_float * len(self.counts))()
+        _check(_lib.astraea_get_dt(self._h, out))
+        return np.array(out[:], dtype=np.float32)
+
+    def energy(self) -> np.ndarray:
+        out = np.zeros(len(self.counts) * ENERGY_STRIDE, dtype=np.float64)
+        _check(_lib.astraea_energy(self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out.reshape(len(self.counts), ENERGY_STRIDE)
+
+    def counters(self) -> dict:
+        out = (C.c_uint64 * 6)()
+        _check(_lib.astraea_counters(self._h, out, 6))
+        keys = ["interactions", "forced_accepts", "nbr_entries", "nbr_capacity", "pass1_pairs", "pass2_pairs"]
+        return dict(zip(keys, [int(v) for v in out]))
+
+    def tree(self, seg: int = 0) -> dict:
+        n = self.counts[seg]
+        ni = max(n - 1, 0)
+        out = dict(keys=np.empty(n, np.uint64), perm=np.empty(n, np.uint32), left=np.empty(ni, np.int32),
+                   right=np.empty(ni, np.int32), parent_internal=np.empty(ni, np.int32),
+                   parent_leaf=np.empty(n, np.int32), node_cm=np.empty((ni, 4), np.float32),
+                   node_min=np.empty((ni, 4), np.float32), node_max=np.empty((ni, 4), np.float32))
+        order = ["keys", "perm", "left", "right", "parent_internal", "parent_leaf", "node_cm", "node_min", "node_max"]
+        _check(_lib.astraea_download_tree(self._h, seg, *[_ptr(out[k]) for k in order]))
+        return out
+
+    def enable_stage_timing(self, on: bool = True):
+        _check(_lib.astraea_enable_stage_timing(self._h, int(on)))
+
+    def stage_times(self) -> dict:
+        ms = (C.c_double * len(STAGES))()
+        n = (C.c_uint64 * len(STAGES))()
+        _check(_lib.astraea_stage_times(self._h, ms, n))
+        return {s: (ms[k], int(n[k])) for k, s in enumerate(STAGES)}
+
+
+def splitmix64(x: int) -> int:
+    return int(_lib.astraea_splitmix64(C.c_uint64(x & 0xFFFFFFFFFFFFFFFF)))
+
+
+def build_body(n: int, radius_m: float, layers, h_over_spacing=1.3, jitter=0.1, seed=42, flaws=False,
+               weibull_m=9.0, weibull_k=4.0e29, surface_temp_k=0.0) -> dict:
+    """A layered body on a jittered HCP lattice (astraea_build_body).
+
+    `layers`: [(material, mass_fraction, density_kgm3 or 0), ...] innermost first.
+    """
+    spec = BodySpec()
+    _lib.astraea_default_body_spec(C.byref(spec))
+    spec.radius_m = radius_m
+    spec.num_layers = len(layers)
+    for k, (mat, frac, dens) in enumerate(layers):
+        spec.layer_material[k] = mat
+        spec.layer_mass_frac[k] = frac
+        spec.layer_density_kgm3[k] = dens
+    spec.h_over_spacing = h_over_spacing
+    spec.jitter = jitter
+    spec.seed = seed
+    spec.strength_flaws = int(flaws)
+    spec.weibull_m = weibull_m
+    spec.weibull_k = weibull_k
+    spec.surface_temp_k = surface_temp_k
+    arrs = {k: np.zeros(n, np.float32) for k in ("x", "y", "z", "m", "h", "rho", "u", "eps_fail")}
+    arrs["mat"] = np.zeros(n, np.uint8)
+    out = BodyOut(*[arrs[k].ctypes.data_as(C.POINTER(C.c_float)) for k in ("x", "y", "z", "m", "h", "rho", "u", "eps_fail")],
+                  arrs["mat"].ctypes.data_as(C.POINTER(C.c_uint8)))
+    mass = C.c_double()
+    spacing = C.c_double()
+    _check(_lib.astraea_build_body(C.byref(spec), n, C.byref(out), C.byref(mass), C.byref(spacing)))
+    arrs["mass_kg"] = mass.value
+    arrs["spacing_m"] = spacing.value
+    return arrs
+
+
+def place_two_bodies(state: dict, n_t: int, n_i: int, r_t_psu: float, r_i_psu: float, v_ratio: float,
+                     b: float, sep_radii: float) -> float:
+    arrs = [state[k] for k in ("pos_x", "pos_y", "pos_z", "vel_x", "vel_y", "vel_z", "mass")]
+    for a in arrs:
+        assert a.dtype == np.float32 and a.flags.c_contiguous
+    vesc = C.c_double()
+    _check(_lib.astraea_place_two_bodies(n_t, n_i, *[_ptr(a) for a in arrs], r_t_psu, r_i_psu, v_ratio, b,
+                                         sep_radii, C.byref(vesc)))
+    return vesc.value
+
+
+from . import scenario  # noqa: E402,F401

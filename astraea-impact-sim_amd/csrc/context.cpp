@@ -1,0 +1,951 @@
+// context.cpp — the C ABI (include/astraea_hip.h): device memory, layout, field transfer,
+// stage calls, the fused KDK step with hipGraph replay, stage timing and reductions.
+//
+// The reference keeps this in Simulation::Impl (engine/simulation.cpp:345-536) and the launch
+// wrappers of engine/sph_gravity.cu:1986-2161; here one context owns one HIP stream and all
+// device memory, and the step never waits on the host (dt stays on the device).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/astraea_hip.h"
+#include "launch.hpp"
+
+using namespace astraea_hip;
+
+namespace {
+
+thread_local std::string gLastError;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    gLastError = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                  \
+    do {                                                                                               \
+        const hipError_t e__ = (expr);                                                                 \
+        if (e__ != hipSuccess) return fail(ASTRAEA_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e__)); \
+    } while (0)
+
+uint32_t roundUp(uint32_t v, uint32_t a) { return (v + a - 1) / a * a; }
+
+enum Kind { kF32, kU8, kU32, kF64 };
+
+struct FieldLoc {
+    int kind;        // element type on the host
+    int set;         // 0 persistent (cur), 1 derived
+    void** arr;      // address of the device pointer
+    int comps;       // components per device element (4 = float4, 2 = float2, 1 = scalar)
+    int comp;        // component index (for u8 inside meta: byte index)
+};
+
+}  // namespace
+
+struct astraea_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t capSlots = 0;
+    uint32_t maxSeg = 0;
+    astraea_params params{};
+    std::vector<uint32_t> counts, offs;
+    uint32_t numSeg = 0, slots = 0;
+    DeviceCtx dc{};
+    std::vector<void*> allocs;
+    double* dEnergy = nullptr;
+    bool loaded = false, treeValid = false, listValid = false, dtRawValid = false, posNewPending = false;
+    int parity = 0;  // number of cur/alt swaps mod 2
+    // graph
+    hipGraphExec_t graph = nullptr;
+    float graphMaxDt = -1.0f;
+    int graphParity = -1;
+    // timing: event pairs recorded on the context stream, read back lazily
+    bool timing = false;
+    static constexpr int kPool = 1024;
+    std::vector<hipEvent_t> evPool;
+    struct Rec { int stage, a, b; };
+    std::vector<Rec> pending;
+    int evNext = 0;
+    int openEv[ASTRAEA_STAGE_COUNT_] = {};
+    double stageMs[ASTRAEA_STAGE_COUNT_] = {};
+    uint64_t stageN[ASTRAEA_STAGE_COUNT_] = {};
+    // counters
+    uint64_t nbrCapacity = 0;
+};
+
+namespace {
+
+template <class T>
+int devAlloc(astraea_ctx* c, T*& p, size_t count) {
+    void* q = nullptr;
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    const hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return fail(ASTRAEA_E_HIP, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+    c->allocs.push_back(q);
+    p = static_cast<T*>(q);
+    return ASTRAEA_OK;
+}
+
+#define TRY(x)                      \
+    do {                            \
+        const int r__ = (x);        \
+        if (r__ != ASTRAEA_OK) return r__; \
+    } while (0)
+
+int allocPersist(astraea_ctx* c, Persist& p, uint32_t cap) {
+    TRY(devAlloc(c, p.pos, cap));
+    TRY(devAlloc(c, p.vel, cap));
+    TRY(devAlloc(c, p.hu, cap));
+    TRY(devAlloc(c, p.str0, cap));
+    TRY(devAlloc(c, p.str1, cap));
+    TRY(devAlloc(c, p.meta, cap));
+    TRY(devAlloc(c, p.oid, cap));
+    TRY(devAlloc(c, p.p64, 3 * (size_t)cap));
+    TRY(devAlloc(c, p.v64, 3 * (size_t)cap));
+    return ASTRAEA_OK;
+}
+
+void freeGraph(astraea_ctx* c) {
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    c->graphParity = -1;
+}
+
+int allocNeighbourList(astraea_ctx* c, uint64_t entries) {
+    if (c->dc.nb.list) {
+        (void)hipFree(c->dc.nb.list);
+        auto it = std::find(c->allocs.begin(), c->allocs.end(), (void*)c->dc.nb.list);
+        if (it != c->allocs.end()) c->allocs.erase(it);
+        c->dc.nb.list = nullptr;
+    }
+    TRY(devAlloc(c, c->dc.nb.list, entries));
+    c->dc.nb.capacity = entries;
+    c->nbrCapacity = entries;
+    freeGraph(c);
+    return ASTRAEA_OK;
+}
+
+Params toDeviceParams(const astraea_params& p) {
+    Params d;
+    const float th = std::max(0.0f, p.theta);
+    d.theta2 = th * th;
+    d.softFactor = p.soft_factor;
+    d.cfl = p.cfl_factor;
+    d.gammaHHe = p.gamma_hhe;
+    d.coolMinT = p.cooling_min_temp_k;
+    d.strengthMask = p.strength_mask & 0x1fu;
+    return d;
+}
+
+void syncDerivedFlags(astraea_ctx* c) {
+    c->dc.prm = toDeviceParams(c->params);
+    c->dc.fp64 = c->params.precision == ASTRAEA_PREC_FP64_ACCUM;
+    c->dc.strength = c->dc.prm.strengthMask != 0u;
+}
+
+bool locate(astraea_ctx* c, int f, FieldLoc& L) {
+    Persist& p = c->dc.cur;
+    Derived& d = c->dc.d;
+    auto F4 = [&](float4*& a, int comp, int set) { L = {kF32, set, (void**)&a, 4, comp}; };
+    switch (f) {
+        case ASTRAEA_F_POS_X: case ASTRAEA_F_POS_Y: case ASTRAEA_F_POS_Z: F4(p.pos, f - ASTRAEA_F_POS_X, 0); return true;
+        case ASTRAEA_F_MASS: F4(p.pos, 3, 0); return true;
+        case ASTRAEA_F_VEL_X: case ASTRAEA_F_VEL_Y: case ASTRAEA_F_VEL_Z: F4(p.vel, f - ASTRAEA_F_VEL_X, 0); return true;
+        case ASTRAEA_F_RHO: F4(p.vel, 3, 0); return true;
+        case ASTRAEA_F_H: F4(p.hu, 0, 0); return true;
+        case ASTRAEA_F_H0: F4(p.hu, 1, 0); return true;
+        case ASTRAEA_F_U: F4(p.hu, 2, 0); return true;
+        case ASTRAEA_F_DAMAGE: F4(p.hu, 3, 0); return true;
+        case ASTRAEA_F_SXX: case ASTRAEA_F_SYY: case ASTRAEA_F_SZZ: case ASTRAEA_F_SXY: F4(p.str0, f - ASTRAEA_F_SXX, 0); return true;
+        case ASTRAEA_F_SXZ: F4(p.str1, 0, 0); return true;
+        case ASTRAEA_F_SYZ: F4(p.str1, 1, 0); return true;
+        case ASTRAEA_F_DAMAGE_CBRT: F4(p.str1, 2, 0); return true;
+        case ASTRAEA_F_EPS_FAIL: F4(p.str1, 3, 0); return true;
+        case ASTRAEA_F_MATERIAL: L = {kU8, 0, (void**)&p.meta, 1, 0}; return true;
+        case ASTRAEA_F_BODY: L = {kU8, 0, (void**)&p.meta, 1, 1}; return true;
+        case ASTRAEA_F_TRACER: L = {kU8, 0, (void**)&p.meta, 1, 2}; return true;
+        case ASTRAEA_F_ORIGINAL_ID: L = {kU32, 0, (void**)&p.oid, 1, 0}; return true;
+        case ASTRAEA_F_POS64_X: case ASTRAEA_F_POS64_Y: case ASTRAEA_F_POS64_Z:
+            L = {kF64, 0, (void**)&p.p64, 1, f - ASTRAEA_F_POS64_X}; return true;
+        case ASTRAEA_F_VEL64_X: case ASTRAEA_F_VEL64_Y: case ASTRAEA_F_VEL64_Z:
+            L = {kF64, 0, (void**)&p.v64, 1, f - ASTRAEA_F_VEL64_X}; return true;
+        case ASTRAEA_F_ACC_X: case ASTRAEA_F_ACC_Y: case ASTRAEA_F_ACC_Z: F4(d.rate, f - ASTRAEA_F_ACC_X, 1); return true;
+        case ASTRAEA_F_DRHO_DT: F4(d.rate, 3, 1); return true;
+        case ASTRAEA_F_ACC_GRAV_X: case ASTRAEA_F_ACC_GRAV_Y: case ASTRAEA_F_ACC_GRAV_Z:
+            F4(d.grav, f - ASTRAEA_F_ACC_GRAV_X, 1); return true;
+        case ASTRAEA_F_PHI: F4(d.grav, 3, 1); return true;
+        case ASTRAEA_F_DU_DT: F4(d.rate2, 0, 1); return true;
+        case ASTRAEA_F_DDAMAGE_CBRT_DT: F4(d.rate2, 1, 1); return true;
+        case ASTRAEA_F_RAD_POWER_W: F4(d.rate2, 2, 1); return true;
+        case ASTRAEA_F_PRESSURE: F4(d.eos, 0, 1); return true;
+        case ASTRAEA_F_SOUND_SPEED: F4(d.eos, 1, 1); return true;
+        case ASTRAEA_F_TEMP_K: F4(d.eos, 2, 1); return true;
+        case ASTRAEA_F_SHEPARD: F4(d.p1a, 0, 1); return true;
+        case ASTRAEA_F_SURF_FACTOR: F4(d.p1a, 1, 1); return true;
+        case ASTRAEA_F_DIV_V: F4(d.p1a, 2, 1); return true;
+        case ASTRAEA_F_NGB_COUNT: L = {kU32, 1, (void**)&d.p1a, 4, 3}; return true;
+        case ASTRAEA_F_CURL_X: case ASTRAEA_F_CURL_Y: case ASTRAEA_F_CURL_Z: F4(d.p1b, f - ASTRAEA_F_CURL_X, 1); return true;
+        case ASTRAEA_F_DS_XX: case ASTRAEA_F_DS_YY: case ASTRAEA_F_DS_ZZ: case ASTRAEA_F_DS_XY:
+            F4(d.dS0, f - ASTRAEA_F_DS_XX, 1); return true;
+        case ASTRAEA_F_DS_XZ: L = {kF32, 1, (void**)&d.dS1, 2, 0}; return true;
+        case ASTRAEA_F_DS_YZ: L = {kF32, 1, (void**)&d.dS1, 2, 1}; return true;
+        case ASTRAEA_F_PHASE: L = {kU8, 1, (void**)&d.phase, 0, 0}; return true;
+        default: return false;
+    }
+}
+
+// Device element bytes of an array described by a FieldLoc.
+size_t elemBytes(const FieldLoc& L) {
+    if (L.kind == kF64) return sizeof(double);
+    if (L.arr == nullptr) return 0;
+    if (L.kind == kU8 && L.comps == 0) return 1;   // phase: plain u8 array
+    if (L.kind == kU8) return 4;                   // meta: u32
+    if (L.kind == kU32 && L.comps == 1) return 4;
+    return (size_t)L.comps * 4;
+}
+
+// Copy a whole padded device array to the host (or back).
+int copyArray(astraea_ctx* c, void* dev, std::vector<uint8_t>& host, size_t bytesPerElem, bool toHost) {
+    const size_t bytes = bytesPerElem * (size_t)c->slots;
+    if (toHost) {
+        host.resize(bytes);
+        HIP_TRY(hipMemcpyAsync(host.data(), dev, bytes, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    } else {
+        HIP_TRY(hipMemcpyAsync(dev, host.data(), bytes, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    return ASTRAEA_OK;
+}
+
+// Run `fn(hostIndex, slot)` over every particle.
+template <class F>
+void forEachParticle(const astraea_ctx* c, F&& fn) {
+    size_t h = 0;
+    for (uint32_t s = 0; s < c->numSeg; ++s)
+        for (uint32_t k = 0; k < c->counts[s]; ++k) fn(h++, (size_t)c->offs[s] + k);
+}
+
+int transferField(astraea_ctx* c, int f, void* hostBuf, bool upload) {
+    FieldLoc L;
+    if (!locate(c, f, L)) return fail(ASTRAEA_E_INVALID, "unknown field id %d", f);
+    if (upload && L.set != 0) return fail(ASTRAEA_E_INVALID, "field %d is derived and cannot be uploaded", f);
+    if (!hostBuf) return fail(ASTRAEA_E_INVALID, "null host buffer");
+    // A drift leaves the new positions in posNew until the next sort.
+    const bool posPending = !upload && c->posNewPending && (f >= ASTRAEA_F_POS_X && f <= ASTRAEA_F_POS_Z);
+    if (L.kind == kF64) {
+        double* base = (double*)(*L.arr) + (size_t)L.comp * c->capSlots;
+        std::vector<uint8_t> buf;
+        TRY(copyArray(c, base, buf, sizeof(double), true));
+        double* d = (double*)buf.data();
+        if (upload) {
+            forEachParticle(c, [&](size_t h, size_t s) { d[s] = ((const double*)hostBuf)[h]; });
+            TRY(copyArray(c, base, buf, sizeof(double), false));
+            // keep the FP32 copy in sync
+            const int f32 = (f <= ASTRAEA_F_POS64_Z) ? ASTRAEA_F_POS_X + (f - ASTRAEA_F_POS64_X) : ASTRAEA_F_VEL_X + (f - ASTRAEA_F_VEL64_X);
+            std::vector<float> hf;
+            size_t n = 0;
+            for (uint32_t s = 0; s < c->numSeg; ++s) n += c->counts[s];
+            hf.resize(n);
+            for (size_t k = 0; k < n; ++k) hf[k] = (float)((const double*)hostBuf)[k];
+            FieldLoc L2;
+            locate(c, f32, L2);
+            std::vector<uint8_t> b2;
+            TRY(copyArray(c, *L2.arr, b2, 16, true));
+            float4* q = (float4*)b2.data();
+            forEachParticle(c, [&](size_t h, size_t s) { ((float*)&q[s])[L2.comp] = hf[h]; });
+            TRY(copyArray(c, *L2.arr, b2, 16, false));
+        } else {
+            forEachParticle(c, [&](size_t h, size_t s) { ((double*)hostBuf)[h] = d[s]; });
+        }
+        return ASTRAEA_OK;
+    }
+    void* dev = posPending ? (void*)c->dc.d.posNew : *L.arr;
+    const size_t eb = elemBytes(L);
+    std::vector<uint8_t> buf;
+    TRY(copyArray(c, dev, buf, eb, true));
+    uint8_t* raw = buf.data();
+    if (L.kind == kF32) {
+        forEachParticle(c, [&](size_t h, size_t s) {
+            float* e = (float*)(raw + s * eb) + L.comp;
+            if (upload) *e = ((const float*)hostBuf)[h]; else ((float*)hostBuf)[h] = *e;
+        });
+    } else if (L.kind == kU8 && L.comps == 0) {
+        forEachParticle(c, [&](size_t h, size_t s) {
+            if (upload) raw[s] = ((const uint8_t*)hostBuf)[h]; else ((uint8_t*)hostBuf)[h] = raw[s];
+        });
+    } else if (L.kind == kU8) {
+        forEachParticle(c, [&](size_t h, size_t s) {
+            uint32_t* m = (uint32_t*)(raw + s * 4);
+            if (upload) {
+                const uint32_t v = ((const uint8_t*)hostBuf)[h];
+                *m = (*m & ~(0xffu << (8 * L.comp))) | (v << (8 * L.comp));
+            } else {
+                ((uint8_t*)hostBuf)[h] = (uint8_t)((*m >> (8 * L.comp)) & 0xffu);
+            }
+        });
+    } else {  // kU32: original id, or a float count stored in a float4
+        forEachParticle(c, [&](size_t h, size_t s) {
+            if (L.comps == 1) {
+                uint32_t* e = (uint32_t*)(raw + s * 4);
+                if (upload) *e = ((const uint32_t*)hostBuf)[h]; else ((uint32_t*)hostBuf)[h] = *e;
+            } else {
+                const float v = ((float*)(raw + s * eb))[L.comp];
+                ((uint32_t*)hostBuf)[h] = (uint32_t)v;
+            }
+        });
+    }
+    if (upload) {
+        TRY(copyArray(c, dev, buf, eb, false));
+        if (c->dc.fp64 && ((f >= ASTRAEA_F_POS_X && f <= ASTRAEA_F_POS_Z) || (f >= ASTRAEA_F_VEL_X && f <= ASTRAEA_F_VEL_Z))) {
+            // the FP64 masters follow an FP32 upload
+            const bool isPos = f <= ASTRAEA_F_POS_Z;
+            const int comp = isPos ? f - ASTRAEA_F_POS_X : f - ASTRAEA_F_VEL_X;
+            double* base = (isPos ? c->dc.cur.p64 : c->dc.cur.v64) + (size_t)comp * c->capSlots;
+            std::vector<uint8_t> b64;
+            TRY(copyArray(c, base, b64, sizeof(double), true));
+            double* d = (double*)b64.data();
+            forEachParticle(c, [&](size_t h, size_t s) { d[s] = (double)((const float*)hostBuf)[h]; });
+            TRY(copyArray(c, base, b64, sizeof(double), false));
+        }
+        c->treeValid = c->listValid = c->dtRawValid = false;
+        c->posNewPending = false;
+    }
+    return ASTRAEA_OK;
+}
+
+// ---- timing helpers: start/end events on the context stream, no host wait ----
+void collectTimes(astraea_ctx* c) {
+    if (c->pending.empty()) return;
+    (void)hipStreamSynchronize(c->stream);
+    for (const auto& r : c->pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, c->evPool[r.a], c->evPool[r.b]) == hipSuccess) {
+            c->stageMs[r.stage] += ms;
+            c->stageN[r.stage] += 1;
+        }
+    }
+    c->pending.clear();
+    c->evNext = 0;
+}
+int takeEvent(astraea_ctx* c) {
+    if (c->evNext >= (int)c->evPool.size()) collectTimes(c);
+    return c->evNext++;
+}
+void stageMark(astraea_ctx* c, int stage) {
+    if (!c->timing || c->evPool.empty()) return;
+    const int e = takeEvent(c);
+    (void)hipEventRecord(c->evPool[e], c->stream);
+    c->openEv[stage] = e;
+}
+void stageClose(astraea_ctx* c, int stage) {
+    if (!c->timing || c->evPool.empty()) return;
+    const int e = takeEvent(c);
+    (void)hipEventRecord(c->evPool[e], c->stream);
+    c->pending.push_back({stage, c->openEv[stage], e});
+}
+
+void swapSets(astraea_ctx* c) {
+    std::swap(c->dc.cur, c->dc.alt);
+    c->parity ^= 1;
+}
+
+// Sort -> permute/Karras -> moments -> neighbour table.  posNew and the bounds must be ready.
+void enqueueTree(astraea_ctx* c, bool timed) {
+    hipStream_t s = c->stream;
+    if (timed) stageMark(c, ASTRAEA_STAGE_MORTON);
+    launchMorton(c->dc, s);
+    if (timed) { stageClose(c, ASTRAEA_STAGE_MORTON); stageMark(c, ASTRAEA_STAGE_SORT); }
+    sortPairs(c->dc, s);
+    if (timed) { stageClose(c, ASTRAEA_STAGE_SORT); stageMark(c, ASTRAEA_STAGE_PERMUTE_TREE); }
+    launchPermuteKarras(c->dc, s);
+    swapSets(c);
+    if (timed) { stageClose(c, ASTRAEA_STAGE_PERMUTE_TREE); stageMark(c, ASTRAEA_STAGE_MOMENTS); }
+    launchMoments(c->dc, s);
+    if (timed) { stageClose(c, ASTRAEA_STAGE_MOMENTS); stageMark(c, ASTRAEA_STAGE_NEIGHBOURS); }
+    launchNeighbours(c->dc, s);
+    if (timed) stageClose(c, ASTRAEA_STAGE_NEIGHBOURS);
+    c->treeValid = c->listValid = true;
+    c->posNewPending = false;
+}
+
+// One full KDK step (Simulation::step, engine/simulation.cpp:1721-1732), fully asynchronous.
+void enqueueStep(astraea_ctx* c, float maxDt, bool timed) {
+    hipStream_t s = c->stream;
+    if (timed) stageMark(c, ASTRAEA_STAGE_DT);
+    launchStepBegin(c->dc, s, maxDt);
+    if (timed) { stageClose(c, ASTRAEA_STAGE_DT); stageMark(c, ASTRAEA_STAGE_KICK1); }
+    launchKick(c->dc, s, false, false);
+    if (timed) { stageClose(c, ASTRAEA_STAGE_KICK1); stageMark(c, ASTRAEA_STAGE_DRIFT); }
+    launchDrift(c->dc, s);
+    if (timed) stageClose(c, ASTRAEA_STAGE_DRIFT);
+    enqueueTree(c, timed);
+    if (timed) stageMark(c, ASTRAEA_STAGE_PASS1);
+    launchPass1(c->dc, s);
+    if (timed) { stageClose(c, ASTRAEA_STAGE_PASS1); stageMark(c, ASTRAEA_STAGE_GRAVITY); }
+    launchGravity(c->dc, s);
+    if (timed) { stageClose(c, ASTRAEA_STAGE_GRAVITY); stageMark(c, ASTRAEA_STAGE_PASS2); }
+    launchPass2(c->dc, s);
+    if (timed) { stageClose(c, ASTRAEA_STAGE_PASS2); stageMark(c, ASTRAEA_STAGE_KICK2); }
+    launchKick(c->dc, s, true, false);
+    launchStepEnd(c->dc, s);
+    if (timed) stageClose(c, ASTRAEA_STAGE_KICK2);
+    c->dtRawValid = true;
+}
+
+int requireLoaded(astraea_ctx* c) {
+    if (!c) return fail(ASTRAEA_E_INVALID, "null context");
+    if (!c->loaded) return fail(ASTRAEA_E_STATE, "no particle layout: call astraea_set_layout first");
+    return ASTRAEA_OK;
+}
+
+int checkLaunch(astraea_ctx* c) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(ASTRAEA_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+    (void)c;
+    return ASTRAEA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t astraea_field_size(int f) {
+    if (f < 0 || f >= ASTRAEA_F_COUNT_) return 0;
+    if (f == ASTRAEA_F_MATERIAL || f == ASTRAEA_F_BODY || f == ASTRAEA_F_TRACER || f == ASTRAEA_F_PHASE) return 1;
+    if (f >= ASTRAEA_F_POS64_X && f <= ASTRAEA_F_VEL64_Z) return 8;
+    return 4;
+}
+
+const char* astraea_last_error(void) { return gLastError.c_str(); }
+int astraea_abi_version(void) { return ASTRAEA_ABI_VERSION; }
+
+void astraea_default_params(astraea_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->theta = 0.5f;
+    p->soft_factor = 0.8f;
+    p->cfl_factor = 0.25f;
+    p->gamma_hhe = 1.4f;
+    p->cooling_min_temp_k = 0.0f;
+    p->strength_mask = 0x17u;
+    p->precision = ASTRAEA_PREC_FP32;
+}
+
+int astraea_create(astraea_ctx** out, int device, uint32_t capacity, uint32_t maxSegments) {
+    if (!out) return fail(ASTRAEA_E_INVALID, "null output pointer");
+    *out = nullptr;
+    if (capacity == 0) return fail(ASTRAEA_E_INVALID, "capacity must be > 0");
+    if (maxSegments == 0 || maxSegments > 256) return fail(ASTRAEA_E_INVALID, "max_segments must be in [1, 256]");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(ASTRAEA_E_INVALID, "device %d out of range (%d devices)", device, ndev);
+    HIP_TRY(hipSetDevice(device));
+    auto* c = new astraea_ctx();
+    c->device = device;
+    c->capSlots = roundUp(capacity, ASTRAEA_SEGMENT_ALIGN) + ASTRAEA_SEGMENT_ALIGN * maxSegments;
+    c->maxSeg = maxSegments;
+    astraea_default_params(&c->params);
+    syncDerivedFlags(c);
+    auto cleanup = [&](int r) {
+        astraea_destroy(c);
+        return r;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(ASTRAEA_E_HIP, "hipStreamCreate failed"));
+    const uint32_t C = c->capSlots;
+    DeviceCtx& d = c->dc;
+    int r = ASTRAEA_OK;
+    auto A = [&](int x) { if (r == ASTRAEA_OK) r = x; };
+    A(allocPersist(c, d.cur, C));
+    A(allocPersist(c, d.alt, C));
+    A(devAlloc(c, d.d.posNew, C));
+    A(devAlloc(c, d.d.grav, C));
+    A(devAlloc(c, d.d.rate, C));
+    A(devAlloc(c, d.d.rate2, C));
+    A(devAlloc(c, d.d.dS0, C));
+    A(devAlloc(c, d.d.dS1, C));
+    A(devAlloc(c, d.d.eos, C));
+    A(devAlloc(c, d.d.phase, C));
+    A(devAlloc(c, d.d.p1a, C));
+    A(devAlloc(c, d.d.p1b, C));
+    A(devAlloc(c, d.d.nb, C));
+    A(devAlloc(c, d.d.sr0, C));
+    A(devAlloc(c, d.d.sr1, C));
+    A(devAlloc(c, d.t.child, C));
+    A(devAlloc(c, d.t.parentI, C));
+    A(devAlloc(c, d.t.parentL, C));
+    A(devAlloc(c, d.t.visited, C));
+    A(devAlloc(c, d.t.cm, C));
+    A(devAlloc(c, d.t.bmin, C));
+    A(devAlloc(c, d.t.bmax, C));
+    A(devAlloc(c, d.nb.count, C));
+    A(devAlloc(c, d.nb.waveMax, C / kWave));
+    A(devAlloc(c, d.nb.waveOff, C / kWave));
+    A(devAlloc(c, d.seg.off, maxSegments + 1));
+    A(devAlloc(c, d.seg.n, maxSegments));
+    A(devAlloc(c, d.seg.slotSeg, C));
+    A(devAlloc(c, d.ss.boundsOrd, 6 * maxSegments));
+    A(devAlloc(c, d.ss.dtRawBits, maxSegments));
+    A(devAlloc(c, d.ss.dt, maxSegments));
+    A(devAlloc(c, d.ss.dtPrev, maxSegments));
+    A(devAlloc(c, d.ss.radSum, maxSegments));
+    A(devAlloc(c, d.ss.radPrev, maxSegments));
+    A(devAlloc(c, d.ss.erad, maxSegments));
+    A(devAlloc(c, d.ss.simTime, maxSegments));
+    A(devAlloc(c, d.ss.living, maxSegments));
+    A(devAlloc(c, d.cnt.v, 8));
+    const uint32_t tiles = (C + 4095) / 4096;
+    A(devAlloc(c, d.sb.keyA, C));
+    A(devAlloc(c, d.sb.keyB, C));
+    A(devAlloc(c, d.sb.valA, C));
+    A(devAlloc(c, d.sb.valB, C));
+    A(devAlloc(c, d.sb.counts, 256 * (size_t)tiles));
+    A(devAlloc(c, d.sb.tilePrefix, 256 * (size_t)tiles));
+    A(devAlloc(c, d.sb.digitTotal, 256));
+    d.sb.tilesCap = tiles;
+    A(devAlloc(c, c->dEnergy, (size_t)ASTRAEA_ENERGY_STRIDE * maxSegments));
+    if (r == ASTRAEA_OK) r = allocNeighbourList(c, (uint64_t)C * 256ull);
+    if (r != ASTRAEA_OK) return cleanup(r);
+    d.stride64 = C;
+    if (hipMemsetAsync(d.cnt.v, 0, 8 * sizeof(unsigned long long), c->stream) != hipSuccess)
+        return cleanup(fail(ASTRAEA_E_HIP, "hipMemset failed"));
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return cleanup(fail(ASTRAEA_E_HIP, "stream sync failed"));
+    *out = c;
+    return ASTRAEA_OK;
+}
+
+int astraea_destroy(astraea_ctx* c) {
+    if (!c) return ASTRAEA_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    freeGraph(c);
+    for (void* p : c->allocs) (void)hipFree(p);
+    for (hipEvent_t e : c->evPool) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return ASTRAEA_OK;
+}
+
+void* astraea_stream(astraea_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int astraea_set_params(astraea_ctx* c, const astraea_params* p) {
+    if (!c || !p) return fail(ASTRAEA_E_INVALID, "null argument");
+    if (!(p->theta >= 0.0f) || !(p->soft_factor >= 0.0f) || !(p->cfl_factor > 0.0f))
+        return fail(ASTRAEA_E_INVALID, "theta, soft_factor must be >= 0 and cfl_factor > 0");
+    if (p->precision != ASTRAEA_PREC_FP32 && p->precision != ASTRAEA_PREC_FP64_ACCUM)
+        return fail(ASTRAEA_E_INVALID, "unknown precision mode %u", p->precision);
+    const bool wasFp64 = c->params.precision == ASTRAEA_PREC_FP64_ACCUM;
+    c->params = *p;
+    syncDerivedFlags(c);
+    freeGraph(c);
+    if (c->loaded && !wasFp64 && c->dc.fp64) {
+        // seed the FP64 masters from the FP32 state
+        for (int comp = 0; comp < 3; ++comp) {
+            const int fp = ASTRAEA_F_POS_X + comp, fv = ASTRAEA_F_VEL_X + comp;
+            size_t n = 0;
+            for (uint32_t s = 0; s < c->numSeg; ++s) n += c->counts[s];
+            std::vector<float> tmp(n);
+            std::vector<double> t64(n);
+            TRY(transferField(c, fp, tmp.data(), false));
+            for (size_t k = 0; k < n; ++k) t64[k] = tmp[k];
+            TRY(transferField(c, ASTRAEA_F_POS64_X + comp, t64.data(), true));
+            TRY(transferField(c, fv, tmp.data(), false));
+            for (size_t k = 0; k < n; ++k) t64[k] = tmp[k];
+            TRY(transferField(c, ASTRAEA_F_VEL64_X + comp, t64.data(), true));
+        }
+    }
+    return ASTRAEA_OK;
+}
+
+int astraea_get_params(astraea_ctx* c, astraea_params* out) {
+    if (!c || !out) return fail(ASTRAEA_E_INVALID, "null argument");
+    *out = c->params;
+    return ASTRAEA_OK;
+}
+
+int astraea_set_layout(astraea_ctx* c, uint32_t ns, const uint32_t* counts) {
+    if (!c) return fail(ASTRAEA_E_INVALID, "null context");
+    if (ns == 0 || ns > c->maxSeg || !counts) return fail(ASTRAEA_E_INVALID, "num_segments must be in [1, %u]", c->maxSeg);
+    uint64_t total = 0, slots = 0;
+    std::vector<uint32_t> offs(ns + 1, 0);
+    for (uint32_t s = 0; s < ns; ++s) {
+        offs[s] = (uint32_t)slots;
+        total += counts[s];
+        slots += roundUp(counts[s], ASTRAEA_SEGMENT_ALIGN);
+    }
+    offs[ns] = (uint32_t)slots;
+    if (total == 0) return fail(ASTRAEA_E_INVALID, "the layout holds no particles (reference: totalParticles == 0)");
+    if (slots > c->capSlots)
+        return fail(ASTRAEA_E_CAPACITY, "layout needs %llu slots, capacity is %u", (unsigned long long)slots, c->capSlots);
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    freeGraph(c);
+    c->numSeg = ns;
+    c->counts.assign(counts, counts + ns);
+    c->offs = offs;
+    c->slots = (uint32_t)std::max<uint64_t>(slots, ASTRAEA_SEGMENT_ALIGN);
+    DeviceCtx& d = c->dc;
+    d.slots = c->slots;
+    d.blocks = c->slots / kBlock;
+    d.seg.count = ns;
+    std::vector<uint32_t> slotSeg(c->slots, 0);
+    for (uint32_t s = 0; s < ns; ++s)
+        for (uint32_t k = offs[s]; k < offs[s + 1]; ++k) slotSeg[k] = s;
+    HIP_TRY(hipMemcpy(d.seg.off, offs.data(), sizeof(uint32_t) * (ns + 1), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.seg.n, counts, sizeof(uint32_t) * ns, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.seg.slotSeg, slotSeg.data(), sizeof(uint32_t) * c->slots, hipMemcpyHostToDevice));
+    // state as uploadHostStateToDevice leaves it (simulation.cpp:469-520): zeros, eps_fail = 1,
+    // original id = index in the concatenation
+    for (Persist* p : {&d.cur, &d.alt}) {
+        HIP_TRY(hipMemset(p->pos, 0, sizeof(float4) * c->slots));
+        HIP_TRY(hipMemset(p->vel, 0, sizeof(float4) * c->slots));
+        HIP_TRY(hipMemset(p->hu, 0, sizeof(float4) * c->slots));
+        HIP_TRY(hipMemset(p->str0, 0, sizeof(float4) * c->slots));
+        HIP_TRY(hipMemset(p->meta, 0, sizeof(uint32_t) * c->slots));
+        HIP_TRY(hipMemset(p->p64, 0, sizeof(double) * 3 * (size_t)c->capSlots));
+        HIP_TRY(hipMemset(p->v64, 0, sizeof(double) * 3 * (size_t)c->capSlots));
+    }
+    std::vector<float4> s1(c->slots, make_float4(0.0f, 0.0f, 0.0f, 1.0f));
+    HIP_TRY(hipMemcpy(d.cur.str1, s1.data(), sizeof(float4) * c->slots, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.alt.str1, s1.data(), sizeof(float4) * c->slots, hipMemcpyHostToDevice));
+    std::vector<uint32_t> oid(c->slots, 0xffffffffu);
+    {
+        uint32_t h = 0;
+        for (uint32_t s = 0; s < ns; ++s)
+            for (uint32_t k = 0; k < counts[s]; ++k) oid[offs[s] + k] = h++;
+    }
+    HIP_TRY(hipMemcpy(d.cur.oid, oid.data(), sizeof(uint32_t) * c->slots, hipMemcpyHostToDevice));
+    for (float4* a : {d.d.posNew, d.d.grav, d.d.rate, d.d.rate2, d.d.dS0, d.d.eos, d.d.p1a, d.d.p1b, d.d.nb, d.d.sr0})
+        HIP_TRY(hipMemset(a, 0, sizeof(float4) * c->slots));
+    HIP_TRY(hipMemset(d.d.dS1, 0, sizeof(float2) * c->slots));
+    HIP_TRY(hipMemset(d.d.sr1, 0, sizeof(float2) * c->slots));
+    HIP_TRY(hipMemset(d.d.phase, 0, c->slots));
+    HIP_TRY(hipMemset(d.nb.count, 0, sizeof(uint32_t) * c->slots));
+    HIP_TRY(hipMemset(d.nb.waveMax, 0, sizeof(uint32_t) * (c->slots / kWave)));
+    HIP_TRY(hipMemset(d.nb.waveOff, 0, sizeof(uint64_t) * (c->slots / kWave)));
+    HIP_TRY(hipMemset(d.ss.dtPrev, 0, sizeof(float) * ns));
+    HIP_TRY(hipMemset(d.ss.dt, 0, sizeof(float) * ns));
+    HIP_TRY(hipMemset(d.ss.erad, 0, sizeof(double) * ns));
+    HIP_TRY(hipMemset(d.ss.simTime, 0, sizeof(double) * ns));
+    HIP_TRY(hipMemset(d.ss.radPrev, 0, sizeof(double) * ns));
+    HIP_TRY(hipMemset(d.ss.radSum, 0, sizeof(double) * ns));
+    HIP_TRY(hipMemset(d.ss.living, 0, sizeof(uint32_t) * ns));
+    HIP_TRY(hipMemset(d.ss.dtRawBits, 0x7f, sizeof(uint32_t) * ns));
+    HIP_TRY(hipMemset(d.cnt.v, 0, 8 * sizeof(unsigned long long)));
+    HIP_TRY(hipDeviceSynchronize());
+    c->loaded = true;
+    c->treeValid = c->listValid = c->dtRawValid = c->posNewPending = false;
+    return ASTRAEA_OK;
+}
+
+int astraea_num_particles(astraea_ctx* c, uint32_t* out) {
+    if (!c || !out) return fail(ASTRAEA_E_INVALID, "null argument");
+    uint32_t n = 0;
+    for (uint32_t v : c->counts) n += v;
+    *out = n;
+    return ASTRAEA_OK;
+}
+
+int astraea_upload_field(astraea_ctx* c, int f, const void* src) {
+    TRY(requireLoaded(c));
+    HIP_TRY(hipSetDevice(c->device));
+    freeGraph(c);
+    return transferField(c, f, const_cast<void*>(src), true);
+}
+
+int astraea_download_field(astraea_ctx* c, int f, void* dst) {
+    TRY(requireLoaded(c));
+    HIP_TRY(hipSetDevice(c->device));
+    return transferField(c, f, dst, false);
+}
+
+int astraea_clear_stats(astraea_ctx* c) {
+    TRY(requireLoaded(c));
+    launchClear(c->dc, c->stream);
+    c->dtRawValid = false;
+    return checkLaunch(c);
+}
+
+int astraea_sort_and_build_bvh(astraea_ctx* c) {
+    TRY(requireLoaded(c));
+    if (!c->posNewPending) {
+        launchResetBounds(c->dc, c->stream);
+        launchPrepareSort(c->dc, c->stream);
+    }
+    enqueueTree(c, false);
+    c->dtRawValid = false;
+    return checkLaunch(c);
+}
+
+int astraea_sph_pass1(astraea_ctx* c) {
+    TRY(requireLoaded(c));
+    if (!c->listValid) return fail(ASTRAEA_E_STATE, "pass 1 needs a tree: call astraea_sort_and_build_bvh");
+    launchPass1(c->dc, c->stream);
+    c->dtRawValid = false;
+    return checkLaunch(c);
+}
+
+int astraea_evaluate_eos(astraea_ctx* c) {
+    TRY(requireLoaded(c));
+    launchEos(c->dc, c->stream);
+    c->dtRawValid = false;
+    return checkLaunch(c);
+}
+
+int astraea_gravity(astraea_ctx* c) {
+    TRY(requireLoaded(c));
+    if (!c->treeValid) return fail(ASTRAEA_E_STATE, "gravity needs a tree: call astraea_sort_and_build_bvh");
+    launchGravity(c->dc, c->stream);
+    c->dtRawValid = false;
+    return checkLaunch(c);
+}
+
+int astraea_sph_pass2(astraea_ctx* c) {
+    TRY(requireLoaded(c));
+    if (!c->listValid) return fail(ASTRAEA_E_STATE, "pass 2 needs a tree: call astraea_sort_and_build_bvh");
+    launchPass2(c->dc, c->stream);
+    c->dtRawValid = false;
+    return checkLaunch(c);
+}
+
+int astraea_reduce_dt(astraea_ctx* c, float maxDt, float* out) {
+    TRY(requireLoaded(c));
+    if (!(maxDt > 0.0f)) return fail(ASTRAEA_E_INVALID, "max_dt must be > 0");
+    launchDtCandidates(c->dc, c->stream);
+    launchStepBeginNoBounds(c->dc, c->stream, maxDt);
+    TRY(checkLaunch(c));
+    if (out) {
+        HIP_TRY(hipMemcpyAsync(out, c->dc.ss.dt, sizeof(float) * c->numSeg, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    return ASTRAEA_OK;
+}
+
+int astraea_kick_half(astraea_ctx* c, int clamp) {
+    TRY(requireLoaded(c));
+    launchKick(c->dc, c->stream, false, clamp != 0);
+    c->dtRawValid = false;
+    return checkLaunch(c);
+}
+
+int astraea_drift(astraea_ctx* c) {
+    TRY(requireLoaded(c));
+    if (c->posNewPending) return fail(ASTRAEA_E_STATE, "a drift must be followed by astraea_sort_and_build_bvh");
+    launchResetBounds(c->dc, c->stream);
+    launchDrift(c->dc, c->stream);
+    c->posNewPending = true;
+    c->dtRawValid = false;
+    return checkLaunch(c);
+}
+
+int astraea_evaluate_initial_forces(astraea_ctx* c) {
+    TRY(requireLoaded(c));
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    launchClear(c->dc, s);
+    launchResetBounds(c->dc, s);
+    launchPrepareSort(c->dc, s);
+    enqueueTree(c, false);
+    launchPass1(c->dc, s);
+    launchGravity(c->dc, s);
+    launchPass2(c->dc, s);
+    launchDtCandidates(c->dc, s);
+    launchInitIntegrator(c->dc, s);
+    c->dtRawValid = true;
+    return checkLaunch(c);
+}
+
+int astraea_step(astraea_ctx* c, float maxDt, uint32_t steps, int useGraph) {
+    TRY(requireLoaded(c));
+    if (!(maxDt > 0.0f)) return fail(ASTRAEA_E_INVALID, "max_dt must be > 0");
+    if (!c->listValid || c->posNewPending)
+        return fail(ASTRAEA_E_STATE, "step needs evaluated forces: call astraea_evaluate_initial_forces");
+    HIP_TRY(hipSetDevice(c->device));
+    if (!c->dtRawValid) {
+        launchDtCandidates(c->dc, c->stream);
+        c->dtRawValid = true;
+    }
+    uint32_t left = steps;
+    if (useGraph && !c->timing) {
+        if (c->graph && (c->graphMaxDt != maxDt || c->graphParity != c->parity)) freeGraph(c);
+        if (left >= 2 && !c->graph) {
+            hipGraph_t g = nullptr;
+            HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            const int parity0 = c->parity;
+            enqueueStep(c, maxDt, false);
+            enqueueStep(c, maxDt, false);
+            const hipError_t e = hipStreamEndCapture(c->stream, &g);
+            if (e != hipSuccess) return fail(ASTRAEA_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+            const hipError_t ei = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) return fail(ASTRAEA_E_HIP, "graph instantiate failed: %s", hipGetErrorString(ei));
+            c->graphMaxDt = maxDt;
+            c->graphParity = parity0;
+            // the capture recorded two steps without running them; the two swaps restored cur/alt
+            c->parity = parity0;
+        }
+        while (left >= 2 && c->graph) {
+            HIP_TRY(hipGraphLaunch(c->graph, c->stream));
+            left -= 2;
+        }
+    }
+    while (left > 0) {
+        enqueueStep(c, maxDt, c->timing);
+        --left;
+    }
+    c->dtRawValid = true;
+    return checkLaunch(c);
+}
+
+int astraea_synchronize(astraea_ctx* c) {
+    if (!c) return fail(ASTRAEA_E_INVALID, "null context");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (!c->loaded) return ASTRAEA_OK;
+    unsigned long long v[8];
+    HIP_TRY(hipMemcpy(v, c->dc.cnt.v, sizeof(v), hipMemcpyDeviceToHost));
+    if (v[3] != 0ull || v[6] != 0ull) {
+        const unsigned long long need = v[2];
+        // clear the sticky flags and grow the table for the following steps
+        HIP_TRY(hipMemset(c->dc.cnt.v + 3, 0, sizeof(unsigned long long)));
+        HIP_TRY(hipMemset(c->dc.cnt.v + 6, 0, sizeof(unsigned long long)));
+        if (v[3] != 0ull) {
+            const uint64_t grow = std::max<uint64_t>(need + need / 4, 2 * c->nbrCapacity);
+            TRY(allocNeighbourList(c, grow));
+            c->listValid = false;
+            return fail(ASTRAEA_E_OVERFLOW,
+                        "neighbour table overflow: %llu entries needed, capacity was %llu (grown to %llu; the "
+                        "affected step is invalid)", need, (unsigned long long)c->nbrCapacity, (unsigned long long)grow);
+        }
+        return fail(ASTRAEA_E_OVERFLOW, "neighbour traversal stack overflow (depth %d)", ASTRAEA_STACK_DEPTH);
+    }
+    return ASTRAEA_OK;
+}
+
+int astraea_get_dt(astraea_ctx* c, float* out) {
+    TRY(requireLoaded(c));
+    if (!out) return fail(ASTRAEA_E_INVALID, "null output");
+    HIP_TRY(hipMemcpyAsync(out, c->dc.ss.dt, sizeof(float) * c->numSeg, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return ASTRAEA_OK;
+}
+
+int astraea_energy(astraea_ctx* c, double* out) {
+    TRY(requireLoaded(c));
+    if (!out) return fail(ASTRAEA_E_INVALID, "null output");
+    HIP_TRY(hipSetDevice(c->device));
+    launchEnergy(c->dc, c->stream, c->dEnergy);
+    TRY(checkLaunch(c));
+    const uint32_t S = c->numSeg;
+    std::vector<double> erad(S), t(S);
+    HIP_TRY(hipMemcpyAsync(out, c->dEnergy, sizeof(double) * ASTRAEA_ENERGY_STRIDE * S, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(erad.data(), c->dc.ss.erad, sizeof(double) * S, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(t.data(), c->dc.ss.simTime, sizeof(double) * S, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (uint32_t s = 0; s < S; ++s) {
+        double* o = out + (size_t)s * ASTRAEA_ENERGY_STRIDE;
+        const double m = o[ASTRAEA_EN_MASS];
+        if (m > 0.0)
+            for (int k = ASTRAEA_EN_COM_X; k <= ASTRAEA_EN_COM_Z; ++k) o[k] /= m;
+        o[ASTRAEA_EN_RADIATED] = erad[s];
+        o[ASTRAEA_EN_SIM_TIME] = t[s];
+    }
+    return ASTRAEA_OK;
+}
+
+int astraea_counters(astraea_ctx* c, uint64_t* out, int n) {
+    TRY(requireLoaded(c));
+    if (!out || n <= 0) return fail(ASTRAEA_E_INVALID, "null output");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    unsigned long long v[8];
+    HIP_TRY(hipMemcpy(v, c->dc.cnt.v, sizeof(v), hipMemcpyDeviceToHost));
+    uint64_t vals[6] = {v[0], v[1], v[2], c->nbrCapacity, 0, 0};
+    if (n > 4) {
+        std::vector<float4> a(c->slots), b(c->slots);
+        HIP_TRY(hipMemcpy(a.data(), c->dc.d.p1a, sizeof(float4) * c->slots, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(b.data(), c->dc.d.rate2, sizeof(float4) * c->slots, hipMemcpyDeviceToHost));
+        forEachParticle(c, [&](size_t, size_t s) {
+            vals[4] += a[s].w >= 1.0f ? (uint64_t)a[s].w - 1 : 0;
+            vals[5] += (uint64_t)b[s].w;
+        });
+    }
+    for (int k = 0; k < n && k < 6; ++k) out[k] = vals[k];
+    // interaction and forced-acceptance counts restart from zero
+    HIP_TRY(hipMemset(c->dc.cnt.v, 0, 2 * sizeof(unsigned long long)));
+    return ASTRAEA_OK;
+}
+
+int astraea_download_tree(astraea_ctx* c, uint32_t seg, uint64_t* keys, uint32_t* perm, int32_t* left, int32_t* right,
+                          int32_t* pI, int32_t* pL, float* cm, float* mn, float* mx) {
+    TRY(requireLoaded(c));
+    if (seg >= c->numSeg) return fail(ASTRAEA_E_INVALID, "segment %u out of range", seg);
+    if (!c->treeValid) return fail(ASTRAEA_E_STATE, "no tree: call astraea_sort_and_build_bvh");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const uint32_t off = c->offs[seg], n = c->counts[seg], ni = n >= 2 ? n - 1 : 0;
+    const int o = (int)off;
+    auto toLocal = [&](int v) { return v < 0 ? ~((~v) - o) : v - o; };
+    if (keys) HIP_TRY(hipMemcpy(keys, c->dc.sb.keyA + off, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+    if (perm) {
+        HIP_TRY(hipMemcpy(perm, c->dc.sb.valA + off, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+        for (uint32_t k = 0; k < n; ++k) perm[k] -= off;
+    }
+    if (ni > 0 && (left || right)) {
+        std::vector<int2> ch(ni);
+        HIP_TRY(hipMemcpy(ch.data(), c->dc.t.child + off, sizeof(int2) * ni, hipMemcpyDeviceToHost));
+        for (uint32_t k = 0; k < ni; ++k) {
+            if (left) left[k] = toLocal(ch[k].x);
+            if (right) right[k] = toLocal(ch[k].y);
+        }
+    }
+    if (ni > 0 && pI) {
+        HIP_TRY(hipMemcpy(pI, c->dc.t.parentI + off, sizeof(int32_t) * ni, hipMemcpyDeviceToHost));
+        for (uint32_t k = 0; k < ni; ++k) pI[k] = pI[k] < 0 ? -1 : pI[k] - o;
+    }
+    if (pL) {
+        HIP_TRY(hipMemcpy(pL, c->dc.t.parentL + off, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        for (uint32_t k = 0; k < n; ++k) pL[k] = pL[k] < 0 ? -1 : pL[k] - o;
+    }
+    if (ni > 0 && cm) HIP_TRY(hipMemcpy(cm, c->dc.t.cm + off, sizeof(float4) * ni, hipMemcpyDeviceToHost));
+    if (ni > 0 && mn) HIP_TRY(hipMemcpy(mn, c->dc.t.bmin + off, sizeof(float4) * ni, hipMemcpyDeviceToHost));
+    if (ni > 0 && mx) HIP_TRY(hipMemcpy(mx, c->dc.t.bmax + off, sizeof(float4) * ni, hipMemcpyDeviceToHost));
+    return ASTRAEA_OK;
+}
+
+int astraea_enable_stage_timing(astraea_ctx* c, int enable) {
+    if (!c) return fail(ASTRAEA_E_INVALID, "null context");
+    if (enable && c->evPool.empty()) {
+        c->evPool.resize(astraea_ctx::kPool, nullptr);
+        for (auto& e : c->evPool) HIP_TRY(hipEventCreate(&e));
+    }
+    collectTimes(c);
+    c->timing = enable != 0;
+    for (int k = 0; k < ASTRAEA_STAGE_COUNT_; ++k) { c->stageMs[k] = 0.0; c->stageN[k] = 0; }
+    return ASTRAEA_OK;
+}
+
+int astraea_stage_times(astraea_ctx* c, double* ms, uint64_t* samples) {
+    if (!c) return fail(ASTRAEA_E_INVALID, "null context");
+    collectTimes(c);
+    for (int k = 0; k < ASTRAEA_STAGE_COUNT_; ++k) {
+        if (ms) ms[k] = c->stageMs[k];
+        if (samples) samples[k] = c->stageN[k];
+        c->stageMs[k] = 0.0;
+        c->stageN[k] = 0;
+    }
+    return ASTRAEA_OK;
+}
+
+}  // extern "C"

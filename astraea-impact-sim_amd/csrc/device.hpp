@@ -1,0 +1,232 @@
+// device.hpp — device data layout and shared helpers of the gfx950 kernels.
+//
+// Layout in HBM (one context):
+//   * particle slots are Morton-ordered inside each segment (scenario); segment s
+//     owns slots [segOff[s], segOff[s] + segPad[s]) with segOff a multiple of
+//     ASTRAEA_SEGMENT_ALIGN (= the 256-thread block), so a block and a wavefront
+//     never straddle two scenarios.  The first segN[s] slots are particles, the
+//     rest are inactive padding.
+//   * persistent state, ping-ponged through the sort's permutation, packed as
+//     float4 so one lane moves one 16-byte record:
+//       pos   (x, y, z, m)          vel (vx, vy, vz, rho)      hu (h, h0, u, D)
+//       str0  (sxx, syy, szz, sxy)  str1 (sxz, syz, D^(1/3), eps_fail)
+//       meta  (material | body << 8 | tracer << 16)            oid (input index)
+//     plus FP64 x/y/z and vx/vy/vz in the FP64-accumulator mode.
+//   * derived per-step arrays in the current order (not permuted).
+//   * tree: internal node of local index i of segment s lives at slot
+//     segOff[s] + i; a child c < 0 is the particle slot ~c.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/astraea_hip.h"
+
+namespace astraea_hip {
+
+constexpr int kBlock = 256;                 // threads per block for every kernel
+constexpr int kWave = 64;                   // CDNA4 wavefront
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kStack = ASTRAEA_STACK_DEPTH;
+constexpr float kPiF = 3.14159265358979323846f;
+constexpr float kGf = 66.7430f;             // G in PSU, engine/units.hpp:28
+constexpr uint64_t kPadKey = 0x7fffffffffffffffULL;
+
+struct Persist {
+    float4* pos;
+    float4* vel;
+    float4* hu;
+    float4* str0;
+    float4* str1;
+    uint32_t* meta;
+    uint32_t* oid;
+    double* p64;   // 3 arrays of cap doubles: x | y | z  (FP64 accumulator mode)
+    double* v64;   // 3 arrays: vx | vy | vz
+};
+
+struct Derived {
+    float4* posNew;   // drift output (x', y', z', m) in the pre-sort order
+    float4* grav;     // (gx, gy, gz, phi)
+    float4* rate;     // (ax, ay, az, drho/dt): total acceleration after pass 2
+    float4* rate2;    // (du/dt, dD^(1/3)/dt, radPowerW, 0)
+    float4* dS0;      // (dsxx, dsyy, dszz, dsxy)
+    float2* dS1;      // (dsxz, dsyz)
+    float4* eos;      // (P, cs, T, 0) of the last EOS evaluation
+    uint8_t* phase;
+    float4* p1a;      // (Shepard C, f_surf, div v, neighbour count incl. self)
+    float4* p1b;      // (curl v x, y, z, Balsara)
+    float4* nb;       // pass-2 neighbour record (h', P_eff/rho^2, max(cs,0.05), Balsara)
+    float4* sr0;      // S/rho^2 (xx, yy, zz, xy) for pass 2
+    float2* sr1;      // S/rho^2 (xz, yz)
+};
+
+struct Tree {
+    int2* child;          // [slot of internal node] (left, right)
+    int32_t* parentI;     // parent of internal node
+    int32_t* parentL;     // parent of leaf slot
+    uint32_t* visited;    // bottom-up arrival counters
+    float4* cm;           // (com x, y, z, mass)
+    float4* bmin;         // (min x, y, z, extent = max side)
+    float4* bmax;         // (max x, y, z, max support bound of the subtree)
+};
+
+struct Neighbours {
+    uint32_t* count;      // entries of each slot
+    uint32_t* waveMax;    // max count in each wavefront
+    uint64_t* waveOff;    // start of each wavefront's block (entries), exclusive scan of 64*waveMax
+    uint32_t* list;       // entry k of slot (w*64 + l) at waveOff[w] + k*64 + l
+    uint64_t capacity;    // entries allocated
+};
+
+struct Segments {
+    uint32_t count;       // number of segments
+    uint32_t* off;        // [count+1] padded slot offsets
+    uint32_t* n;          // [count] particles
+    uint32_t* slotSeg;    // [slots] segment of every slot
+};
+
+// Per-segment device scalars.
+struct SegScalars {
+    uint32_t* boundsOrd;  // [6 per segment] min xyz / max xyz as ordered uints
+    uint32_t* dtRawBits;  // float bits of min dt_i (atomicMin target)
+    float* dt;            // dt of the current step
+    float* dtPrev;        // previous dt (0 before the first step)
+    double* radSum;       // sum of radPowerW after the last pass 2 (W)
+    double* radPrev;      // radSum of the previous step (W)
+    double* erad;         // radiated energy integral (PSU energy)
+    double* simTime;      // sum of dt (PSU time)
+    uint32_t* living;     // any tracer anchor with D < 0.8 (sph_gravity.cu:478-483)
+};
+
+struct Counters {
+    unsigned long long* v;  // [8]: 0 interactions, 1 stack overflows (gravity), 2 nbr entries,
+                            // 3 nbr overflow flag, 4 pass-1 pairs, 5 pass-2 pairs, 6 nbr stack overflow
+};
+
+struct Params {
+    float theta2;
+    float softFactor;
+    float cfl;
+    float gammaHHe;
+    float coolMinT;
+    uint32_t strengthMask;
+};
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int laneId() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+__device__ __forceinline__ int uniformInt(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint32_t uniformU32(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+
+// Logical block index that keeps consecutive blocks on one XCD: the dispatcher
+// places block b on XCD b % 8, so XCD x receives a contiguous range of logical
+// blocks and its private L2 holds one region of the Morton-ordered domain.
+__device__ __forceinline__ uint32_t xcdBlock(uint32_t b, uint32_t nb) {
+    const uint32_t x = b & 7u, i = b >> 3, per = nb >> 3, rem = nb & 7u;
+    return x * per + (x < rem ? x : rem) + i;
+}
+
+// Squared distance, ((dx*dx) + dy*dy) + dz*dz as two FMAs: the operation order every
+// neighbour and opening decision uses (reproduced by the oracle's FP32 replica).
+__device__ __forceinline__ float dist2f(float dx, float dy, float dz) {
+#pragma clang fp contract(off)
+    float t = dx * dx;
+    t = __builtin_fmaf(dy, dy, t);
+    t = __builtin_fmaf(dz, dz, t);
+    return t;
+}
+
+__device__ __forceinline__ float boxDist2f(float px, float py, float pz, float4 mn, float4 mx) {
+#pragma clang fp contract(off)
+    const float dx = fmaxf(0.0f, fmaxf(mn.x - px, px - mx.x));
+    const float dy = fmaxf(0.0f, fmaxf(mn.y - py, py - mx.y));
+    const float dz = fmaxf(0.0f, fmaxf(mn.z - pz, pz - mx.z));
+    return dist2f(dx, dy, dz);
+}
+
+// Largest support 2*max(h',1e-6) a particle can reach in pass 1's update
+// h' = clamp(h * clamp(cbrt(56/n), 0.92, 1.08), 0.35 h0, 6 h0) (sph_gravity.cu:785-787):
+// rounding is monotone, so h' <= max(1.08 h, 0.35 h0) holds in FP32 too.
+__device__ __forceinline__ float supportBound(float h, float h0) {
+#pragma clang fp contract(off)
+    return 2.0f * fmaxf(fmaxf(fmaxf(h, 1.0e-6f) * 1.08f, 0.35f * fmaxf(h0, 1.0e-6f)), 1.0e-6f);
+}
+
+// Ordered-uint encoding of floats for atomic min/max (sph_gravity.cu:140-149).
+__device__ __forceinline__ uint32_t floatToOrdered(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return u ^ ((uint32_t)(-(int32_t)(u >> 31)) | 0x80000000u);
+}
+__device__ __forceinline__ float orderedToFloat(uint32_t u) {
+    return __uint_as_float(u ^ (((u >> 31) - 1u) | 0x80000000u));
+}
+
+// Wendland C2 kernel and gradient coefficient, support H (sph_gravity.cu:199-220).
+__device__ __forceinline__ float wendland(float r, float H) {
+    if (r >= H || H <= 0.0f) return 0.0f;
+    const float q = r / H, o = 1.0f - q, o2 = o * o;
+    return 21.0f / (2.0f * kPiF * H * H * H) * o2 * o2 * (1.0f + 4.0f * q);
+}
+__device__ __forceinline__ float wendlandGrad(float r, float H) {
+    if (r >= H || H <= 0.0f) return 0.0f;
+    const float q = r / H, o = 1.0f - q, H2 = H * H;
+    return -(210.0f / (kPiF * H2 * H2 * H)) * o * o * o;
+}
+
+// Strength table (engine/sph_gravity.cu:35-73) gated by the params' strength mask.
+struct Strength {
+    float mu, tMelt, bulk, y0;
+};
+__device__ __forceinline__ Strength strengthOf(int mat, uint32_t mask) {
+    const int mi = (mat >= 0 && mat <= 4) ? mat : 1;
+    Strength s{0.0f, 0.0f, 0.0f, 0.0f};
+    if (!((mask >> mi) & 1u)) return s;
+    switch (mi) {
+        case 0: s = {7.70e10f / 1.0e12f, 1811.0f, 1.28e11f / 1.0e12f, 6.80e8f / 1.0e12f}; break;
+        case 1: s = {2.20e10f / 1.0e12f, 1600.0f, 1.80e10f / 1.0e12f, 3.50e9f / 1.0e12f}; break;
+        case 2: s = {3.50e9f / 1.0e12f, 273.15f, 2.18e9f / 1.0e12f, 1.00e7f / 1.0e12f}; break;
+        case 4: s = {8.00e9f / 1.0e12f, 1500.0f, 1.20e10f / 1.0e12f, 5.00e7f / 1.0e12f}; break;
+        default: break;
+    }
+    return s;
+}
+
+// Per-material reference energy c_v * 300 K in PSU (sph_gravity.cu:27-33).
+__device__ __forceinline__ float u0Ref(int mat) {
+    switch ((mat >= 0 && mat <= 4) ? mat : 1) {
+        case 0: return (float)((449.0 * 300.0) / 1.0e6);
+        case 2: return (float)((4184.0 * 300.0) / 1.0e6);
+        case 3: return (float)((8260.0 * 300.0) / 1.0e6);
+        case 4: return (float)((800.0 * 300.0) / 1.0e6);
+        default: return (float)((790.0 * 300.0) / 1.0e6);
+    }
+}
+
+__device__ __forceinline__ float waveMinF(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ float waveMaxF(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double waveSumD(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double waveMaxD(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ uint32_t waveMaxU(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t w = (uint32_t)__shfl_xor((int)v, o, 64);
+        v = v > w ? v : w;
+    }
+    return v;
+}
+
+}  // namespace astraea_hip

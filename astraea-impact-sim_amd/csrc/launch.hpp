@@ -1,0 +1,74 @@
+// launch.hpp — host-side launchers of the gfx950 kernels (one translation unit per stage).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device.hpp"
+
+namespace astraea_hip {
+
+struct SortBuffers {
+    uint64_t* keyA;
+    uint64_t* keyB;
+    uint32_t* valA;
+    uint32_t* valB;
+    uint32_t* counts;      // [256 * tiles] digit-major tile histograms
+    uint32_t* tilePrefix;  // [256 * tiles] exclusive prefix over tiles per digit
+    uint32_t* digitTotal;  // [256]
+    uint32_t tilesCap;
+};
+
+// Everything a kernel may need, passed by value.
+struct DeviceCtx {
+    uint32_t slots;      // padded particle slots in use
+    uint32_t blocks;     // slots / kBlock
+    uint32_t stride64;   // element stride between the x, y, z parts of p64 / v64 (allocated slots)
+    Segments seg;
+    SegScalars ss;
+    Persist cur;         // state in the current order
+    Persist alt;         // permutation target
+    Derived d;
+    Tree t;
+    Neighbours nb;
+    Counters cnt;
+    SortBuffers sb;
+    Params prm;
+    bool fp64;           // FP64 position/velocity accumulators
+    bool strength;       // any material has strength enabled
+};
+
+// sort.hip: stable LSD radix sort of (key, value) over `bits` key bits, then (segments > 1)
+// one stable pass on the segment of each value.  Result in keyA/valA.
+void sortPairs(const DeviceCtx& c, hipStream_t s);
+
+// tree.hip
+void launchPrepareSort(const DeviceCtx& c, hipStream_t s);    // posNew = pos, bounds
+void launchMorton(const DeviceCtx& c, hipStream_t s);
+void launchPermuteKarras(const DeviceCtx& c, hipStream_t s);  // cur -> alt + topology
+void launchMoments(const DeviceCtx& c, hipStream_t s);
+
+// neighbours.hip
+void launchNeighbours(const DeviceCtx& c, hipStream_t s);
+
+// sph.hip
+void launchPass1(const DeviceCtx& c, hipStream_t s);
+void launchEos(const DeviceCtx& c, hipStream_t s);
+void launchPass2(const DeviceCtx& c, hipStream_t s);
+
+// gravity.hip
+void launchGravity(const DeviceCtx& c, hipStream_t s);
+
+// integrate.hip
+void launchStepBegin(const DeviceCtx& c, hipStream_t s, float maxDt);
+void launchStepEnd(const DeviceCtx& c, hipStream_t s);
+void launchKick(const DeviceCtx& c, hipStream_t s, bool second, bool clampThermal);
+void launchDrift(const DeviceCtx& c, hipStream_t s);
+void launchStepBeginNoBounds(const DeviceCtx& c, hipStream_t s, float maxDt);
+void launchDtCandidates(const DeviceCtx& c, hipStream_t s);   // dt_i min + radSum from the current state
+void launchResetBounds(const DeviceCtx& c, hipStream_t s);
+void launchInitIntegrator(const DeviceCtx& c, hipStream_t s); // radPrev = radSum, erad = time = dtPrev = 0
+void launchClear(const DeviceCtx& c, hipStream_t s);
+void launchEnergy(const DeviceCtx& c, hipStream_t s, double* dOut);
+
+}  // namespace astraea_hip

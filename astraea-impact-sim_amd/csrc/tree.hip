@@ -1,0 +1,220 @@
+// tree.hip — scene bounds, Morton keys, permutation + Karras radix tree, bottom-up moments.
+//
+// Restates launchSortAndBuildBvh (engine/sph_gravity.cu:1996-2061) per segment:
+//   bounds   min/max of positions as ordered-uint atomics (sph_gravity.cu:140-149, 273-333);
+//   Morton   q = clamp((x - min) * (2097151 / span), 0, 2097151) truncated, 21-bit spread,
+//            key = X << 2 | Y << 1 | Z (sph_gravity.cu:151-171, 335-361);
+//   permute  gather of the persistent state by the sorted slot (sph_gravity.cu:363-430), fused
+//            with the Karras build (sph_gravity.cu:432-529), which needs only the sorted keys;
+//   moments  leaf-to-root climb with an arrival counter per node (sph_gravity.cu:531-620): the
+//            counter is an agent-scope acq_rel atomic, so the second arriver sees the first
+//            one's node record across the 8 XCD-private L2 caches.
+// All FP32 arithmetic that decides integer results (quantisation, combine of moments) is
+// written with explicit operation order; the oracle repeats it bit for bit.
+#include "launch.hpp"
+
+namespace astraea_hip {
+namespace {
+
+__device__ __forceinline__ uint64_t spread21(uint64_t v) {
+    v &= 0x1fffffULL;
+    v = (v | (v << 32)) & 0x001f00000000ffffULL;
+    v = (v | (v << 16)) & 0x001f0000ff0000ffULL;
+    v = (v | (v << 8)) & 0x100f00f00f00f00fULL;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ULL;
+    v = (v | (v << 2)) & 0x1249249249249249ULL;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t quantise(float x, float mn, float inv) {
+#pragma clang fp contract(off)
+    const float f = fminf(fmaxf((x - mn) * inv, 0.0f), 2097151.0f);
+    return (uint32_t)f;
+}
+
+__device__ __forceinline__ float invSpan(float mn, float mx) {
+#pragma clang fp contract(off)
+    const float span = mx - mn;
+    return span > 1.0e-6f ? __fdiv_rn(2097151.0f, span) : 0.0f;
+}
+
+__device__ __forceinline__ int karrasDelta(const uint64_t* __restrict__ k, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    const uint64_t a = k[i], b = k[j];
+    if (a != b) return __clzll((long long)(a ^ b));
+    return 64 + __clz((int)((uint32_t)i ^ (uint32_t)j));
+}
+
+// Block min/max of positions, then 6 ordered-uint atomics on the block's segment.
+__device__ __forceinline__ void boundsAtomics(float x, float y, float z, bool active, uint32_t* boundsOrd) {
+    __shared__ float sb[kWavesPerBlock][6];
+    const int wave = threadIdx.x >> 6, lane = laneId();
+    float v[6] = {active ? x : 1.0e30f, active ? y : 1.0e30f, active ? z : 1.0e30f,
+                  active ? x : -1.0e30f, active ? y : -1.0e30f, active ? z : -1.0e30f};
+    for (int a = 0; a < 3; ++a) v[a] = waveMinF(v[a]);
+    for (int a = 3; a < 6; ++a) v[a] = waveMaxF(v[a]);
+    if (lane == 0)
+        for (int a = 0; a < 6; ++a) sb[wave][a] = v[a];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int a = threadIdx.x;
+        float r = sb[0][a];
+        for (int w = 1; w < kWavesPerBlock; ++w) r = a < 3 ? fminf(r, sb[w][a]) : fmaxf(r, sb[w][a]);
+        if (a < 3) atomicMin(&boundsOrd[a], floatToOrdered(r));
+        else atomicMax(&boundsOrd[a], floatToOrdered(r));
+    }
+}
+
+// posNew = current positions (the sort input when no drift precedes it) + bounds.
+__global__ __launch_bounds__(kBlock) void kPrepareSort(DeviceCtx c) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t sg = c.seg.slotSeg[i];
+    const bool active = i - c.seg.off[sg] < c.seg.n[sg];
+    const float4 p = c.cur.pos[i];  // in the FP64 mode pos always holds the rounded FP64 position
+    if (active) c.d.posNew[i] = p;
+    boundsAtomics(p.x, p.y, p.z, active, c.ss.boundsOrd + 6 * sg);
+}
+
+__global__ __launch_bounds__(kBlock) void kMorton(DeviceCtx c) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t sg = c.seg.slotSeg[i];
+    const bool active = i - c.seg.off[sg] < c.seg.n[sg];
+    uint64_t key = kPadKey;
+    if (active) {
+        const uint32_t* bo = c.ss.boundsOrd + 6 * sg;
+        const float mnx = orderedToFloat(bo[0]), mny = orderedToFloat(bo[1]), mnz = orderedToFloat(bo[2]);
+        const float mxx = orderedToFloat(bo[3]), mxy = orderedToFloat(bo[4]), mxz = orderedToFloat(bo[5]);
+        const float4 p = c.d.posNew[i];
+        key = (spread21(quantise(p.x, mnx, invSpan(mnx, mxx))) << 2) |
+              (spread21(quantise(p.y, mny, invSpan(mny, mxy))) << 1) | spread21(quantise(p.z, mnz, invSpan(mnz, mxz)));
+    }
+    c.sb.keyA[i] = key;
+    c.sb.valA[i] = i;
+}
+
+// Gather the persistent state into the sorted order and build the Karras tree of each segment.
+__global__ __launch_bounds__(kBlock) void kPermuteKarras(DeviceCtx c) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t sg = c.seg.slotSeg[i];
+    const uint32_t off = c.seg.off[sg], n = c.seg.n[sg];
+    const uint32_t local = i - off;
+    if (local >= n) return;
+    const uint32_t src = c.sb.valA[i];
+    const size_t cap = (size_t)c.stride64;
+    float4 p = c.d.posNew[src];
+    if (c.fp64) {
+        const double x = c.cur.p64[src], y = c.cur.p64[cap + src], z = c.cur.p64[2 * cap + src];
+        c.alt.p64[i] = x; c.alt.p64[cap + i] = y; c.alt.p64[2 * cap + i] = z;
+        c.alt.v64[i] = c.cur.v64[src]; c.alt.v64[cap + i] = c.cur.v64[cap + src]; c.alt.v64[2 * cap + i] = c.cur.v64[2 * cap + src];
+        p.x = (float)x; p.y = (float)y; p.z = (float)z;
+    }
+    c.alt.pos[i] = p;
+    c.alt.vel[i] = c.cur.vel[src];
+    const float4 hu = c.cur.hu[src];
+    c.alt.hu[i] = hu;
+    c.alt.str0[i] = c.cur.str0[src];
+    c.alt.str1[i] = c.cur.str1[src];
+    const uint32_t meta = c.cur.meta[src];
+    c.alt.meta[i] = meta;
+    c.alt.oid[i] = c.cur.oid[src];
+    if (((meta >> 16) & 0xffu) != 0u && hu.w < 0.8f) atomicOr(&c.ss.living[sg], 1u);
+
+    if (n == 1u) {
+        c.t.parentL[i] = -1;
+        return;
+    }
+    if (local + 1u < n) {
+        const uint64_t* k = c.sb.keyA + off;
+        const int li = (int)local, ni = (int)n;
+        const int d = (karrasDelta(k, ni, li, li + 1) - karrasDelta(k, ni, li, li - 1)) >= 0 ? 1 : -1;
+        const int dmin = karrasDelta(k, ni, li, li - d);
+        int lmax = 2;
+        while (karrasDelta(k, ni, li, li + lmax * d) > dmin) lmax <<= 1;
+        int l = 0;
+        for (int t = lmax >> 1; t >= 1; t >>= 1)
+            if (karrasDelta(k, ni, li, li + (l + t) * d) > dmin) l += t;
+        const int j = li + l * d;
+        const int dnode = karrasDelta(k, ni, li, j);
+        int s = 0, t = l;
+        do {
+            t = (t + 1) >> 1;
+            if (karrasDelta(k, ni, li, li + (s + t) * d) > dnode) s += t;
+        } while (t > 1);
+        const int gamma = li + s * d + (d < 0 ? d : 0);
+        const int first = li < j ? li : j, last = li > j ? li : j;
+        const int o = (int)off;
+        const bool lLeaf = first == gamma, rLeaf = last == gamma + 1;
+        c.t.child[i] = make_int2(lLeaf ? ~(o + gamma) : o + gamma, rLeaf ? ~(o + gamma + 1) : o + gamma + 1);
+        if (lLeaf) c.t.parentL[o + gamma] = (int)i; else c.t.parentI[o + gamma] = (int)i;
+        if (rLeaf) c.t.parentL[o + gamma + 1] = (int)i; else c.t.parentI[o + gamma + 1] = (int)i;
+        c.t.visited[i] = 0u;
+        if (local == 0u) c.t.parentI[i] = -1;
+    }
+}
+
+struct NodeRec {
+    float4 cm, mn, mx;
+};
+
+__device__ __forceinline__ NodeRec loadChild(const DeviceCtx& c, int ch) {
+    NodeRec r;
+    if (ch < 0) {
+        const int j = ~ch;
+        const float4 p = c.cur.pos[j];
+        const float4 hu = c.cur.hu[j];
+        r.cm = p;
+        r.mn = make_float4(p.x, p.y, p.z, 0.0f);
+        r.mx = make_float4(p.x, p.y, p.z, supportBound(hu.x, hu.y));
+    } else {
+        r.cm = c.t.cm[ch];
+        r.mn = c.t.bmin[ch];
+        r.mx = c.t.bmax[ch];
+    }
+    return r;
+}
+
+// Leaf-to-root climb over the permuted state (the host has swapped the sets: `cur` is sorted).
+__global__ __launch_bounds__(kBlock) void kMoments(DeviceCtx c) {
+#pragma clang fp contract(off)
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t sg = c.seg.slotSeg[i];
+    const uint32_t n = c.seg.n[sg];
+    if (i - c.seg.off[sg] >= n || n < 2u) return;
+    int p = c.t.parentL[i];
+    while (p >= 0) {
+        const uint32_t prior = __hip_atomic_fetch_add(&c.t.visited[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (prior == 0u) return;
+        const int2 ch = c.t.child[p];
+        const NodeRec L = loadChild(c, ch.x), R = loadChild(c, ch.y);
+        float4 mn, mx, cm;
+        mn.x = fminf(L.mn.x, R.mn.x); mn.y = fminf(L.mn.y, R.mn.y); mn.z = fminf(L.mn.z, R.mn.z);
+        mx.x = fmaxf(L.mx.x, R.mx.x); mx.y = fmaxf(L.mx.y, R.mx.y); mx.z = fmaxf(L.mx.z, R.mx.z);
+        mx.w = fmaxf(L.mx.w, R.mx.w);
+        const float mS = L.cm.w + R.cm.w;
+        if (mS > 0.0f) {
+            const float invM = __fdiv_rn(1.0f, mS);
+            cm.x = __builtin_fmaf(L.cm.w, L.cm.x, R.cm.w * R.cm.x) * invM;
+            cm.y = __builtin_fmaf(L.cm.w, L.cm.y, R.cm.w * R.cm.y) * invM;
+            cm.z = __builtin_fmaf(L.cm.w, L.cm.z, R.cm.w * R.cm.z) * invM;
+        } else {
+            cm.x = 0.5f * (L.cm.x + R.cm.x);
+            cm.y = 0.5f * (L.cm.y + R.cm.y);
+            cm.z = 0.5f * (L.cm.z + R.cm.z);
+        }
+        cm.w = mS;
+        mn.w = fmaxf(mx.x - mn.x, fmaxf(mx.y - mn.y, mx.z - mn.z));
+        c.t.cm[p] = cm;
+        c.t.bmin[p] = mn;
+        c.t.bmax[p] = mx;
+        p = c.t.parentI[p];
+    }
+}
+
+}  // namespace
+
+void launchPrepareSort(const DeviceCtx& c, hipStream_t s) { kPrepareSort<<<c.blocks, kBlock, 0, s>>>(c); }
+void launchMorton(const DeviceCtx& c, hipStream_t s) { kMorton<<<c.blocks, kBlock, 0, s>>>(c); }
+void launchPermuteKarras(const DeviceCtx& c, hipStream_t s) { kPermuteKarras<<<c.blocks, kBlock, 0, s>>>(c); }
+void launchMoments(const DeviceCtx& c, hipStream_t s) { kMoments<<<c.blocks, kBlock, 0, s>>>(c); }
+
+}  // namespace astraea_hip

@@ -1,0 +1,288 @@
+/*
+ * astraea_hip.h — C ABI of the MI355X (gfx950) per-step physics library.
+ *
+ * This is the drop-in boundary under the reference's `astraea::Simulation`
+ * (engine/simulation.hpp:12-47).  It replaces the reference's CUDA launch
+ * layer `astraea::cuda::*` (engine/sph_gravity.cuh:61-92) and the device-buffer
+ * management of `Simulation::Impl` (engine/simulation.cpp:345-536).  Every
+ * signature uses plain pointers, sizes and status codes so that C, C++ and
+ * Python ctypes can bind it.
+ *
+ * Conventions
+ *  - Every function returns 0 on success or a negative ASTRAEA_E_* code.  The
+ *    message of the last failure on the calling thread is returned by
+ *    astraea_last_error().  The reference throws std::runtime_error /
+ *    std::invalid_argument instead (engine/sph_gravity.cuh:12-22,
+ *    engine/sph_gravity.cu:2001-2003, engine/simulation.cpp:1296-1298,
+ *    1712-1714); the C++ Simulation layer above this ABI rethrows.
+ *  - A context owns one HIP stream and all device memory.  Calls on one
+ *    context must be serialised by the caller (the reference's contract:
+ *    one host thread drives one default stream, simulation.cpp:1711-1753).
+ *  - Particles are grouped in segments (independent scenarios).  A single
+ *    scenario is one segment.  Host arrays passed to upload/download are
+ *    the segments' particles concatenated in segment order, without padding.
+ *  - Units are the reference's PSU system (engine/units.hpp:13-27):
+ *    length 1e6 m, mass 1e24 kg, time 1e3 s.
+ */
+#ifndef ASTRAEA_HIP_H
+#define ASTRAEA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ASTRAEA_ABI_VERSION 1
+
+/* Status codes */
+#define ASTRAEA_OK 0
+#define ASTRAEA_E_INVALID (-1)   /* bad argument (reference: std::invalid_argument) */
+#define ASTRAEA_E_HIP (-2)       /* HIP runtime failure (reference: ASTRAEA_CUDA_CHECK) */
+#define ASTRAEA_E_STATE (-3)     /* call order (reference: step before loadScenario) */
+#define ASTRAEA_E_CAPACITY (-4)  /* capacity < count (sph_gravity.cu:2001-2003) */
+#define ASTRAEA_E_OVERFLOW (-5)  /* a device-side capacity (neighbour table, stack) overflowed */
+
+/* Physics knobs.  Defaults (astraea_default_params) reproduce the reference's
+ * compile-time constants; the benchmark configurations change them. */
+typedef struct astraea_params {
+    float theta;              /* BH opening angle; reference literal 0.5 (simulation.cpp:533,1729) */
+    float soft_factor;        /* eps_i = soft_factor * h0_i; reference 0.8 (sph_gravity.cu:842,873) */
+    float cfl_factor;         /* CFL coefficient; reference 0.25 (sph_gravity.cu:1510) */
+    float gamma_hhe;          /* H/He ideal-gas gamma; reference 1.4 (eos.cu:93) */
+    float cooling_min_temp_k; /* radiative cooling only where T > this; reference 0 (sph_gravity.cu:1222-1235) */
+    uint32_t strength_mask;   /* bit m: material m carries deviatoric stress and damage;
+                                 reference: every material with mu > 0 (sph_gravity.cu:42-73) = 0x17 */
+    uint32_t precision;       /* ASTRAEA_PREC_FP32 (reference layout) or ASTRAEA_PREC_FP64_ACCUM */
+    uint32_t reserved[5];
+} astraea_params;
+
+#define ASTRAEA_PREC_FP32 0u       /* FP32 positions and velocities, as the reference */
+#define ASTRAEA_PREC_FP64_ACCUM 1u /* FP64 master positions/velocities for kick and drift */
+
+/* Traversal group: the gravity opening decision is shared by one wavefront. */
+#define ASTRAEA_GROUP_SIZE 64
+/* Each segment starts at a multiple of this many particle slots. */
+#define ASTRAEA_SEGMENT_ALIGN 256
+/* Depth of the per-wavefront traversal stack; the reference uses 192 (sph_gravity.cu:22). */
+#define ASTRAEA_STACK_DEPTH 192
+
+/* Per-particle fields for upload/download. */
+enum astraea_field {
+    /* persistent state, float32 */
+    ASTRAEA_F_POS_X = 0, ASTRAEA_F_POS_Y, ASTRAEA_F_POS_Z,
+    ASTRAEA_F_VEL_X, ASTRAEA_F_VEL_Y, ASTRAEA_F_VEL_Z,
+    ASTRAEA_F_MASS, ASTRAEA_F_H, ASTRAEA_F_H0, ASTRAEA_F_RHO, ASTRAEA_F_U,
+    ASTRAEA_F_SXX, ASTRAEA_F_SYY, ASTRAEA_F_SZZ, ASTRAEA_F_SXY, ASTRAEA_F_SXZ, ASTRAEA_F_SYZ,
+    ASTRAEA_F_DAMAGE, ASTRAEA_F_DAMAGE_CBRT, ASTRAEA_F_EPS_FAIL,
+    /* persistent state, uint8 / uint32 */
+    ASTRAEA_F_MATERIAL, ASTRAEA_F_BODY, ASTRAEA_F_TRACER, ASTRAEA_F_ORIGINAL_ID,
+    /* persistent state, float64 (ASTRAEA_PREC_FP64_ACCUM only) */
+    ASTRAEA_F_POS64_X, ASTRAEA_F_POS64_Y, ASTRAEA_F_POS64_Z,
+    ASTRAEA_F_VEL64_X, ASTRAEA_F_VEL64_Y, ASTRAEA_F_VEL64_Z,
+    /* derived, float32 (download only) */
+    ASTRAEA_F_ACC_X, ASTRAEA_F_ACC_Y, ASTRAEA_F_ACC_Z,
+    ASTRAEA_F_ACC_GRAV_X, ASTRAEA_F_ACC_GRAV_Y, ASTRAEA_F_ACC_GRAV_Z, ASTRAEA_F_PHI,
+    ASTRAEA_F_DRHO_DT, ASTRAEA_F_DU_DT, ASTRAEA_F_PRESSURE, ASTRAEA_F_SOUND_SPEED, ASTRAEA_F_TEMP_K,
+    ASTRAEA_F_SHEPARD, ASTRAEA_F_SURF_FACTOR, ASTRAEA_F_DIV_V,
+    ASTRAEA_F_CURL_X, ASTRAEA_F_CURL_Y, ASTRAEA_F_CURL_Z,
+    ASTRAEA_F_DS_XX, ASTRAEA_F_DS_YY, ASTRAEA_F_DS_ZZ, ASTRAEA_F_DS_XY, ASTRAEA_F_DS_XZ, ASTRAEA_F_DS_YZ,
+    ASTRAEA_F_DDAMAGE_CBRT_DT, ASTRAEA_F_RAD_POWER_W,
+    /* derived, uint8 */
+    ASTRAEA_F_PHASE,
+    /* derived, uint32: neighbour count of the last pass-1 (self included, as the reference) */
+    ASTRAEA_F_NGB_COUNT,
+    ASTRAEA_F_COUNT_
+};
+
+/* Element size in bytes of a field (4, 1, 8), or 0 for an unknown id. */
+size_t astraea_field_size(int field);
+
+const char* astraea_last_error(void);
+int astraea_abi_version(void);
+void astraea_default_params(astraea_params* out);
+
+typedef struct astraea_ctx astraea_ctx;
+
+/* Allocate a context on `device` for up to `capacity` particles in up to
+ * `max_segments` segments.  Replaces allocateParticleBuffers/allocateRadixBvh
+ * (sph_gravity.cuh:83-86) and Simulation::Impl::ensureCapacity (simulation.cpp:428-454). */
+int astraea_create(astraea_ctx** out, int device, uint32_t capacity, uint32_t max_segments);
+int astraea_destroy(astraea_ctx* ctx);
+/* The context's HIP stream (a hipStream_t), for callers that time or order work on it. */
+void* astraea_stream(astraea_ctx* ctx);
+
+int astraea_set_params(astraea_ctx* ctx, const astraea_params* params);
+int astraea_get_params(astraea_ctx* ctx, astraea_params* out);
+
+/* Declare the segment layout: `counts[s]` particles in segment s.  Clears all
+ * state.  Replaces the count bookkeeping of uploadHostStateToDevice
+ * (simulation.cpp:469-475). */
+int astraea_set_layout(astraea_ctx* ctx, uint32_t num_segments, const uint32_t* counts);
+int astraea_num_particles(astraea_ctx* ctx, uint32_t* out_total);
+
+/* Copy one field between host (concatenated segments) and device.  Upload is
+ * valid for persistent fields; download for every field.  Fields that are not
+ * uploaded start as in uploadHostStateToDevice (simulation.cpp:469-520):
+ * zeros, except EPS_FAIL = 1, ORIGINAL_ID = index in the concatenation. */
+int astraea_upload_field(astraea_ctx* ctx, int field, const void* host_src);
+int astraea_download_field(astraea_ctx* ctx, int field, void* host_dst);
+
+/* ---- stage entry points (reference launch layer, sph_gravity.cuh:61-92) ---- */
+
+/* launchClearStats (sph_gravity.cu:1986-1994): zero the derived accumulators. */
+int astraea_clear_stats(astraea_ctx* ctx);
+/* launchSortAndBuildBvh (sph_gravity.cu:1996-2061): per-segment AABB, 63-bit Morton
+ * keys, stable radix sort, permutation of the particle state, Karras radix
+ * tree, bottom-up node moments, and the neighbour table of the new order. */
+int astraea_sort_and_build_bvh(astraea_ctx* ctx);
+/* launchSphPass1Neighbors (sph_gravity.cu:2063-2074), plus the EOS of the same
+ * particle (launchEvaluateEos) fused into its epilogue. */
+int astraea_sph_pass1(astraea_ctx* ctx);
+/* launchEvaluateEos (sph_gravity.cu:2076-2083). */
+int astraea_evaluate_eos(astraea_ctx* ctx);
+/* launchGravityBarnesHut (sph_gravity.cu:2085-2097); theta from the params. */
+int astraea_gravity(astraea_ctx* ctx);
+/* launchSphPass2Forces (sph_gravity.cu:2099-2111). */
+int astraea_sph_pass2(astraea_ctx* ctx);
+/* launchReduceDtPsu (sph_gravity.cu:2136-2161) for every segment.  dt stays on
+ * the device; `out_dt` (num_segments floats, may be NULL) forces a copy-out. */
+int astraea_reduce_dt(astraea_ctx* ctx, float max_dt_psu, float* out_dt);
+/* launchLeapfrogKickHalf (sph_gravity.cu:2113-2120) with half of the device dt. */
+int astraea_kick_half(astraea_ctx* ctx, int clamp_thermal);
+/* launchLeapfrogDrift (sph_gravity.cu:2122-2134) with the device dt. */
+int astraea_drift(astraea_ctx* ctx);
+
+/* Simulation::Impl::evaluateInitialForcesOnDevice (simulation.cpp:525-536):
+ * clear -> sort/BVH -> pass 1 (+EOS) -> gravity -> pass 2, then the dt reduction
+ * input state is ready. */
+int astraea_evaluate_initial_forces(astraea_ctx* ctx);
+
+/* `steps` full KDK steps, the kernel sequence of Simulation::step
+ * (simulation.cpp:1721-1732): dt -> kick -> drift -> sort/BVH -> pass 1 -> EOS ->
+ * gravity -> pass 2 -> kick -> EOS.  Asynchronous on the context stream; with
+ * `use_graph` the sequence is captured once in a hipGraph and replayed. */
+int astraea_step(astraea_ctx* ctx, float max_dt_psu, uint32_t steps, int use_graph);
+
+/* Block until the context stream is idle; reports device-side overflows. */
+int astraea_synchronize(astraea_ctx* ctx);
+
+/* Per-segment scalars after the last dt reduction: dt (PSU) of the last step. */
+int astraea_get_dt(astraea_ctx* ctx, float* out_dt);
+
+/* Per-segment FP64 reductions on the device (Simulation telemetry,
+ * simulation.cpp:985-1087).  `out` receives ASTRAEA_ENERGY_STRIDE doubles per
+ * segment, in the order of the ASTRAEA_EN_* indices (PSU units). */
+#define ASTRAEA_EN_MASS 0
+#define ASTRAEA_EN_KINETIC 1
+#define ASTRAEA_EN_POTENTIAL 2   /* 1/2 sum m phi */
+#define ASTRAEA_EN_INTERNAL 3
+#define ASTRAEA_EN_MOM_X 4
+#define ASTRAEA_EN_MOM_Y 5
+#define ASTRAEA_EN_MOM_Z 6
+#define ASTRAEA_EN_ANG_X 7
+#define ASTRAEA_EN_ANG_Y 8
+#define ASTRAEA_EN_ANG_Z 9
+#define ASTRAEA_EN_RADIATED 10   /* radiated energy integral (PSU energy), trapezoid as simulation.cpp:1741-1744 */
+#define ASTRAEA_EN_RAD_POWER_W 11 /* sum of radPowerW (W) */
+#define ASTRAEA_EN_MAX_TEMP 12
+#define ASTRAEA_EN_MAX_PRESSURE 13
+#define ASTRAEA_EN_COM_X 14
+#define ASTRAEA_EN_COM_Y 15
+#define ASTRAEA_EN_COM_Z 16
+#define ASTRAEA_EN_SIM_TIME 17   /* accumulated sum of dt (PSU time) */
+#define ASTRAEA_ENERGY_STRIDE 18
+int astraea_energy(astraea_ctx* ctx, double* out);
+
+/* Counters of the last gravity evaluation: `out[0]` = sum over active particles
+ * of the particle-node and particle-particle interactions evaluated, `out[1]` =
+ * traversal-stack overflows (forced acceptances, sph_gravity.cu:933), `out[2]`
+ * = neighbour-table entries, `out[3]` = neighbour-table capacity, `out[4]` =
+ * sum of pass-1 neighbour counts (self excluded), `out[5]` = sum of pass-2 pairs. */
+int astraea_counters(astraea_ctx* ctx, uint64_t* out, int n);
+
+/* Tree of segment `seg` from the last sort, in the reference's local encoding
+ * (sph_gravity.cuh:26-59): internal nodes 0..n-2, a negative child c is the leaf
+ * ~c.  Any pointer may be NULL.  `keys`/`perm` have n entries (sorted Morton
+ * keys and indicesSorted, local to the segment); `left`, `right`,
+ * `parent_internal` have n-1; `parent_leaf` has n; `node_cm` (com.xyz, mass),
+ * `node_min` (aabb min.xyz, extent), `node_max` (aabb max.xyz, max support)
+ * have 4*(n-1) floats. */
+int astraea_download_tree(astraea_ctx* ctx, uint32_t seg,
+                          uint64_t* keys, uint32_t* perm,
+                          int32_t* left, int32_t* right,
+                          int32_t* parent_internal, int32_t* parent_leaf,
+                          float* node_cm, float* node_min, float* node_max);
+
+/* Stage timing: when enabled, astraea_step and the stage calls record HIP events
+ * around each stage; astraea_stage_times returns the accumulated milliseconds
+ * per ASTRAEA_STAGE_* and the number of samples, and resets them. */
+#define ASTRAEA_STAGE_DT 0
+#define ASTRAEA_STAGE_KICK1 1
+#define ASTRAEA_STAGE_DRIFT 2
+#define ASTRAEA_STAGE_MORTON 3
+#define ASTRAEA_STAGE_SORT 4
+#define ASTRAEA_STAGE_PERMUTE_TREE 5
+#define ASTRAEA_STAGE_MOMENTS 6
+#define ASTRAEA_STAGE_NEIGHBOURS 7
+#define ASTRAEA_STAGE_PASS1 8
+#define ASTRAEA_STAGE_GRAVITY 9
+#define ASTRAEA_STAGE_PASS2 10
+#define ASTRAEA_STAGE_KICK2 11
+#define ASTRAEA_STAGE_COUNT_ 12
+int astraea_enable_stage_timing(astraea_ctx* ctx, int enable);
+int astraea_stage_times(astraea_ctx* ctx, double* out_ms, uint64_t* out_samples);
+
+/* ---- host-side initial conditions (no device work) ---- */
+
+/* A differentiated body on a seed-jittered hexagonal-close-packed lattice:
+ * uniform density per layer, u from the closed-form hydrostatic pressure of
+ * piecewise-constant density (see astraea_body_spec).  Writes `n` particles
+ * into the caller's arrays (any may be NULL).  Positions are centred on the
+ * body's centre of mass; velocities are zero. */
+#define ASTRAEA_MAX_LAYERS 4
+typedef struct astraea_body_spec {
+    double radius_m;                        /* outer radius */
+    uint32_t num_layers;                    /* 1..ASTRAEA_MAX_LAYERS, innermost first */
+    uint8_t layer_material[ASTRAEA_MAX_LAYERS];
+    double layer_mass_frac[ASTRAEA_MAX_LAYERS];
+    double layer_density_kgm3[ASTRAEA_MAX_LAYERS]; /* <= 0: the material's rho0 */
+    double h_over_spacing;                  /* smoothing length / lattice spacing (1.3) */
+    double jitter;                          /* jitter amplitude / spacing (0.1) */
+    double surface_temp_k;                  /* temperature where P = 0 (reference T_ref); <= 0: material tRefK */
+    uint32_t strength_flaws;                /* 1: draw Weibull flaws eps_fail */
+    double weibull_m;                       /* Weibull modulus (9) */
+    double weibull_k;                       /* Weibull k (m^-3), 4e29 */
+    uint64_t seed;
+} astraea_body_spec;
+
+typedef struct astraea_body_out {
+    float *x, *y, *z, *m, *h, *rho, *u, *eps_fail;
+    uint8_t* mat;
+} astraea_body_out;
+
+void astraea_default_body_spec(astraea_body_spec* out);
+/* Returns the body's total mass (kg) in *out_mass_kg and lattice spacing (m). */
+int astraea_build_body(const astraea_body_spec* spec, uint32_t n, astraea_body_out* out,
+                       double* out_mass_kg, double* out_spacing_m);
+
+/* Two-body placement on a Keplerian approach orbit in the XZ plane, the
+ * formulas of Simulation::loadScenario (simulation.cpp:1389-1436), followed by
+ * the zero-COM / zero-momentum shift of mergeTwoBodies (simulation.cpp:919-944).
+ * Adjusts x and vx/vz in place for n_t target then n_i impactor particles. */
+int astraea_place_two_bodies(uint32_t n_t, uint32_t n_i,
+                             float* x, float* y, float* z,
+                             float* vx, float* vy, float* vz, const float* m,
+                             double r_target_psu, double r_impactor_psu,
+                             double v_ratio_esc, double impact_param_b, double sep_radii,
+                             double* out_v_esc_psu);
+
+/* splitmix64: the seed sequence used for per-scenario seeds. */
+uint64_t astraea_splitmix64(uint64_t x);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ASTRAEA_HIP_H */

@@ -1,0 +1,365 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 oracle on the same seeded inputs.
+
+Integer stages (Morton keys, sort permutation, Karras topology, FP32 node moments, gravity
+interaction counts, pass-1 neighbour counts) must be bit-exact.  Floating-point results are
+compared by original particle id as relative RMS errors against the FP64 oracle:
+    rel_rms(x) = sqrt(sum |x_gpu - x_fp64|^2 / sum |x_fp64|^2)
+with the tolerances written in each test.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import rel_rms
+
+pytestmark = pytest.mark.gpu
+
+MAX_DT = 5.0 / 1.0e3  # Simulation::step default maxDtSec = 5 s, in PSU time
+
+
+def perturbed_small_state(seed=7, strength=False):
+    """The 2 x 8,192 impact with a random velocity dispersion (so every SPH term is active)."""
+    from astraea_amd import scenario as S
+
+    st = S.config_small()
+    rng = np.random.default_rng(seed)
+    n = len(st["mass"])
+    for k in ("vel_x", "vel_y", "vel_z"):
+        st[k] = (st[k] + rng.normal(0.0, 0.3, n)).astype(np.float32)
+    st["u"] = (st["u"] * (1.0 + 0.05 * rng.random(n))).astype(np.float32)
+    if strength:
+        for k in ("sxx", "syy", "szz", "sxy", "sxz", "syz"):
+            st[k] = rng.normal(0.0, 1.0e-3, n).astype(np.float32)
+        tr = (st["sxx"] + st["syy"] + st["szz"]) / 3.0
+        for k in ("sxx", "syy", "szz"):
+            st[k] = (st[k] - tr).astype(np.float32)
+        dc = (0.5 * rng.random(n)).astype(np.float32)
+        st["damage_cbrt"] = dc
+        st["damage"] = (dc * dc * dc).astype(np.float32)
+        st["eps_fail"] = (1.0e-4 * (1.0 + rng.random(n))).astype(np.float32)
+    return st
+
+
+def run_pair(lib, orc, state, counts, params, stages):
+    ctx = lib.Context(capacity=int(sum(counts)), max_segments=len(counts))
+    ctx.set_params(params)
+    ctx.set_layout(counts)
+    ctx.load_state(state)
+    o = orc.Oracle(params, group_size=lib.GROUP_SIZE)
+    o.set_layout(counts)
+    o.load_state(state)
+    for s in stages:
+        getattr(ctx, s[0])(*s[1:])
+        getattr(o, s[0])(*s[1:])
+    ctx.synchronize()
+    return ctx, o
+
+
+def by_id(ctx, o, name, f64=False):
+    """Field `name` of both, in original-id order."""
+    ig = ctx.download("original_id")
+    io = o.download("original_id")
+    g = ctx.download(name)
+    r = o.download64(name) if f64 else o.download(name)
+    out_g = np.empty_like(g)
+    out_r = np.empty_like(r)
+    out_g[ig] = g
+    out_r[io] = r
+    return out_g, out_r
+
+
+def vec_by_id(ctx, o, names):
+    gs, rs = zip(*[by_id(ctx, o, n, f64=True) for n in names])
+    return np.stack(gs, axis=1).astype(np.float64), np.stack(rs, axis=1)
+
+
+def assert_tree_equal(tg, to):
+    for k in ("keys", "perm", "left", "right", "parent_internal", "parent_leaf"):
+        assert np.array_equal(tg[k], to[k]), f"{k} differs at {np.flatnonzero(tg[k] != to[k])[:10]}"
+    for k in ("node_cm", "node_min", "node_max"):
+        a = tg[k].view(np.uint32)
+        b = to[k].view(np.uint32)
+        bad = np.flatnonzero(np.any(a != b, axis=1))
+        assert bad.size == 0, f"{k} differs at nodes {bad[:10]}: {tg[k][bad[:3]]} vs {to[k][bad[:3]]}"
+
+
+def test_abi_version(lib):
+    assert lib._lib.astraea_abi_version() == 1
+
+
+def test_tree_bit_exact_small_config(lib, orc):
+    from astraea_amd import scenario as S
+
+    st = S.config_small()
+    n = len(st["mass"])
+    ctx, o = run_pair(lib, orc, st, [n], S.plain_params(), [("sort_and_build",)])
+    assert_tree_equal(ctx.tree(0), o.tree(0))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 8, 64, 65, 257, 4097])
+def test_tree_bit_exact_edge_sizes(lib, orc, n):
+    from astraea_amd import scenario as S
+
+    rng = np.random.default_rng(n)
+    st = {
+        "pos_x": rng.normal(0, 1, n).astype(np.float32),
+        "pos_y": rng.normal(0, 1, n).astype(np.float32),
+        "pos_z": rng.normal(0, 1, n).astype(np.float32),
+        "mass": rng.uniform(0.5, 1.5, n).astype(np.float32),
+        "h": np.full(n, 0.1, np.float32), "h0": np.full(n, 0.1, np.float32),
+    }
+    if n >= 8:  # duplicate keys: identical positions (ties broken by index)
+        for k in ("pos_x", "pos_y", "pos_z"):
+            st[k][: n // 4] = st[k][0]
+    ctx, o = run_pair(lib, orc, st, [n], S.plain_params(), [("sort_and_build",)])
+    assert_tree_equal(ctx.tree(0), o.tree(0))
+
+
+def test_tree_bit_exact_degenerate_span(lib, orc):
+    """All particles on a plane (z span 0) and on a line: zero-span axes quantise to 0."""
+    from astraea_amd import scenario as S
+
+    n = 1000
+    rng = np.random.default_rng(3)
+    st = {
+        "pos_x": rng.random(n).astype(np.float32), "pos_y": np.full(n, 2.0, np.float32),
+        "pos_z": np.where(np.arange(n) % 2 == 0, 0.0, rng.random(n)).astype(np.float32) * 0.0,
+        "mass": np.ones(n, np.float32), "h": np.full(n, 0.01, np.float32), "h0": np.full(n, 0.01, np.float32),
+    }
+    ctx, o = run_pair(lib, orc, st, [n], S.plain_params(), [("sort_and_build",)])
+    assert_tree_equal(ctx.tree(0), o.tree(0))
+
+
+def test_tree_bit_exact_segments(lib, orc):
+    """Three independent scenarios in one batched layout: per-segment bounds, keys and trees."""
+    from astraea_amd import scenario as S
+
+    specs = S.sweep_scenarios(count=3, n_min=300, n_max=3000)
+    st, counts = S.build_sweep(specs)
+    ctx, o = run_pair(lib, orc, st, counts, S.plain_params(), [("sort_and_build",)])
+    for s in range(len(counts)):
+        assert_tree_equal(ctx.tree(s), o.tree(s))
+
+
+def force_parity(lib, orc, st, counts, params, tol):
+    ctx, o = run_pair(lib, orc, st, counts, params, [("initial_forces",)])
+    cg, co = ctx.counters(), o.counters()
+    # integer results of the decision replicas
+    assert cg["interactions"] == co["interactions"], (cg, co)
+    assert cg["forced_accepts"] == co["forced_accepts"] == 0
+    ng, no = by_id(ctx, o, "ngb_count")
+    assert np.array_equal(ng, no), f"pass-1 neighbour counts differ for {np.count_nonzero(ng != no)} particles"
+    assert cg["pass1_pairs"] == co["pass1_pairs"]
+    assert cg["pass2_pairs"] == co["pass2_pairs"]
+    err = {}
+    ag, ar = vec_by_id(ctx, o, ["acc_grav_x", "acc_grav_y", "acc_grav_z"])
+    err["acc_grav"] = rel_rms(ag, ar)
+    pg, pr = by_id(ctx, o, "phi", f64=True)
+    err["phi"] = rel_rms(pg, pr)
+    ag, ar = vec_by_id(ctx, o, ["acc_x", "acc_y", "acc_z"])
+    gg, gr = vec_by_id(ctx, o, ["acc_grav_x", "acc_grav_y", "acc_grav_z"])
+    # the SPH part on its own, normalised by itself (the total cancels against gravity)
+    err["acc_sph"] = rel_rms(ag - gg, ar - gr)
+    for name in ("drho_dt", "du_dt", "h", "shepard", "div_v", "pressure", "sound_speed", "temp_k"):
+        g, r = by_id(ctx, o, name, f64=True)
+        err[name] = rel_rms(g, r)
+    for k, v in err.items():
+        assert v < tol.get(k, tol["default"]), f"{k}: rel RMS {v:.3e} (all: {err})"
+    return err
+
+
+# Tolerances (relative RMS vs FP64).  Gravity, density rate and du/dt: 1e-5 (the north-star
+# bar).  The SPH acceleration is a sum of O(100) pressure terms that cancel to a few percent
+# of each term; FP32 rounding of the terms then shows up at ~1e-6..1e-5 of the result.
+FORCE_TOL = {"default": 1e-5, "acc_sph": 2e-5, "pressure": 1e-6, "sound_speed": 1e-6, "temp_k": 1e-6, "h": 1e-6}
+
+
+def test_force_parity_small_config(lib, orc):
+    from astraea_amd import scenario as S
+
+    st = perturbed_small_state()
+    err = force_parity(lib, orc, st, [len(st["mass"])], S.plain_params(), FORCE_TOL)
+    print(err)
+
+
+def test_force_parity_strength(lib, orc):
+    from astraea_amd import scenario as S
+
+    st = perturbed_small_state(strength=True)
+    p = S.plain_params(strength_mask=0x17)
+    ctx, o = run_pair(lib, orc, st, [len(st["mass"])], p, [("initial_forces",)])
+    for name in ("ds_xx", "ds_xy", "ds_yz", "ddamage_cbrt_dt"):
+        g, r = by_id(ctx, o, name, f64=True)
+        assert rel_rms(g, r) < 1e-4, (name, rel_rms(g, r))
+    ag, ar = vec_by_id(ctx, o, ["acc_x", "acc_y", "acc_z"])
+    gg, gr = vec_by_id(ctx, o, ["acc_grav_x", "acc_grav_y", "acc_grav_z"])
+    assert rel_rms(ag - gg, ar - gr) < 2e-5
+    g, r = by_id(ctx, o, "du_dt", f64=True)
+    assert rel_rms(g, r) < 1e-5
+
+
+def test_force_parity_segments(lib, orc):
+    from astraea_amd import scenario as S
+
+    specs = S.sweep_scenarios(count=3, n_min=2000, n_max=6000)
+    st, counts = S.build_sweep(specs)
+    force_parity(lib, orc, st, counts, S.plain_params(), FORCE_TOL)
+
+
+def test_gravity_matches_direct_sum_ac3(lib):
+    """tests/cpp/test_gravity_sph.cpp:293-395: 2 x 2048 bodies at +-12 PSU, theta = 0.5 vs FP64 N^2
+    with the same pairwise softening: median relative error < 1.5 %, p99 < 5 %, all phi < 0."""
+    from astraea_amd import scenario as S
+
+    a = lib.build_body(2048, 3.39e6, S.FE_GRANITE, seed=42)
+    b = lib.build_body(2048, 2.44e6, [(0, 1.0, 0.0)], seed=43)
+    n = 4096
+    st = {
+        "pos_x": np.concatenate([a["x"] - 12.0, b["x"] + 12.0]).astype(np.float32),
+        "pos_y": np.concatenate([a["y"], b["y"]]), "pos_z": np.concatenate([a["z"], b["z"]]),
+        "mass": np.concatenate([a["m"], b["m"]]), "h": np.concatenate([a["h"], b["h"]]),
+        "h0": np.concatenate([a["h"], b["h"]]),
+    }
+    params = lib.default_params()  # reference softening 0.8 h0
+    ctx = lib.Context(n)
+    ctx.set_params(params)
+    ctx.set_layout([n])
+    ctx.load_state(st)
+    ctx.sort_and_build()
+    ctx.gravity()
+    ctx.synchronize()
+    x = np.stack([ctx.download("pos_x"), ctx.download("pos_y"), ctx.download("pos_z")], 1).astype(np.float64)
+    m = ctx.download("mass").astype(np.float64)
+    h0 = ctx.download("h0").astype(np.float64)
+    acc = np.stack([ctx.download("acc_grav_x"), ctx.download("acc_grav_y"), ctx.download("acc_grav_z")], 1)
+    phi = ctx.download("phi")
+    eps = 0.8 * np.maximum(h0, 1e-6)
+    d = x[:, None, :] - x[None, :, :]
+    e = 0.5 * (eps[:, None] + eps[None, :])
+    d2 = np.sum(d * d, axis=2) + e * e
+    np.fill_diagonal(d2, np.inf)
+    ref = -66.743 * np.sum((m[None, :] / (d2 * np.sqrt(d2)))[:, :, None] * d, axis=1)
+    rel = np.linalg.norm(acc - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    rel.sort()
+    assert np.all(phi < 0.0)
+    assert rel[len(rel) // 2] < 0.015
+    assert rel[int(0.99 * (len(rel) - 1))] < 0.05
+
+
+def test_eos_matches_oracle(lib, orc):
+    """The fused FP64 EOS on the device against the oracle's, over a material x density x energy grid."""
+    from astraea_amd import scenario as S
+
+    mats, rhos, us, ds = [], [], [], []
+    for mat in range(5):
+        rho0 = orc.material(mat)["rho0"] / 1e6
+        for eta in (0.05, 0.2, 0.5, 0.7, 0.9, 0.99, 1.0, 1.00001, 1.5, 3.0):
+            for u in (0.0, 0.05, 0.3, 1.0, 3.0, 10.0, 40.0):
+                for dmg in (0.0, 0.5, 1.0):
+                    mats.append(mat); rhos.append(eta * rho0); us.append(u); ds.append(dmg)
+    n = len(mats)
+    st = {"material": np.array(mats, np.uint8), "rho": np.array(rhos, np.float32),
+          "u": np.array(us, np.float32), "damage": np.array(ds, np.float32),
+          "pos_x": np.arange(n, dtype=np.float32), "mass": np.ones(n, np.float32),
+          "h": np.full(n, 0.1, np.float32), "h0": np.full(n, 0.1, np.float32)}
+    ctx = lib.Context(n)
+    ctx.set_params(S.plain_params())
+    ctx.set_layout([n])
+    ctx.load_state(st)
+    ctx.eos()
+    ctx.synchronize()
+    P, cs, T, ph = (ctx.download(k) for k in ("pressure", "sound_speed", "temp_k", "phase"))
+    ref = [orc.eos_eval(int(m), float(r), float(u), float(d)) for m, r, u, d in
+           zip(st["material"], st["rho"], st["u"], st["damage"])]
+    rp, rc, rt, rph = (np.array(v) for v in zip(*ref))
+    assert np.array_equal(ph, rph.astype(np.uint8))
+    for g, r in ((P, rp), (cs, rc), (T, rt)):
+        assert np.all(np.abs(g.astype(np.float64) - r) <= 2e-7 * np.abs(r) + 1e-30)
+
+
+def test_step_parity_one_step(lib, orc):
+    """One full KDK step from the initial force evaluation: kick, drift, sort, passes, kick, EOS."""
+    from astraea_amd import scenario as S
+
+    st = perturbed_small_state()
+    n = len(st["mass"])
+    ctx, o = run_pair(lib, orc, st, [n], S.plain_params(), [("initial_forces",)])
+    ctx.step(MAX_DT, 1)
+    o.step(MAX_DT, 1)
+    ctx.synchronize()
+    assert abs(float(ctx.dt()[0]) - float(o.dt()[0])) <= 1e-6 * float(o.dt()[0])
+    xg, xr = vec_by_id(ctx, o, ["pos_x", "pos_y", "pos_z"])
+    # positions: the FP32 state rounds every drift; compare to the FP64 trajectory at 1e-6 of the extent
+    assert np.max(np.abs(xg - xr)) < 1e-6 * np.max(np.abs(xr)) * 4
+    vg, vr = vec_by_id(ctx, o, ["vel_x", "vel_y", "vel_z"])
+    assert rel_rms(vg, vr) < 1e-6
+    for name in ("rho", "u", "du_dt", "drho_dt"):
+        g, r = by_id(ctx, o, name, f64=True)
+        assert rel_rms(g, r) < 2e-5, (name, rel_rms(g, r))
+    ag, ar = vec_by_id(ctx, o, ["acc_grav_x", "acc_grav_y", "acc_grav_z"])
+    assert rel_rms(ag, ar) < 1e-4
+
+
+def test_graph_replay_matches_eager(lib):
+    """Four steps through the captured hipGraph give bit-identical state to four eager steps."""
+    from astraea_amd import scenario as S
+
+    st = perturbed_small_state()
+    n = len(st["mass"])
+    out = []
+    for graph in (False, True):
+        ctx = lib.Context(n)
+        ctx.set_params(S.plain_params())
+        ctx.set_layout([n])
+        ctx.load_state(st)
+        ctx.initial_forces()
+        ctx.step(MAX_DT, 2, graph=graph)
+        ctx.synchronize()
+        ctx.download("pos_x")  # host work between replays
+        ctx.step(MAX_DT, 2, graph=graph)
+        ctx.synchronize()
+        out.append({k: ctx.download(k) for k in ("pos_x", "vel_y", "u", "rho", "original_id")})
+    for k in out[0]:
+        assert np.array_equal(out[0][k], out[1][k]), k
+
+
+def test_conservation_small_config(lib):
+    """engine/headless_main.cpp:263-281 gates over 60 steps of the 2 x 8,192 impact."""
+    from astraea_amd import scenario as S
+
+    st = S.config_small()
+    n = len(st["mass"])
+    ctx = lib.Context(n)
+    ctx.set_params(S.plain_params())
+    ctx.set_layout([n])
+    ctx.load_state(st)
+    ctx.initial_forces()
+    e0 = ctx.energy()[0]
+    ctx.step(MAX_DT, 60, graph=True)
+    ctx.synchronize()
+    e1 = ctx.energy()[0]
+    tot = lambda e: e[1] + e[2] + e[3] + e[10]
+    drift = (tot(e1) - tot(e0)) / max(abs(tot(e0)), 1.0)
+    m = e0[0]
+    r_sum = 6.371 * (1 + 0.1 ** (1 / 3))
+    vesc = math.sqrt(2 * 66.743 * m / r_sum)
+    dp = np.linalg.norm(e1[4:7] - e0[4:7]) / (m * vesc)
+    l0 = np.linalg.norm(e0[7:10])
+    dl = np.linalg.norm(e1[7:10] - e0[7:10]) / max(l0, m * r_sum * vesc * 1e-6)
+    assert abs(drift) < 0.01 and dp < 1e-4 and dl < 0.005, (drift, dp, dl)
+
+
+def test_neighbour_table_overflow_is_reported(lib):
+    """A neighbour table too small for the step is reported at synchronize, not silently used."""
+    from astraea_amd import scenario as S
+
+    st = S.config_small()
+    n = len(st["mass"])
+    ctx = lib.Context(n)
+    ctx.set_params(S.plain_params())
+    ctx.set_layout([n])
+    ctx.load_state(st)
+    ctx.initial_forces()
+    ctx.synchronize()
+    assert ctx.counters()["nbr_entries"] <= ctx.counters()["nbr_capacity"]
