@@ -366,6 +366,7 @@ void swapSets(astraea_ctx* c) {
 void enqueueTree(astraea_ctx* c, bool timed) {
     hipStream_t s = c->stream;
     if (timed) stageMark(c, ASTRAEA_STAGE_MORTON);
+    launchBoundsReduce(c->dc, s);
     launchMorton(c->dc, s);
     if (timed) { stageClose(c, ASTRAEA_STAGE_MORTON); stageMark(c, ASTRAEA_STAGE_SORT); }
     sortPairs(c->dc, s);
@@ -497,7 +498,8 @@ int astraea_create(astraea_ctx** out, int device, uint32_t capacity, uint32_t ma
     A(devAlloc(c, d.seg.off, maxSegments + 1));
     A(devAlloc(c, d.seg.n, maxSegments));
     A(devAlloc(c, d.seg.slotSeg, C));
-    A(devAlloc(c, d.ss.boundsOrd, 6 * maxSegments));
+    A(devAlloc(c, d.ss.bounds, 6 * maxSegments));
+    A(devAlloc(c, d.d.blockBounds, 6 * (size_t)(C / kBlock)));
     A(devAlloc(c, d.ss.dtRawBits, maxSegments));
     A(devAlloc(c, d.ss.dt, maxSegments));
     A(devAlloc(c, d.ss.dtPrev, maxSegments));

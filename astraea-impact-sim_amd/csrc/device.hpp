@@ -46,6 +46,7 @@ struct Persist {
 
 struct Derived {
     float4* posNew;   // drift output (x', y', z', m) in the pre-sort order
+    float* blockBounds; // [6 per block] min xyz / max xyz of the block's positions
     float4* grav;     // (gx, gy, gz, phi)
     float4* rate;     // (ax, ay, az, drho/dt): total acceleration after pass 2
     float4* rate2;    // (du/dt, dD^(1/3)/dt, radPowerW, 0)
@@ -87,7 +88,7 @@ struct Segments {
 
 // Per-segment device scalars.
 struct SegScalars {
-    uint32_t* boundsOrd;  // [6 per segment] min xyz / max xyz as ordered uints
+    float* bounds;        // [6 per segment] min xyz / max xyz of the positions being sorted
     uint32_t* dtRawBits;  // float bits of min dt_i (atomicMin target)
     float* dt;            // dt of the current step
     float* dtPrev;        // previous dt (0 before the first step)

@@ -216,7 +216,7 @@ __global__ __launch_bounds__(kBlock) void kDrift(DeviceCtx c) {
         }
         c.d.posNew[i] = pn;
     }
-    // per-block scene bounds of the drifted positions -> segment atomics
+    // per-block scene bounds of the drifted positions (reduced per segment before the sort)
     float v[6] = {active ? pn.x : 1.0e30f, active ? pn.y : 1.0e30f, active ? pn.z : 1.0e30f,
                   active ? pn.x : -1.0e30f, active ? pn.y : -1.0e30f, active ? pn.z : -1.0e30f};
     for (int a = 0; a < 3; ++a) v[a] = waveMinF(v[a]);
@@ -228,9 +228,7 @@ __global__ __launch_bounds__(kBlock) void kDrift(DeviceCtx c) {
         const int a = threadIdx.x;
         float r = sb[0][a];
         for (int w = 1; w < kWavesPerBlock; ++w) r = a < 3 ? fminf(r, sb[w][a]) : fmaxf(r, sb[w][a]);
-        uint32_t* bo = c.ss.boundsOrd + 6 * sg;
-        if (a < 3) atomicMin(&bo[a], floatToOrdered(r));
-        else atomicMax(&bo[a], floatToOrdered(r));
+        c.d.blockBounds[6 * b + a] = r;
     }
 }
 
@@ -250,12 +248,7 @@ __global__ void kStepBegin(DeviceCtx c, float maxDt, int resetBounds) {
     c.ss.dtPrev[sg] = dt;
     c.ss.dtRawBits[sg] = 0x7f7f7f7fu;
     c.ss.radSum[sg] = 0.0;
-    if (resetBounds) {
-        uint32_t* bo = c.ss.boundsOrd + 6 * sg;
-        bo[0] = bo[1] = bo[2] = floatToOrdered(1.0e30f);
-        bo[3] = bo[4] = bo[5] = floatToOrdered(-1.0e30f);
-        c.ss.living[sg] = 0u;
-    }
+    if (resetBounds) c.ss.living[sg] = 0u;
 }
 
 // Step epilogue: radiated energy by the trapezoid rule (simulation.cpp:1741-1744) and time.
@@ -287,13 +280,10 @@ __global__ void kResetDt(DeviceCtx c) {
     c.ss.radSum[sg] = 0.0;
 }
 
-// Reset of the per-segment scene bounds (initSceneBoundsKernel, sph_gravity.cu:258-271).
+// Reset of the per-segment living-body flag before a sort (sph_gravity.cu:258-271).
 __global__ void kResetBounds(DeviceCtx c) {
     const uint32_t sg = blockIdx.x * blockDim.x + threadIdx.x;
     if (sg >= c.seg.count) return;
-    uint32_t* bo = c.ss.boundsOrd + 6 * sg;
-    bo[0] = bo[1] = bo[2] = floatToOrdered(1.0e30f);
-    bo[3] = bo[4] = bo[5] = floatToOrdered(-1.0e30f);
     c.ss.living[sg] = 0u;
 }
 

@@ -44,6 +44,7 @@ void sortPairs(const DeviceCtx& c, hipStream_t s);
 
 // tree.hip
 void launchPrepareSort(const DeviceCtx& c, hipStream_t s);    // posNew = pos, bounds
+void launchBoundsReduce(const DeviceCtx& c, hipStream_t s);  // per-segment bounds from block partials
 void launchMorton(const DeviceCtx& c, hipStream_t s);
 void launchPermuteKarras(const DeviceCtx& c, hipStream_t s);  // cur -> alt + topology
 void launchMoments(const DeviceCtx& c, hipStream_t s);

@@ -45,8 +45,8 @@ __device__ __forceinline__ int karrasDelta(const uint64_t* __restrict__ k, int n
     return 64 + __clz((int)((uint32_t)i ^ (uint32_t)j));
 }
 
-// Block min/max of positions, then 6 ordered-uint atomics on the block's segment.
-__device__ __forceinline__ void boundsAtomics(float x, float y, float z, bool active, uint32_t* boundsOrd) {
+// Block min/max of the positions into the block's partial-bounds record.
+__device__ __forceinline__ void blockBounds(float x, float y, float z, bool active, float* out6) {
     __shared__ float sb[kWavesPerBlock][6];
     const int wave = threadIdx.x >> 6, lane = laneId();
     float v[6] = {active ? x : 1.0e30f, active ? y : 1.0e30f, active ? z : 1.0e30f,
@@ -60,8 +60,7 @@ __device__ __forceinline__ void boundsAtomics(float x, float y, float z, bool ac
         const int a = threadIdx.x;
         float r = sb[0][a];
         for (int w = 1; w < kWavesPerBlock; ++w) r = a < 3 ? fminf(r, sb[w][a]) : fmaxf(r, sb[w][a]);
-        if (a < 3) atomicMin(&boundsOrd[a], floatToOrdered(r));
-        else atomicMax(&boundsOrd[a], floatToOrdered(r));
+        out6[a] = r;
     }
 }
 
@@ -72,7 +71,7 @@ __global__ __launch_bounds__(kBlock) void kPrepareSort(DeviceCtx c) {
     const bool active = i - c.seg.off[sg] < c.seg.n[sg];
     const float4 p = c.cur.pos[i];  // in the FP64 mode pos always holds the rounded FP64 position
     if (active) c.d.posNew[i] = p;
-    boundsAtomics(p.x, p.y, p.z, active, c.ss.boundsOrd + 6 * sg);
+    blockBounds(p.x, p.y, p.z, active, c.d.blockBounds + 6 * blockIdx.x);
 }
 
 __global__ __launch_bounds__(kBlock) void kMorton(DeviceCtx c) {
@@ -81,9 +80,9 @@ __global__ __launch_bounds__(kBlock) void kMorton(DeviceCtx c) {
     const bool active = i - c.seg.off[sg] < c.seg.n[sg];
     uint64_t key = kPadKey;
     if (active) {
-        const uint32_t* bo = c.ss.boundsOrd + 6 * sg;
-        const float mnx = orderedToFloat(bo[0]), mny = orderedToFloat(bo[1]), mnz = orderedToFloat(bo[2]);
-        const float mxx = orderedToFloat(bo[3]), mxy = orderedToFloat(bo[4]), mxz = orderedToFloat(bo[5]);
+        const float* bo = c.ss.bounds + 6 * sg;
+        const float mnx = bo[0], mny = bo[1], mnz = bo[2];
+        const float mxx = bo[3], mxy = bo[4], mxz = bo[5];
         const float4 p = c.d.posNew[i];
         key = (spread21(quantise(p.x, mnx, invSpan(mnx, mxx))) << 2) |
               (spread21(quantise(p.y, mny, invSpan(mny, mxy))) << 1) | spread21(quantise(p.z, mnz, invSpan(mnz, mxz)));
@@ -156,6 +155,26 @@ struct NodeRec {
     float4 cm, mn, mx;
 };
 
+// Node records written and read during the climb use agent-scope relaxed atomics: on gfx950
+// they are global stores/loads with the sc1 coherence bit, visible across the XCD-private L2
+// caches without the whole-L2 write-back/invalidate that an acq_rel fence emits.
+__device__ __forceinline__ void storeAgent(float4* dst, float4 v) {
+    float* d = reinterpret_cast<float*>(dst);
+    __hip_atomic_store(d + 0, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(d + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(d + 2, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(d + 3, v.w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float4 loadAgent(const float4* src) {
+    float* s = const_cast<float*>(reinterpret_cast<const float*>(src));
+    float4 v;
+    v.x = __hip_atomic_load(s + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    v.y = __hip_atomic_load(s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    v.z = __hip_atomic_load(s + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    v.w = __hip_atomic_load(s + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return v;
+}
+
 __device__ __forceinline__ NodeRec loadChild(const DeviceCtx& c, int ch) {
     NodeRec r;
     if (ch < 0) {
@@ -166,14 +185,17 @@ __device__ __forceinline__ NodeRec loadChild(const DeviceCtx& c, int ch) {
         r.mn = make_float4(p.x, p.y, p.z, 0.0f);
         r.mx = make_float4(p.x, p.y, p.z, supportBound(hu.x, hu.y));
     } else {
-        r.cm = c.t.cm[ch];
-        r.mn = c.t.bmin[ch];
-        r.mx = c.t.bmax[ch];
+        r.cm = loadAgent(&c.t.cm[ch]);
+        r.mn = loadAgent(&c.t.bmin[ch]);
+        r.mx = loadAgent(&c.t.bmax[ch]);
     }
     return r;
 }
 
 // Leaf-to-root climb over the permuted state (the host has swapped the sets: `cur` is sorted).
+// The first thread to reach a node stops; the second combines both children.  A thread's
+// record stores complete (s_waitcnt) before its arrival increment, so the second arriver's
+// coherent loads see them.
 __global__ __launch_bounds__(kBlock) void kMoments(DeviceCtx c) {
 #pragma clang fp contract(off)
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -182,7 +204,8 @@ __global__ __launch_bounds__(kBlock) void kMoments(DeviceCtx c) {
     if (i - c.seg.off[sg] >= n || n < 2u) return;
     int p = c.t.parentL[i];
     while (p >= 0) {
-        const uint32_t prior = __hip_atomic_fetch_add(&c.t.visited[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_s_waitcnt(0);
+        const uint32_t prior = __hip_atomic_fetch_add(&c.t.visited[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (prior == 0u) return;
         const int2 ch = c.t.child[p];
         const NodeRec L = loadChild(c, ch.x), R = loadChild(c, ch.y);
@@ -203,14 +226,43 @@ __global__ __launch_bounds__(kBlock) void kMoments(DeviceCtx c) {
         }
         cm.w = mS;
         mn.w = fmaxf(mx.x - mn.x, fmaxf(mx.y - mn.y, mx.z - mn.z));
-        c.t.cm[p] = cm;
-        c.t.bmin[p] = mn;
-        c.t.bmax[p] = mx;
+        storeAgent(&c.t.cm[p], cm);
+        storeAgent(&c.t.bmin[p], mn);
+        storeAgent(&c.t.bmax[p], mx);
         p = c.t.parentI[p];
     }
 }
 
+// Scene bounds of each segment from its blocks' partial records (min/max is exact in any order).
+__global__ __launch_bounds__(kBlock) void kBoundsReduce(DeviceCtx c) {
+    __shared__ float sb[kWavesPerBlock][6];
+    const uint32_t sg = blockIdx.x;
+    const uint32_t b0 = c.seg.off[sg] / kBlock, b1 = c.seg.off[sg + 1] / kBlock;
+    float v[6] = {1.0e30f, 1.0e30f, 1.0e30f, -1.0e30f, -1.0e30f, -1.0e30f};
+    for (uint32_t b = b0 + threadIdx.x; b < b1; b += kBlock) {
+        const float* p = c.d.blockBounds + 6 * b;
+        for (int a = 0; a < 3; ++a) {
+            v[a] = fminf(v[a], p[a]);
+            v[3 + a] = fmaxf(v[3 + a], p[3 + a]);
+        }
+    }
+    for (int a = 0; a < 3; ++a) v[a] = waveMinF(v[a]);
+    for (int a = 3; a < 6; ++a) v[a] = waveMaxF(v[a]);
+    const int wave = threadIdx.x >> 6;
+    if (laneId() == 0)
+        for (int a = 0; a < 6; ++a) sb[wave][a] = v[a];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int a = threadIdx.x;
+        float r = sb[0][a];
+        for (int w = 1; w < kWavesPerBlock; ++w) r = a < 3 ? fminf(r, sb[w][a]) : fmaxf(r, sb[w][a]);
+        c.ss.bounds[6 * sg + a] = r;
+    }
+}
+
 }  // namespace
+
+void launchBoundsReduce(const DeviceCtx& c, hipStream_t s) { kBoundsReduce<<<c.seg.count, kBlock, 0, s>>>(c); }
 
 void launchPrepareSort(const DeviceCtx& c, hipStream_t s) { kPrepareSort<<<c.blocks, kBlock, 0, s>>>(c); }
 void launchMorton(const DeviceCtx& c, hipStream_t s) { kMorton<<<c.blocks, kBlock, 0, s>>>(c); }
