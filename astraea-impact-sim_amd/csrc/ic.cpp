@@ -176,6 +176,14 @@ int astraea_build_body(const astraea_body_spec* spec, uint32_t n, astraea_body_o
 
     const double vCell = V / (double)n;
     const double hPsu = spec->h_over_spacing * s / kLUnit;
+    // centre the selection, assign layers by radius, then shift to the centre of mass; the
+    // hydrostatic pressure is evaluated at the final radius
+    {
+        double gx = 0.0, gy = 0.0, gz = 0.0;
+        for (const P& q : pts) { gx += q.x; gy += q.y; gz += q.z; }
+        gx /= n; gy /= n; gz /= n;
+        for (P& q : pts) { q.x -= gx; q.y -= gy; q.z -= gz; q.r2 = q.x * q.x + q.y * q.y + q.z * q.z; }
+    }
     std::vector<double> mass(n);
     std::vector<int> lay(n);
     double cx = 0.0, cy = 0.0, cz = 0.0, mt = 0.0;
@@ -193,13 +201,14 @@ int astraea_build_body(const astraea_body_spec* spec, uint32_t n, astraea_body_o
     for (uint32_t i = 0; i < n; ++i) {
         const Layer& l = L[lay[i]];
         const int mat = l.mat;
-        const double r = std::sqrt(pts[i].r2);
+        const double px = pts[i].x - cx, py = pts[i].y - cy, pz = pts[i].z - cz;
+        const double r = std::sqrt(px * px + py * py + pz * pz);
         const double rhoPsu = l.rho / astraea_hip::kRhoUnit;
         const double tS = spec->surface_temp_k > 0.0 ? spec->surface_temp_k : astraea_hip::material(mat).tRef;
         const double u = energyForPressure(mat, rhoPsu, pressureAt(L, pTop, r), tS, 1.4);
-        if (out->x) out->x[i] = (float)((pts[i].x - cx) / kLUnit);
-        if (out->y) out->y[i] = (float)((pts[i].y - cy) / kLUnit);
-        if (out->z) out->z[i] = (float)((pts[i].z - cz) / kLUnit);
+        if (out->x) out->x[i] = (float)(px / kLUnit);
+        if (out->y) out->y[i] = (float)(py / kLUnit);
+        if (out->z) out->z[i] = (float)(pz / kLUnit);
         if (out->m) out->m[i] = (float)(mass[i] * scale / kMUnit);
         if (out->h) out->h[i] = (float)hPsu;
         if (out->rho) out->rho[i] = (float)rhoPsu;

@@ -105,6 +105,16 @@ def num_threads() -> int:
     return int(_lib.or_num_threads())
 
 
+def default_params(**kw) -> Params:
+    """The reference's compile-time constants (engine/sph_gravity.cu, eos.cu, simulation.cpp)."""
+    p = Params()
+    p.theta, p.soft_factor, p.cfl_factor, p.gamma_hhe = 0.5, 0.8, 0.25, 1.4
+    p.cooling_min_temp_k, p.strength_mask, p.precision = 0.0, 0x17, 0
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
 def params_from(p) -> Params:
     q = Params()
     for name, _ in Params._fields_:

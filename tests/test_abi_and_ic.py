@@ -1,0 +1,173 @@
+"""CPU checks of the C-ABI library and of the host-side initial conditions.
+
+- the shared library loads and exports every function that include/astraea_hip.h declares
+  (no device calls);
+- field sizes and the default parameters equal the reference's compile-time constants;
+- without a device, creating a context fails with a status code and a message;
+- bodies: exact particle count, mass fractions per layer, centre of mass at the origin,
+  the hydrostatic energy reproduces the analytic pressure P(r) through the EOS, determinism;
+- placement: the Keplerian approach of engine/simulation.cpp:1389-1436 (separation, speed at
+  contact from energy conservation, zero total momentum).
+"""
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_functions():
+    text = (ROOT / "include" / "astraea_hip.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(astraea_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol(lib):
+    names = declared_functions()
+    assert len(names) >= 35
+    missing = [n for n in names if not hasattr(lib._lib, n)]
+    assert not missing, missing
+    assert sorted(lib.EXPORTED_SYMBOLS) == names
+
+
+def test_field_sizes_and_defaults(lib):
+    f = lib.FIELD
+    assert lib._lib.astraea_field_size(f["pos_x"]) == 4
+    assert lib._lib.astraea_field_size(f["material"]) == 1
+    assert lib._lib.astraea_field_size(f["phase"]) == 1
+    assert lib._lib.astraea_field_size(f["pos64_x"]) == 8
+    assert lib._lib.astraea_field_size(f["original_id"]) == 4
+    assert lib._lib.astraea_field_size(999) == 0
+    p = lib.default_params()
+    # sph_gravity.cu:842 (0.8 h0), 1510 (0.25), eos.cu:93 (1.4), simulation.cpp:1729 (theta 0.5)
+    assert (p.theta, p.cfl_factor, p.strength_mask, p.precision) == (0.5, 0.25, 0x17, 0)
+    assert abs(p.soft_factor - 0.8) < 1e-7 and abs(p.gamma_hhe - 1.4) < 1e-7
+
+
+def test_context_without_device_reports_error(lib):
+    import ctypes
+
+    h = ctypes.c_void_p()
+    rc = lib._lib.astraea_create(ctypes.byref(h), 0, 1000, 1)
+    if rc == 0:
+        lib._lib.astraea_destroy(h)
+        pytest.skip("a device is present")
+    assert rc < 0
+    assert len(lib._lib.astraea_last_error()) > 0
+
+
+def test_splitmix64_known_values(lib):
+    # splitmix64 (Steele et al. 2014) reference outputs for state 0 -> first value
+    assert lib.splitmix64(0) == 0xE220A8397B1DCDAF
+    assert lib.splitmix64(42) != lib.splitmix64(43)
+
+
+@pytest.fixture(scope="module")
+def body(lib):
+    from astraea_amd import scenario as S
+
+    return lib.build_body(20000, 6.371e6, S.FE_GRANITE, seed=42)
+
+
+def test_body_count_mass_layers(body, orc):
+    n = len(body["x"])
+    assert n == 20000
+    m = body["m"].astype(np.float64)
+    fe = body["mat"] == 0
+    assert abs(m.sum() * 1e24 - body["mass_kg"]) <= 1e-5 * body["mass_kg"]
+    assert abs(m[fe].sum() / m.sum() - 0.3) < 0.01
+    # total mass of the two uniform layers: (4/3) pi R^3 (rho_c x^3 + rho_m (1 - x^3)), x^3 from the core fraction
+    rc, rm = 7800.0, 2680.0
+    x3 = 0.3 * rm / (0.7 * rc + 0.3 * rm)
+    mass = 4 / 3 * math.pi * 6.371e6 ** 3 * (rc * x3 + rm * (1 - x3))
+    assert abs(body["mass_kg"] - mass) <= 1e-9 * mass
+    com = [(m * body[k]).sum() / m.sum() for k in ("x", "y", "z")]
+    assert max(abs(c) for c in com) < 1e-5
+    r = np.sqrt(body["x"] ** 2 + body["y"] ** 2 + body["z"] ** 2)
+    assert r.max() < 6.371 * 1.02
+    # h = 1.3 * lattice spacing; HCP: n s^3 / sqrt(2) = volume
+    s = (math.sqrt(2) * 4 / 3 * math.pi * 6.371e6 ** 3 / n) ** (1 / 3)
+    assert abs(body["spacing_m"] - s) < 1e-6 * s
+    assert np.allclose(body["h"], 1.3 * s / 1e6)
+
+
+def test_body_hydrostatic_energy(body, orc):
+    """The EOS pressure of every particle equals the closed-form hydrostatic pressure at its radius."""
+    G = 6.6743e-11
+    R = 6.371e6
+    rc_, rm_ = 7800.0, 2680.0
+    x3 = 0.3 * rm_ / (0.7 * rc_ + 0.3 * rm_)
+    Rc = R * x3 ** (1 / 3)
+    Mc = 4 / 3 * math.pi * rc_ * Rc ** 3
+
+    def p_of_r(r):
+        km = 4 / 3 * math.pi * rm_
+        if r >= Rc:
+            return rm_ * G * ((Mc - km * Rc ** 3) * (1 / r - 1 / R) + 0.5 * km * (R * R - r * r))
+        pc = rm_ * G * ((Mc - km * Rc ** 3) * (1 / Rc - 1 / R) + 0.5 * km * (R * R - Rc * Rc))
+        kc = 4 / 3 * math.pi * rc_
+        return pc + rc_ * G * 0.5 * kc * (Rc * Rc - r * r)
+
+    rng = np.random.default_rng(0)
+    # the lattice is centred at the origin before the COM shift, which is below 1e-5 PSU
+    for k in rng.choice(len(body["x"]), 200, replace=False):
+        r = math.sqrt(body["x"][k] ** 2 + body["y"][k] ** 2 + body["z"][k] ** 2) * 1e6
+        mat = int(body["mat"][k])
+        if abs(r - Rc) < 2e4:
+            continue  # particles within the COM shift of the interface
+        p, _, _, _ = orc.eos_eval(mat, float(body["rho"][k]), float(body["u"][k]), 0.0)
+        exp = p_of_r(r)
+        assert abs(p * 1e12 - exp) <= 2e-5 * max(exp, 1e8) + 2e4 * 1e4, (k, p * 1e12, exp)
+
+
+def test_body_deterministic_and_seeded(lib):
+    from astraea_amd import scenario as S
+
+    a = lib.build_body(3000, 3.0e6, S.FE_GRANITE, seed=7)
+    b = lib.build_body(3000, 3.0e6, S.FE_GRANITE, seed=7)
+    c = lib.build_body(3000, 3.0e6, S.FE_GRANITE, seed=8)
+    assert all(np.array_equal(a[k], b[k]) for k in ("x", "y", "z", "m", "u"))
+    assert not np.array_equal(a["x"], c["x"])
+
+
+def test_weibull_flaws(lib):
+    from astraea_amd import scenario as S
+
+    a = lib.build_body(5000, 3.0e6, S.FE_GRANITE, seed=3, flaws=True)
+    e = a["eps_fail"].astype(np.float64)
+    v = 4 / 3 * math.pi * 3.0e6 ** 3 / 5000
+    # k V eps^m = -ln U is Exp(1): its sample mean is 1
+    t = 4.0e29 * v * e ** 9
+    assert abs(t.mean() - 1.0) < 0.08
+
+
+def test_two_body_placement(lib):
+    from astraea_amd import scenario as S
+
+    st = S.impact_pair(3000, 600, 45.0, 1.1)
+    m = st["mass"].astype(np.float64)
+    x = np.stack([st["pos_x"], st["pos_y"], st["pos_z"]], 1).astype(np.float64)
+    v = np.stack([st["vel_x"], st["vel_y"], st["vel_z"]], 1).astype(np.float64)
+    assert np.abs((m[:, None] * v).sum(0)).max() < 1e-6 * m.sum()
+    assert np.abs((m[:, None] * x).sum(0)).max() < 1e-5 * m.sum()
+    nt = 3000
+    mt, mi = m[:nt].sum(), m[nt:].sum()
+    ct = (m[:nt, None] * x[:nt]).sum(0) / mt
+    ci = (m[nt:, None] * x[nt:]).sum(0) / mi
+    vt = (m[:nt, None] * v[:nt]).sum(0) / mt
+    vi = (m[nt:, None] * v[nt:]).sum(0) / mi
+    rt, ri = 6.371, 6.371 * 0.1 ** (1 / 3)
+    rc = rt + ri
+    d0 = np.linalg.norm(ci - ct)
+    assert abs(d0 - 2.0 * rc) < 1e-4 * rc
+    vrel = np.linalg.norm(vi - vt)
+    G = 66.743
+    vesc = math.sqrt(2 * G * (mt + mi) / rc)
+    vc2 = vrel ** 2 + 2 * G * (mt + mi) * (1 / rc - 1 / d0)
+    assert abs(math.sqrt(vc2) - 1.1 * vesc) < 1e-4 * vesc
+    # angular momentum of the relative orbit: |d x v| = rc * v_contact * b, b = sin 45 deg
+    lrel = np.linalg.norm(np.cross(ci - ct, vi - vt))
+    assert abs(lrel - rc * 1.1 * vesc * math.sin(math.radians(45))) < 1e-4 * lrel
