@@ -1,0 +1,225 @@
+// headless_main.cpp — sim_headless: the reference's command-line runner (engine/headless_main.cpp:98-288)
+// on the gfx950 Simulation.  Same flags, summary line and --verify-conservation gates
+// (|dE|/|E0| < 0.01, |dP|/(M v_esc) < 1e-4, |dL|/|L0| < 0.005; exit 1 on failure or non-finite
+// telemetry, 2 on an exception).  The mutual escape speed uses the bodies' own radii (uniform
+// density per layer) instead of PlanetBuilder::solveHydrostatic1D.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/astraea/simulation.hpp"
+
+using namespace astraea;
+
+namespace {
+
+BodyInitSpec body(PlanetArchetype a, double massKg, double surfaceK, double coreK, double spin) {
+    return BodyInitSpec{a, massKg, surfaceK, coreK, spin, 80};
+}
+
+// Scenario presets of engine/presets.cpp:128-210 with the archetype defaults of presets.cpp:9-88.
+CollisionScenarioConfig preset(const std::string& name, uint32_t particles) {
+    CollisionScenarioConfig c;
+    c.presetName = name;
+    c.totalParticles = particles;
+    c.tracerGridRes = 16;
+    c.randomSeed = 42;
+    const double ME = units::M_EARTH_SI;
+    if (name == "EarthTheiaMoonForming") {
+        c.target = body(PlanetArchetype::LivingTerra, 1.0 * ME, 288.0, 5500.0, 7.292115e-5);
+        c.impactor = body(PlanetArchetype::ProtoTheia, 0.13 * ME, 240.0, 2200.0, 7.09e-5);
+        c.impactVelocityRatioVesc = 1.05; c.impactParamB = 0.72; c.initialSepRadii = 3.0;
+    } else if (name == "MercuryMantleStrip") {
+        c.target = body(PlanetArchetype::ProtoTheia, 0.45 * ME, 440.0, 2200.0, 7.09e-5);
+        c.impactor = body(PlanetArchetype::IronCore, 0.22 * ME, 440.0, 3200.0, 1.24e-6);
+        c.impactVelocityRatioVesc = 2.80; c.impactParamB = 0.35; c.initialSepRadii = 3.2;
+    } else if (name == "OceanWorldSteamPlume") {
+        c.target = body(PlanetArchetype::OceanIce, 0.75 * ME, 275.0, 3800.0, 5.0e-5);
+        c.impactor = body(PlanetArchetype::ProtoTheia, 0.15 * ME, 240.0, 2200.0, 7.09e-5);
+        c.impactVelocityRatioVesc = 1.35; c.impactParamB = 0.25; c.initialSepRadii = 3.2;
+    } else if (name == "GasGiantCoreImpact") {
+        c.target = body(PlanetArchetype::GasGiant, 20.0 * ME, 165.0, 18000.0, 1.76e-4);
+        c.impactor = body(PlanetArchetype::IronCore, 1.0 * ME, 440.0, 3200.0, 1.24e-6);
+        c.impactVelocityRatioVesc = 1.40; c.impactParamB = 0.15; c.initialSepRadii = 2.8;
+    } else if (name == "LivingGaiaAsteroidCataclysm") {
+        c.target = body(PlanetArchetype::LivingTerra, 1.0 * ME, 288.0, 5500.0, 7.292115e-5);
+        c.impactor = body(PlanetArchetype::ProtoTheia, 0.025 * ME, 240.0, 1800.0, 0.0);
+        c.impactVelocityRatioVesc = 1.65; c.impactParamB = 0.30; c.initialSepRadii = 3.2;
+    } else {
+        throw std::invalid_argument("Unknown preset name: " + name);
+    }
+    return c;
+}
+
+// Radius of a body of uniform material densities (the builder's bodies), metres.
+double bodyRadius(const BodyInitSpec& b) {
+    struct L { double frac, rho; };
+    std::vector<L> layers;
+    switch (b.archetype) {
+        case PlanetArchetype::IronCore: layers = {{0.68, 7800.0}, {0.32, 2680.0}}; break;
+        case PlanetArchetype::OceanIce: layers = {{0.15, 7800.0}, {0.45, 2680.0}, {0.40, 1000.0}}; break;
+        case PlanetArchetype::LavaIo: layers = {{0.25, 7800.0}, {0.75, 2680.0}}; break;
+        case PlanetArchetype::GasGiant: layers = {{0.05, 7800.0}, {0.10, 2680.0}, {0.85, 0.178}}; break;
+        case PlanetArchetype::ProtoTheia: layers = {{0.22, 7800.0}, {0.58, 2680.0}, {0.20, 2000.0}}; break;
+        default: layers = {{0.32, 7800.0}, {0.675, 2680.0}, {0.005, 1000.0}}; break;
+    }
+    double v = 0.0;
+    for (const L& l : layers) v += l.frac / l.rho;
+    return std::cbrt(3.0 * std::max(b.massKg, 1.0) * v / (4.0 * 3.14159265358979323846));
+}
+
+double norm3(const double v[3]) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+bool finiteSnapshot(const TelemetrySnapshot& s) {
+    const double v[] = {s.simTimeSec, (double)s.dtSec, s.kineticEnergyJ, s.potentialEnergyJ, s.internalEnergyJ,
+                        s.radiatedEnergyJ, s.totalEnergyJ, s.initialTotalEnergyJ, s.energyDriftRel,
+                        s.momentumKgMS[0], s.momentumKgMS[1], s.momentumKgMS[2], s.angularMomentumKgM2S[0],
+                        s.angularMomentumKgM2S[1], s.angularMomentumKgM2S[2], s.virialRatio, s.maxTempK,
+                        s.maxPressureGPa, s.boundMassFrac, s.diskMassFrac, s.ejectaMassFrac, s.qRatioQrd, s.gpuStepMs};
+    for (double d : v)
+        if (!std::isfinite(d)) return false;
+    return true;
+}
+
+const char* outcomeName(CollisionOutcome o) {
+    switch (o) {
+        case CollisionOutcome::Merging: return "Merging";
+        case CollisionOutcome::GrazeAndMerge: return "GrazeAndMerge";
+        case CollisionOutcome::HitAndRun: return "HitAndRun";
+        case CollisionOutcome::CatastrophicDisruption: return "CatastrophicDisruption";
+        default: return "Approaching";
+    }
+}
+
+void usage(const char* argv0) {
+    std::cout << "Usage: " << argv0
+              << " [--preset NAME] [--particles N] [--steps N] [--dt-max SEC] [--relax-steps N]\n"
+                 "       [--dump-telemetry FILE.json] [--output-csv FILE.csv] [--verify-conservation]"
+                 " [--list-presets]\n";
+}
+
+bool writeFile(const std::string& path, const std::string& text) {
+    std::ofstream f(path, std::ios::binary);
+    if (!f) return false;
+    f << text;
+    return (bool)f;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    std::string presetName = "LivingGaiaAsteroidCataclysm";
+    uint32_t particles = 100000, steps = 100;
+    double dtMaxSec = 5.0;
+    bool relaxOverride = false, verify = false;
+    uint32_t relaxSteps = 80;
+    std::string jsonPath, csvPath;
+    auto need = [&](int& i, const char* flag) -> std::string {
+        if (i + 1 >= argc) throw std::invalid_argument(std::string("Missing value for ") + flag);
+        return std::string(argv[++i]);
+    };
+    try {
+        for (int i = 1; i < argc; ++i) {
+            const std::string a = argv[i];
+            if (a == "--help" || a == "-h") { usage(argv[0]); return 0; }
+            if (a == "--list-presets") {
+                for (const char* p : {"EarthTheiaMoonForming", "MercuryMantleStrip", "OceanWorldSteamPlume",
+                                      "GasGiantCoreImpact", "LivingGaiaAsteroidCataclysm"})
+                    std::cout << p << '\n';
+                return 0;
+            }
+            if (a == "--preset") presetName = need(i, "--preset");
+            else if (a == "--particles") {
+                const long long v = std::stoll(need(i, "--particles"));
+                if (v <= 0) throw std::invalid_argument("--particles must be > 0");
+                particles = (uint32_t)v;
+            } else if (a == "--steps") {
+                const long long v = std::stoll(need(i, "--steps"));
+                if (v < 0) throw std::invalid_argument("--steps must be >= 0");
+                steps = (uint32_t)v;
+            } else if (a == "--dt-max") {
+                dtMaxSec = std::stod(need(i, "--dt-max"));
+                if (!(dtMaxSec > 0.0) || !std::isfinite(dtMaxSec)) throw std::invalid_argument("--dt-max must be > 0");
+            } else if (a == "--relax-steps") {
+                const long long v = std::stoll(need(i, "--relax-steps"));
+                if (v < 0) throw std::invalid_argument("--relax-steps must be >= 0");
+                relaxOverride = true;
+                relaxSteps = (uint32_t)v;
+            } else if (a == "--dump-telemetry") jsonPath = need(i, "--dump-telemetry");
+            else if (a == "--output-csv") csvPath = need(i, "--output-csv");
+            else if (a == "--verify-conservation") verify = true;
+            else throw std::invalid_argument("Unknown option: " + a);
+        }
+        CollisionScenarioConfig cfg = preset(presetName, particles);
+        if (relaxOverride) cfg.target.relaxSteps = cfg.impactor.relaxSteps = relaxSteps;
+        const double mTot = cfg.target.massKg + std::max(0.0, cfg.impactor.massKg);
+        const double rScale = std::max(bodyRadius(cfg.target) + (cfg.impactor.massKg > 0 ? bodyRadius(cfg.impactor) : 0.0), 1.0);
+        const double vEsc = std::sqrt(2.0 * units::G_SI * mTot / rScale);
+
+        Simulation sim(particles);
+        sim.loadScenario(cfg);
+        const TelemetrySnapshot t0 = sim.computeTelemetry();
+        if (!finiteSnapshot(t0)) { std::cerr << "ERROR: Non-finite initial telemetry at t=0\n"; return 1; }
+        std::vector<double> ms;
+        TelemetrySnapshot last = t0;
+        for (uint32_t s = 0; s < steps; ++s) {
+            last = sim.step(dtMaxSec);
+            ms.push_back(last.gpuStepMs);
+            if (!finiteSnapshot(last)) { std::cerr << "ERROR: Non-finite telemetry at step " << (s + 1) << '\n'; return 1; }
+        }
+        double med = 0.0;
+        if (!ms.empty()) {
+            std::sort(ms.begin(), ms.end());
+            const size_t k = ms.size() / 2;
+            med = ms.size() % 2 == 0 ? 0.5 * (ms[k - 1] + ms[k]) : ms[k];
+        }
+        const double dP[3] = {last.momentumKgMS[0] - t0.momentumKgMS[0], last.momentumKgMS[1] - t0.momentumKgMS[1],
+                              last.momentumKgMS[2] - t0.momentumKgMS[2]};
+        const double dL[3] = {last.angularMomentumKgM2S[0] - t0.angularMomentumKgM2S[0],
+                              last.angularMomentumKgM2S[1] - t0.angularMomentumKgM2S[1],
+                              last.angularMomentumKgM2S[2] - t0.angularMomentumKgM2S[2]};
+        const double pRel = mTot * vEsc > 0.0 ? norm3(dP) / (mTot * vEsc) : 0.0;
+        const double lRel = norm3(dL) / std::max(norm3(t0.angularMomentumKgM2S), mTot * rScale * vEsc * 1.0e-6);
+        if (!jsonPath.empty() && !writeFile(jsonPath, sim.exportTelemetryJson())) {
+            std::cerr << "ERROR: Failed to write JSON telemetry to " << jsonPath << '\n';
+            return 1;
+        }
+        if (!csvPath.empty() && !writeFile(csvPath, sim.exportTelemetryCsv())) {
+            std::cerr << "ERROR: Failed to write CSV telemetry to " << csvPath << '\n';
+            return 1;
+        }
+        std::cout << std::setprecision(6) << "preset=" << cfg.presetName << " particles=" << last.particleCount
+                  << " tracers=" << last.tracerCount << " steps=" << steps << " simTimeSec=" << last.simTimeSec
+                  << " dtSec=" << last.dtSec << " medianGpuStepMs=" << med << " energyDriftRel=" << last.energyDriftRel
+                  << " dPRel=" << pRel << " dLRel=" << lRel << " maxTempK=" << last.maxTempK
+                  << " maxPressureGPa=" << last.maxPressureGPa << " boundMassFrac=" << last.boundMassFrac
+                  << " diskMassFrac=" << last.diskMassFrac << " ejectaMassFrac=" << last.ejectaMassFrac
+                  << " qRatioQrd=" << last.qRatioQrd << " outcome=" << outcomeName(last.outcome) << '\n';
+        if (verify) {
+            bool ok = true;
+            if (!(std::abs(last.energyDriftRel) < 0.01)) {
+                std::cerr << "CONSERVATION FAIL: |energyDriftRel| = " << std::abs(last.energyDriftRel) << " >= 0.01\n";
+                ok = false;
+            }
+            if (!(pRel < 1.0e-4)) {
+                std::cerr << "CONSERVATION FAIL: |dP|/(M*v_esc) = " << pRel << " >= 1e-4\n";
+                ok = false;
+            }
+            if (!(lRel < 0.005)) {
+                std::cerr << "CONSERVATION FAIL: |dL|/|L0| = " << lRel << " >= 0.005\n";
+                ok = false;
+            }
+            if (!ok) return 1;
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        std::cerr << "ERROR: " << e.what() << '\n';
+        return 2;
+    }
+}

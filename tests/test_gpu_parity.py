@@ -363,3 +363,36 @@ def test_neighbour_table_overflow_is_reported(lib):
     ctx.initial_forces()
     ctx.synchronize()
     assert ctx.counters()["nbr_entries"] <= ctx.counters()["nbr_capacity"]
+
+
+def test_sim_headless_verify_conservation():
+    """tests/cpp/test_conservation.cpp:418-477 (AC10): the real runner with --verify-conservation exits 0
+    and writes its telemetry exports."""
+    import json
+    import subprocess
+    import tempfile
+    from pathlib import Path
+
+    exe = Path(__file__).resolve().parents[1] / "astraea-impact-sim_amd" / "astraea_amd" / "sim_headless"
+    with tempfile.TemporaryDirectory() as d:
+        js, csv = Path(d) / "t.json", Path(d) / "t.csv"
+        r = subprocess.run([str(exe), "--particles", "20000", "--steps", "20", "--verify-conservation",
+                            "--dump-telemetry", str(js), "--output-csv", str(csv)],
+                           capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout + r.stderr
+        assert "preset=LivingGaiaAsteroidCataclysm particles=20000" in r.stdout
+        doc = json.loads(js.read_text())
+        assert len(doc["history"]) == 20 and doc["particleCount"] == 20000
+        assert csv.read_text().count("\n") == 21
+
+
+def test_simulation_errors_match_reference():
+    """Unknown preset -> exit 2 (exception); zero particles rejected (simulation.cpp:1296-1298)."""
+    import subprocess
+    from pathlib import Path
+
+    exe = Path(__file__).resolve().parents[1] / "astraea-impact-sim_amd" / "astraea_amd" / "sim_headless"
+    r = subprocess.run([str(exe), "--preset", "NoSuchPreset"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "Unknown preset" in r.stderr
+    r = subprocess.run([str(exe), "--particles", "0"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2
