@@ -140,6 +140,7 @@ _sig = {
     "astraea_get_dt": (C.c_int, [_P, C.POINTER(C.c_float)]),
     "astraea_energy": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "astraea_counters": (C.c_int, [_P, C.POINTER(C.c_uint64), C.c_int]),
+    "astraea_set_neighbour_stride": (C.c_int, [_P, C.c_uint32]),
     "astraea_download_tree": (C.c_int, [_P, C.c_uint32] + [_P] * 9),
     "astraea_enable_stage_timing": (C.c_int, [_P, C.c_int]),
     "astraea_stage_times": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
@@ -293,10 +294,15 @@ class Context:
         return out.reshape(len(self.counts), ENERGY_STRIDE)
 
     def counters(self) -> dict:
-        out = (C.c_uint64 * 6)()
-        _check(_lib.astraea_counters(self._h, out, 6))
-        keys = ["interactions", "forced_accepts", "nbr_entries", "nbr_capacity", "pass1_pairs", "pass2_pairs"]
+        out = (C.c_uint64 * 7)()
+        _check(_lib.astraea_counters(self._h, out, 7))
+        keys = ["interactions", "forced_accepts", "nbr_entries", "nbr_stride", "pass1_pairs", "pass2_pairs",
+                "nbr_max"]
         return dict(zip(keys, [int(v) for v in out]))
+
+    def set_neighbour_stride(self, stride: int) -> None:
+        """Width of the neighbour table in entries per particle; the table must be rebuilt after."""
+        _check(_lib.astraea_set_neighbour_stride(self._h, stride))
 
     def tree(self, seg: int = 0) -> dict:
         n = self.counts[seg]

@@ -79,8 +79,6 @@ struct astraea_ctx {
     int openEv[ASTRAEA_STAGE_COUNT_] = {};
     double stageMs[ASTRAEA_STAGE_COUNT_] = {};
     uint64_t stageN[ASTRAEA_STAGE_COUNT_] = {};
-    // counters
-    uint64_t nbrCapacity = 0;
 };
 
 namespace {
@@ -123,16 +121,17 @@ void freeGraph(astraea_ctx* c) {
     c->graphParity = -1;
 }
 
-int allocNeighbourList(astraea_ctx* c, uint64_t entries) {
+constexpr uint32_t kMaxNeighbourStride = 1u << 16;
+
+int allocNeighbourList(astraea_ctx* c, uint32_t stride) {
     if (c->dc.nb.list) {
         (void)hipFree(c->dc.nb.list);
         auto it = std::find(c->allocs.begin(), c->allocs.end(), (void*)c->dc.nb.list);
         if (it != c->allocs.end()) c->allocs.erase(it);
         c->dc.nb.list = nullptr;
     }
-    TRY(devAlloc(c, c->dc.nb.list, entries));
-    c->dc.nb.capacity = entries;
-    c->nbrCapacity = entries;
+    TRY(devAlloc(c, c->dc.nb.list, (size_t)stride * c->capSlots));
+    c->dc.nb.stride = stride;
     freeGraph(c);
     return ASTRAEA_OK;
 }
@@ -493,8 +492,8 @@ int astraea_create(astraea_ctx** out, int device, uint32_t capacity, uint32_t ma
     A(devAlloc(c, d.t.bmin, C));
     A(devAlloc(c, d.t.bmax, C));
     A(devAlloc(c, d.nb.count, C));
+    A(devAlloc(c, d.nb.waveSum, C / kWave));
     A(devAlloc(c, d.nb.waveMax, C / kWave));
-    A(devAlloc(c, d.nb.waveOff, C / kWave));
     A(devAlloc(c, d.seg.off, maxSegments + 1));
     A(devAlloc(c, d.seg.n, maxSegments));
     A(devAlloc(c, d.seg.slotSeg, C));
@@ -519,7 +518,7 @@ int astraea_create(astraea_ctx** out, int device, uint32_t capacity, uint32_t ma
     A(devAlloc(c, d.sb.digitTotal, 256));
     d.sb.tilesCap = tiles;
     A(devAlloc(c, c->dEnergy, (size_t)ASTRAEA_ENERGY_STRIDE * maxSegments));
-    if (r == ASTRAEA_OK) r = allocNeighbourList(c, (uint64_t)C * 256ull);
+    if (r == ASTRAEA_OK) r = allocNeighbourList(c, ASTRAEA_DEFAULT_NEIGHBOUR_STRIDE);
     if (r != ASTRAEA_OK) return cleanup(r);
     d.stride64 = C;
     if (hipMemsetAsync(d.cnt.v, 0, 8 * sizeof(unsigned long long), c->stream) != hipSuccess)
@@ -636,8 +635,8 @@ int astraea_set_layout(astraea_ctx* c, uint32_t ns, const uint32_t* counts) {
     HIP_TRY(hipMemset(d.d.sr1, 0, sizeof(float2) * c->slots));
     HIP_TRY(hipMemset(d.d.phase, 0, c->slots));
     HIP_TRY(hipMemset(d.nb.count, 0, sizeof(uint32_t) * c->slots));
+    HIP_TRY(hipMemset(d.nb.waveSum, 0, sizeof(uint32_t) * (c->slots / kWave)));
     HIP_TRY(hipMemset(d.nb.waveMax, 0, sizeof(uint32_t) * (c->slots / kWave)));
-    HIP_TRY(hipMemset(d.nb.waveOff, 0, sizeof(uint64_t) * (c->slots / kWave)));
     HIP_TRY(hipMemset(d.ss.dtPrev, 0, sizeof(float) * ns));
     HIP_TRY(hipMemset(d.ss.dt, 0, sizeof(float) * ns));
     HIP_TRY(hipMemset(d.ss.erad, 0, sizeof(double) * ns));
@@ -681,6 +680,26 @@ int astraea_clear_stats(astraea_ctx* c) {
     return checkLaunch(c);
 }
 
+// Waits for the tree just enqueued; when a particle has more neighbour candidates than the table
+// holds, widens the table and rebuilds it from the same tree.  Used where the caller is not
+// inside a run of steps (a load or an explicit tree build), so no step is ever lost to it.
+static int fitNeighbourTable(astraea_ctx* c) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    unsigned long long v[8];
+    HIP_TRY(hipMemcpy(v, c->dc.cnt.v, sizeof(v), hipMemcpyDeviceToHost));
+    if (v[3] == 0ull) return ASTRAEA_OK;
+    const uint64_t need = v[7];
+    if (need > kMaxNeighbourStride)
+        return fail(ASTRAEA_E_OVERFLOW, "a particle has %llu neighbour candidates, more than the table can hold (%u)",
+                    (unsigned long long)need, kMaxNeighbourStride);
+    HIP_TRY(hipMemset(c->dc.cnt.v + 3, 0, sizeof(unsigned long long)));
+    HIP_TRY(hipMemset(c->dc.cnt.v + 7, 0, sizeof(unsigned long long)));
+    const uint64_t want = std::min<uint64_t>(need + need / 4, kMaxNeighbourStride);
+    TRY(allocNeighbourList(c, std::max<uint32_t>(roundUp((uint32_t)want, 64), c->dc.nb.stride)));
+    launchNeighbours(c->dc, c->stream);
+    return ASTRAEA_OK;
+}
+
 int astraea_sort_and_build_bvh(astraea_ctx* c) {
     TRY(requireLoaded(c));
     if (!c->posNewPending) {
@@ -688,6 +707,8 @@ int astraea_sort_and_build_bvh(astraea_ctx* c) {
         launchPrepareSort(c->dc, c->stream);
     }
     enqueueTree(c, false);
+    TRY(checkLaunch(c));
+    TRY(fitNeighbourTable(c));
     c->dtRawValid = false;
     return checkLaunch(c);
 }
@@ -761,6 +782,8 @@ int astraea_evaluate_initial_forces(astraea_ctx* c) {
     launchResetBounds(c->dc, s);
     launchPrepareSort(c->dc, s);
     enqueueTree(c, false);
+    TRY(checkLaunch(c));
+    TRY(fitNeighbourTable(c));
     launchPass1(c->dc, s);
     launchGravity(c->dc, s);
     launchPass2(c->dc, s);
@@ -820,19 +843,21 @@ int astraea_synchronize(astraea_ctx* c) {
     unsigned long long v[8];
     HIP_TRY(hipMemcpy(v, c->dc.cnt.v, sizeof(v), hipMemcpyDeviceToHost));
     if (v[3] != 0ull || v[6] != 0ull) {
-        const unsigned long long need = v[2];
-        // clear the sticky flags and grow the table for the following steps
+        // clear the sticky flags; a table too narrow is widened for the following steps
         HIP_TRY(hipMemset(c->dc.cnt.v + 3, 0, sizeof(unsigned long long)));
-        HIP_TRY(hipMemset(c->dc.cnt.v + 6, 0, sizeof(unsigned long long)));
+        HIP_TRY(hipMemset(c->dc.cnt.v + 6, 0, 2 * sizeof(unsigned long long)));
         if (v[3] != 0ull) {
-            const uint64_t grow = std::max<uint64_t>(need + need / 4, 2 * c->nbrCapacity);
-            TRY(allocNeighbourList(c, grow));
+            const uint32_t was = c->dc.nb.stride;
+            const uint64_t need = v[7];
+            const uint64_t want = std::min<uint64_t>(need + need / 4, kMaxNeighbourStride);
+            TRY(allocNeighbourList(c, std::max<uint32_t>(roundUp((uint32_t)want, 64), was)));
             c->listValid = false;
             return fail(ASTRAEA_E_OVERFLOW,
-                        "neighbour table overflow: %llu entries needed, capacity was %llu (grown to %llu; the "
-                        "affected step is invalid)", need, (unsigned long long)c->nbrCapacity, (unsigned long long)grow);
+                        "neighbour table overflow: a particle has %llu neighbour candidates, the table holds %u per "
+                        "particle (widened to %u; the affected step is invalid)",
+                        (unsigned long long)need, was, c->dc.nb.stride);
         }
-        return fail(ASTRAEA_E_OVERFLOW, "neighbour traversal stack overflow (depth %d)", ASTRAEA_STACK_DEPTH);
+        return fail(ASTRAEA_E_OVERFLOW, "neighbour traversal stack overflow");
     }
     return ASTRAEA_OK;
 }
@@ -868,13 +893,34 @@ int astraea_energy(astraea_ctx* c, double* out) {
     return ASTRAEA_OK;
 }
 
+int astraea_set_neighbour_stride(astraea_ctx* c, uint32_t stride) {
+    if (!c) return fail(ASTRAEA_E_INVALID, "null context");
+    if (stride == 0 || stride > kMaxNeighbourStride)
+        return fail(ASTRAEA_E_INVALID, "neighbour stride %u outside [1, %u]", stride, kMaxNeighbourStride);
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    TRY(allocNeighbourList(c, stride));
+    c->listValid = false;
+    return ASTRAEA_OK;
+}
+
 int astraea_counters(astraea_ctx* c, uint64_t* out, int n) {
     TRY(requireLoaded(c));
     if (!out || n <= 0) return fail(ASTRAEA_E_INVALID, "null output");
     HIP_TRY(hipStreamSynchronize(c->stream));
     unsigned long long v[8];
     HIP_TRY(hipMemcpy(v, c->dc.cnt.v, sizeof(v), hipMemcpyDeviceToHost));
-    uint64_t vals[6] = {v[0], v[1], v[2], c->nbrCapacity, 0, 0};
+    uint64_t vals[7] = {v[0], v[1], 0, c->dc.nb.stride, 0, 0, 0};
+    {
+        const uint32_t waves = c->slots / kWave;
+        std::vector<uint32_t> ws(waves), wm(waves);
+        HIP_TRY(hipMemcpy(ws.data(), c->dc.nb.waveSum, sizeof(uint32_t) * waves, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(wm.data(), c->dc.nb.waveMax, sizeof(uint32_t) * waves, hipMemcpyDeviceToHost));
+        for (uint32_t w = 0; w < waves; ++w) {
+            vals[2] += ws[w];
+            vals[6] = std::max<uint64_t>(vals[6], wm[w]);
+        }
+    }
     if (n > 4) {
         std::vector<float4> a(c->slots), b(c->slots);
         HIP_TRY(hipMemcpy(a.data(), c->dc.d.p1a, sizeof(float4) * c->slots, hipMemcpyDeviceToHost));
@@ -884,7 +930,7 @@ int astraea_counters(astraea_ctx* c, uint64_t* out, int n) {
             vals[5] += (uint64_t)b[s].w;
         });
     }
-    for (int k = 0; k < n && k < 6; ++k) out[k] = vals[k];
+    for (int k = 0; k < n && k < 7; ++k) out[k] = vals[k];
     // interaction and forced-acceptance counts restart from zero
     HIP_TRY(hipMemset(c->dc.cnt.v, 0, 2 * sizeof(unsigned long long)));
     return ASTRAEA_OK;

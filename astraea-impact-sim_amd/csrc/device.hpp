@@ -72,12 +72,17 @@ struct Tree {
 };
 
 struct Neighbours {
-    uint32_t* count;      // entries of each slot
-    uint32_t* waveMax;    // max count in each wavefront
-    uint64_t* waveOff;    // start of each wavefront's block (entries), exclusive scan of 64*waveMax
-    uint32_t* list;       // entry k of slot (w*64 + l) at waveOff[w] + k*64 + l
-    uint64_t capacity;    // entries allocated
+    uint32_t* count;      // entries of each slot (at most stride)
+    uint32_t* waveSum;    // entries wanted by each wavefront's particles (uncapped)
+    uint32_t* waveMax;    // largest entry count one particle of the wavefront wanted
+    uint32_t* list;       // entry k of slot (w*64 + l) at w*64*stride + k*64 + l
+    uint32_t stride;      // entries per slot
 };
+
+// First table entry of slot i; entry k is at 64*k past it.
+__device__ __forceinline__ const uint32_t* tableRow(const Neighbours& nb, uint32_t i) {
+    return nb.list + (uint64_t)(i >> 6) * (64ull * nb.stride) + (i & 63u);
+}
 
 struct Segments {
     uint32_t count;       // number of segments
@@ -100,8 +105,8 @@ struct SegScalars {
 };
 
 struct Counters {
-    unsigned long long* v;  // [8]: 0 interactions, 1 stack overflows (gravity), 2 nbr entries,
-                            // 3 nbr overflow flag, 4 pass-1 pairs, 5 pass-2 pairs, 6 nbr stack overflow
+    unsigned long long* v;  // [8]: 0 interactions, 1 stack overflows (gravity), 3 nbr overflow flag,
+                            // 6 nbr stack overflow, 7 largest per-slot entry count wanted (sticky max)
 };
 
 struct Params {
@@ -121,6 +126,9 @@ __device__ __forceinline__ int laneId() { return __builtin_amdgcn_mbcnt_hi(~0u, 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 __device__ __forceinline__ int uniformInt(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ uint32_t uniformU32(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ float readLaneF(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
 
 // Logical block index that keeps consecutive blocks on one XCD: the dispatcher
 // places block b on XCD b % 8, so XCD x receives a contiguous range of logical

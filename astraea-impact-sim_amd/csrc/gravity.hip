@@ -9,41 +9,34 @@
 // The group is the 64-lane wavefront (the reference's is the 32-lane warp).
 //
 // The reference pops one node per iteration, so each wave waits on two dependent loads per node.
-// Here the wave expands up to 64 nodes of its LDS stack at once, one per lane: each lane loads its
-// node's children and decides the group opening test for them.  Most decisions come from the
-// group's bounding sphere (centre c, radius R): every particle's distance to a node's centre of
-// mass lies in [d - R, d + R], d = |c - com|, and with a relative margin far above FP32 rounding
-// the per-particle FP32 test is then known to accept (or reject) for all particles; only the
-// remaining nodes run the exact per-particle test, reading the group's positions from LDS.  The
-// interaction set is exactly the reference's.  Accepted nodes and leaves go to an LDS list that
-// the whole wave then evaluates, four records in flight at a time.
+// Here the wave expands up to 64 stack nodes of its LDS stack at once, one per lane: each lane
+// loads its node's children and decides the group opening test for them.  Most decisions come
+// from bounds on the distance between the node's centre of mass and the group's particles (the
+// group's bounding sphere and bounding box): with a relative margin far above FP32 rounding the
+// per-particle FP32 test is then known to accept (or reject) for all particles.  The remaining
+// nodes are tested exactly, one node at a time across the lanes (lane = particle), with the node
+// broadcast from the lane that loaded it.  The interaction set is exactly the reference's.
+//
+// Accepted nodes and leaves are staged in LDS as ready records (x, y, z, G m), loaded by their
+// lanes in parallel; the wave then evaluates them with broadcast LDS reads.  A particle's own
+// leaf is evaluated like the others (zero separation: no force) and its potential term is taken
+// back at the end, which keeps the per-pair test out of the inner loop.
 #include "launch.hpp"
 
 namespace astraea_hip {
 namespace {
 
-constexpr int kSCap = 1024;  // per-wave node stack
-constexpr int kLCap = 512;   // per-wave interaction list
-constexpr int kFlushAt = kLCap - 256;
+constexpr int kSCap = 1024;                 // per-wave node stack
+constexpr int kFlushAt = 32;                // evaluate once either list holds more than this
+constexpr int kRCap = kFlushAt + 2 * kWave; // one round lists at most 128 records
 
 struct WaveLds {
     int stack[kSCap];
-    int list[kLCap];
-    float4 gp[kWave];  // group particle positions, w = 1 when active
+    float4 node[kRCap];     // accepted node: (com x, y, z, G m)
+    float4 leaf[kRCap];     // leaf: (x, y, z, G m)
+    float leafEps[kRCap];   // eps_j / 2
+    uint32_t self[kWave];   // lane l's own leaf was listed
 };
-
-__device__ __forceinline__ void interact(float px, float py, float pz, float4 q, float e2, float& ax, float& ay, float& az,
-                                         float& phi) {
-    const float dx = px - q.x, dy = py - q.y, dz = pz - q.z;
-    const float d2 = dx * dx + dy * dy + dz * dz + e2 + 1.0e-20f;
-    const float ir = rsqrtf(d2);
-    const float gm = kGf * q.w;
-    const float f = -gm * (ir * ir * ir);
-    ax += f * dx;
-    ay += f * dy;
-    az += f * dz;
-    phi -= gm * ir;
-}
 
 template <bool CHECK_INSIDE>
 __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
@@ -71,31 +64,32 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
         pi = c.cur.pos[i];
         epsI = soft * fmaxf(c.cur.hu[i].y, 1.0e-6f);
     }
-    const float epsI2 = epsI * epsI;
-    L.gp[lane] = make_float4(pi.x, pi.y, pi.z, active ? 1.0f : 0.0f);
+    const float halfEpsI = 0.5f * epsI;
+    const float e2Node = __builtin_fmaf(epsI, epsI, 1.0e-20f);
+    L.self[lane] = 0u;
 
-    // group bounding sphere over the active particles (centre of the box, radius of the farthest)
-    float cx, cy, cz, R;
+    // group bounds over the active particles: box [l, h], sphere (centre of the box, radius R)
+    const float lx = waveMinF(active ? pi.x : 3.0e38f), hx = waveMaxF(active ? pi.x : -3.0e38f);
+    const float ly = waveMinF(active ? pi.y : 3.0e38f), hy = waveMaxF(active ? pi.y : -3.0e38f);
+    const float lz = waveMinF(active ? pi.z : 3.0e38f), hz = waveMaxF(active ? pi.z : -3.0e38f);
+    const float cx = 0.5f * (lx + hx), cy = 0.5f * (ly + hy), cz = 0.5f * (lz + hz);
+    float R;
     {
-        const float lx = waveMinF(active ? pi.x : 3.0e38f), hx = waveMaxF(active ? pi.x : -3.0e38f);
-        const float ly = waveMinF(active ? pi.y : 3.0e38f), hy = waveMaxF(active ? pi.y : -3.0e38f);
-        const float lz = waveMinF(active ? pi.z : 3.0e38f), hz = waveMaxF(active ? pi.z : -3.0e38f);
-        cx = 0.5f * (lx + hx); cy = 0.5f * (ly + hy); cz = 0.5f * (lz + hz);
         const float dx = pi.x - cx, dy = pi.y - cy, dz = pi.z - cz;
         R = waveMaxF(active ? sqrtf(dx * dx + dy * dy + dz * dz) : 0.0f);
         R = R * (1.0f + 1.0e-5f) + 1.0e-30f;
     }
 
     float ax = 0.0f, ay = 0.0f, az = 0.0f, phi = 0.0f;
-    uint32_t listed = 0, selfVisits = 0, forced = 0;
+    uint32_t listed = 0, forced = 0;
     const uint64_t lt = (1ull << lane) - 1ull;
 
-    int sp = 1, nl = 0;
+    int sp = 1, nn = 0, nf = 0;
     L.stack[0] = (int)off;
     __builtin_amdgcn_wave_barrier();
     for (;;) {
         // ---- expand: up to 64 stack nodes per round, one per lane ----
-        while (sp > 0 && nl <= kFlushAt) {
+        while (sp > 0 && nn <= kFlushAt && nf <= kFlushAt) {
             const int k = sp < kWave ? sp : kWave;
             sp -= k;
             const bool has = lane < k;
@@ -107,110 +101,130 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
                 const int cc = side == 0 ? ch.x : ch.y;
                 const bool isLeaf = has && cc < 0;
                 const bool isInt = has && cc >= 0;
-                bool accept = false;
+                float4 cm = make_float4(0.0f, 0.0f, 0.0f, 0.0f), mn = cm, mx = cm;
+                float4 rec = cm;
+                float leafHalfEps = 0.0f;
+                bool accept = false, undecided = false;
                 if (isInt) {
-                    const float4 cm = c.t.cm[cc];
-                    const float4 mn = c.t.bmin[cc];
+                    cm = c.t.cm[cc];
+                    mn = c.t.bmin[cc];
+                    if (CHECK_INSIDE) mx = c.t.bmax[cc];
                     const float ext = mn.w;
                     const float ddx = cx - cm.x, ddy = cy - cm.y, ddz = cz - cm.z;
                     const float d = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
+                    // distance from the com to the group box, and to its farthest corner
+                    const float gx = fmaxf(0.0f, fmaxf(lx - cm.x, cm.x - hx));
+                    const float gy = fmaxf(0.0f, fmaxf(ly - cm.y, cm.y - hy));
+                    const float gz = fmaxf(0.0f, fmaxf(lz - cm.z, cm.z - hz));
+                    const float fx = fmaxf(fabsf(cm.x - lx), fabsf(cm.x - hx));
+                    const float fy = fmaxf(fabsf(cm.y - ly), fabsf(cm.y - hy));
+                    const float fz = fmaxf(fabsf(cm.z - lz), fabsf(cm.z - hz));
+                    const float dBox = sqrtf(gx * gx + gy * gy + gz * gz);
+                    const float dFar = sqrtf(fx * fx + fy * fy + fz * fz);
                     const float slack = 2.0e-6f * (d + R) + 1.0e-30f;
-                    const float dLo = d - R - slack, dHi = d + R + slack;
-                    int decided = 0;  // 1 accept, 2 reject, 0 exact test
-                    if (!CHECK_INSIDE && dLo > 1.0e-9f && theta * dLo > ext * (1.0f + 1.0e-5f)) decided = 1;
-                    else if (theta * dHi < ext * (1.0f - 1.0e-5f)) decided = 2;
-                    if (decided == 1) {
-                        accept = true;
-                    } else if (decided == 0) {
-                        float4 mx = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                        if (CHECK_INSIDE) mx = c.t.bmax[cc];
-                        accept = true;
-                        for (int j = 0; j < kWave && accept; ++j) {
-                            const float4 g = L.gp[j];
-                            if (g.w == 0.0f) continue;
-                            bool a;
-                            {
-#pragma clang fp contract(off)
-                                const float r2 = dist2f(g.x - cm.x, g.y - cm.y, g.z - cm.z);
-                                const float lhs = ext * ext;
-                                const float rhs = theta2 * r2;
-                                a = (lhs < rhs) && (r2 > 1.0e-20f);
-                            }
-                            if (CHECK_INSIDE && a && g.x >= mn.x && g.x <= mx.x && g.y >= mn.y && g.y <= mx.y &&
-                                g.z >= mn.z && g.z <= mx.z)
-                                a = false;
-                            accept = a;
-                        }
-                    }
+                    const float dLo = fmaxf(d - R, dBox) - slack;
+                    const float dHi = fminf(d + R, dFar) + slack;
+                    bool apart = true;  // no group particle can lie inside the node's box
+                    if (CHECK_INSIDE)
+                        apart = hx < mn.x || lx > mx.x || hy < mn.y || ly > mx.y || hz < mn.z || lz > mx.z;
+                    if (apart && dLo > 1.0e-9f && theta * dLo > ext * (1.0f + 1.0e-5f)) accept = true;
+                    else if (!(theta * dHi < ext * (1.0f - 1.0e-5f))) undecided = true;
+                    rec = make_float4(cm.x, cm.y, cm.z, kGf * cm.w);
+                } else if (isLeaf) {
+                    const uint32_t j = (uint32_t)~cc;
+                    const float4 pj = c.cur.pos[j];
+                    rec = make_float4(pj.x, pj.y, pj.z, kGf * pj.w);
+                    leafHalfEps = 0.5f * (soft * fmaxf(c.cur.hu[j].y, 1.0e-6f));
+                    const uint32_t rel = j - waveBase;
+                    if (rel < 64u && ((am >> rel) & 1ull)) L.self[rel] = 1u;
                 }
-                const bool toList = isLeaf || (isInt && accept);
+                // exact group test of the undecided nodes: lane = particle
+                uint64_t um = ballot(undecided);
+                while (um != 0ull) {
+                    const int src = (int)__builtin_ctzll(um);
+                    um &= um - 1ull;
+                    const float qx = readLaneF(cm.x, src), qy = readLaneF(cm.y, src), qz = readLaneF(cm.z, src);
+                    const float ext = readLaneF(mn.w, src);
+                    bool a;
+                    {
+#pragma clang fp contract(off)
+                        const float r2 = dist2f(pi.x - qx, pi.y - qy, pi.z - qz);
+                        const float lhs = ext * ext;
+                        const float rhs = theta2 * r2;
+                        a = (lhs < rhs) && (r2 > 1.0e-20f);
+                    }
+                    if (CHECK_INSIDE && a) {
+                        const float x0 = readLaneF(mn.x, src), y0 = readLaneF(mn.y, src), z0 = readLaneF(mn.z, src);
+                        const float x1 = readLaneF(mx.x, src), y1 = readLaneF(mx.y, src), z1 = readLaneF(mx.z, src);
+                        if (pi.x >= x0 && pi.x <= x1 && pi.y >= y0 && pi.y <= y1 && pi.z >= z0 && pi.z <= z1) a = false;
+                    }
+                    const bool all = ballot(active && !a) == 0ull;
+                    if (lane == src) accept = all;
+                }
                 bool toPush = isInt && !accept;
                 const uint64_t mPush = ballot(toPush);
                 const int nPush = __popcll(mPush);
-                int pushAt = sp + __popcll(mPush & lt);
+                const int pushAt = sp + __popcll(mPush & lt);
                 if (toPush && pushAt >= kSCap) toPush = false;  // forced acceptance (reference: full stack)
                 const bool forcedHere = isInt && !accept && !toPush;
-                const bool listIt = toList || forcedHere;
-                const uint64_t mList = ballot(listIt);
-                if (listIt) L.list[nl + __popcll(mList & lt)] = cc;
+                const bool asNode = isInt && (accept || forcedHere);
+                const uint64_t mNode = ballot(asNode), mLeaf = ballot(isLeaf);
+                if (asNode) L.node[nn + __popcll(mNode & lt)] = rec;
+                if (isLeaf) {
+                    const int at = nf + __popcll(mLeaf & lt);
+                    L.leaf[at] = rec;
+                    L.leafEps[at] = leafHalfEps;
+                }
                 if (toPush) L.stack[pushAt] = cc;
-                nl += __popcll(mList);
+                nn += __popcll(mNode);
+                nf += __popcll(mLeaf);
                 sp = sp + nPush < kSCap ? sp + nPush : kSCap;
                 forced += (uint32_t)__popcll(ballot(forcedHere));
-                if (isLeaf) {
-                    const uint32_t rel = (uint32_t)(~cc) - waveBase;
-                    if (rel < 64u && ((am >> rel) & 1ull)) selfVisits += 1;  // counted per lane, summed below
-                }
             }
             __builtin_amdgcn_wave_barrier();
         }
-        // ---- evaluate the list: four records in flight ----
-        listed += (uint32_t)nl;
-        int e = 0;
-        for (; e + 4 <= nl; e += 4) {
-            int id[4];
-            float4 q[4];
-            float ej[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                id[u] = uniformInt(L.list[e + u]);
-                if (id[u] < 0) {
-                    q[u] = c.cur.pos[~id[u]];
-                    ej[u] = c.cur.hu[~id[u]].y;
-                } else {
-                    q[u] = c.t.cm[id[u]];
-                    ej[u] = 0.0f;
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if (id[u] < 0) {
-                    const float eij = 0.5f * (epsI + soft * fmaxf(ej[u], 1.0e-6f));
-                    if (active && (uint32_t)(~id[u]) != i) interact(pi.x, pi.y, pi.z, q[u], eij * eij, ax, ay, az, phi);
-                } else if (active) {
-                    interact(pi.x, pi.y, pi.z, q[u], epsI2, ax, ay, az, phi);
-                }
-            }
+        // ---- evaluate the staged records ----
+        listed += (uint32_t)(nn + nf);
+#pragma unroll 4
+        for (int e = 0; e < nn; ++e) {
+            const float4 q = L.node[e];
+            const float dx = pi.x - q.x, dy = pi.y - q.y, dz = pi.z - q.z;
+            const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, e2Node)));
+            const float ir = rsqrtf(d2);
+            const float t = q.w * ir;
+            const float f = t * (ir * ir);
+            ax = __builtin_fmaf(-f, dx, ax);
+            ay = __builtin_fmaf(-f, dy, ay);
+            az = __builtin_fmaf(-f, dz, az);
+            phi -= t;
         }
-        for (; e < nl; ++e) {
-            const int id = uniformInt(L.list[e]);
-            if (id < 0) {
-                const uint32_t j = (uint32_t)~id;
-                const float4 q = c.cur.pos[j];
-                const float eij = 0.5f * (epsI + soft * fmaxf(c.cur.hu[j].y, 1.0e-6f));
-                if (active && j != i) interact(pi.x, pi.y, pi.z, q, eij * eij, ax, ay, az, phi);
-            } else if (active) {
-                interact(pi.x, pi.y, pi.z, c.t.cm[id], epsI2, ax, ay, az, phi);
-            }
+#pragma unroll 4
+        for (int e = 0; e < nf; ++e) {
+            const float4 q = L.leaf[e];
+            const float eij = halfEpsI + L.leafEps[e];
+            const float dx = pi.x - q.x, dy = pi.y - q.y, dz = pi.z - q.z;
+            const float e2 = __builtin_fmaf(eij, eij, 1.0e-20f);
+            const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, e2)));
+            const float ir = rsqrtf(d2);
+            const float t = q.w * ir;
+            const float f = t * (ir * ir);
+            ax = __builtin_fmaf(-f, dx, ax);
+            ay = __builtin_fmaf(-f, dy, ay);
+            az = __builtin_fmaf(-f, dz, az);
+            phi -= t;
         }
-        nl = 0;
+        nn = 0;
+        nf = 0;
         __builtin_amdgcn_wave_barrier();
         if (sp == 0) break;
     }
+    // the particle's own leaf: zero separation, so only its potential term is taken back
+    // (the same expression as in the loop: eps_ii = eps_i)
+    const bool selfListed = L.self[lane] != 0u;
+    if (active && selfListed) phi += (kGf * pi.w) * rsqrtf(__builtin_fmaf(epsI, epsI, 1.0e-20f));
     if (active) c.d.grav[i] = make_float4(ax, ay, az, phi);
-    // interactions = active lanes x listed entries - each active particle's own leaf
-    uint32_t selfSum = selfVisits;
-    for (int o = 32; o > 0; o >>= 1) selfSum += (uint32_t)__shfl_xor((int)selfSum, o, 64);
+    // interactions = active lanes x listed records - the particles' own leaves
+    const uint32_t selfSum = (uint32_t)__popcll(ballot(active && selfListed));
     if (lane == 0) {
         const unsigned long long lanes = (unsigned long long)__popcll(am);
         atomicAdd(&c.cnt.v[0], lanes * (unsigned long long)listed - selfSum);

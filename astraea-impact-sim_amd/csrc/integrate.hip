@@ -170,7 +170,6 @@ __global__ __launch_bounds__(kBlock) void kDrift(DeviceCtx c) {
     __shared__ float sb[kWavesPerBlock][6];
     const uint32_t b = xcdBlock(blockIdx.x, gridDim.x);
     const uint32_t i = b * kBlock + threadIdx.x;
-    const uint32_t gw = i >> 6;
     const int lane = laneId(), wave = threadIdx.x >> 6;
     const uint32_t sg = c.seg.slotSeg[i];
     const bool active = i - c.seg.off[sg] < c.seg.n[sg];
@@ -181,9 +180,8 @@ __global__ __launch_bounds__(kBlock) void kDrift(DeviceCtx c) {
         const float hi = fmaxf(c.cur.hu[i].x, 1.0e-6f), Hi = 2.0f * hi;
         const float rhoI = fmaxf(vi.w, 1.0e-9f);
         float xs = 0.0f, ys = 0.0f, zs = 0.0f;
-        const uint64_t o = c.nb.waveOff[gw];
-        const uint32_t cnt = (o + 64ull * c.nb.waveMax[gw] <= c.nb.capacity) ? c.nb.count[i] : 0u;
-        const uint32_t* lst = c.nb.list + o + (uint64_t)lane;
+        const uint32_t cnt = c.nb.count[i];
+        const uint32_t* lst = tableRow(c.nb, i);
         for (uint32_t k = 0; k < cnt; ++k) {
             const uint32_t j = lst[64ull * k];
             const float4 pj = c.cur.pos[j];

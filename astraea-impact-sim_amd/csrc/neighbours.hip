@@ -13,30 +13,33 @@
 // sets are exactly the reference's.  Each lane then loops over its own list: the pair
 // physics runs only for real pairs.
 //
-// Layout: wavefront w owns a block at waveOff[w]; entry k of lane l is at
-// waveOff[w] + 64 k + l, so a pass reading entry k on every lane issues one coalesced load.
-// Build = count walk, exclusive scan of 64 * max count per wavefront, fill walk.
+// Layout: slot i owns `stride` entries; entry k of slot i = 64 w + l is at
+// 64 w stride + 64 k + l, so a pass reading entry k on every lane issues one coalesced load.
+// The table is written in a single walk; a slot that wants more than `stride` entries keeps the
+// first `stride`, and the overflow flag and the largest count wanted are left for the host.
 #include "launch.hpp"
 
 namespace astraea_hip {
 namespace {
 
-constexpr int kSCap = 1024;  // per-wave node stack
-constexpr int kLCap = 512;   // per-wave candidate-leaf list
-constexpr int kFlushAt = kLCap - 256;
+constexpr int kSCap = 1024;                  // per-wave node stack
+constexpr int kCCap = 256;                   // per-wave candidate records
+constexpr int kFlushAt = kCCap - 2 * kWave;  // one round lists at most 128 leaves
 
 struct WaveLds {
     int stack[kSCap];
-    int list[kLCap];
-    float4 gp[kWave];  // group particle (x, y, z, support bound), w = -1 when inactive
+    float4 cand[kCCap];  // candidate particle (x, y, z, support bound)
+    int candId[kCCap];
 };
 
-// Wide walk: the wave expands up to 64 stack nodes per round (one per lane).  A node is opened
-// when any group particle's sphere of radius max(Hb_p, node's max Hb) meets its box; the group's
-// bounding sphere rejects most non-overlapping nodes without the per-particle test (a margin far
-// above FP32 rounding keeps those decisions exact).  Leaves become candidates that every lane then
-// tests against its own particle.
-template <bool FILL>
+// Wide walk: the wave expands up to 64 stack nodes per round, one per lane.  A node is opened
+// when any group particle's sphere of radius max(Hb_p, node's max Hb) meets its box.  The box
+// distance between the group's bounding box and the node's box is, in FP32 as in exact arithmetic,
+// a lower bound of every particle's box distance (rounding is monotone), so nodes it puts out of
+// reach are closed for every particle.  The survivors are tested exactly, one node at a time
+// across the lanes (lane = particle), with the node's box broadcast from the lane that loaded it.
+// Leaf children are loaded by their lanes in parallel, filtered by the same box bound, and
+// staged in LDS, where every lane tests its own particle against them.
 __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
     __shared__ WaveLds lds[kWavesPerBlock];
     const uint32_t b = xcdBlock(blockIdx.x, gridDim.x);
@@ -46,14 +49,10 @@ __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
     const uint32_t sg = uniformU32(c.seg.slotSeg[b * kBlock]);
     const uint32_t off = uniformU32(c.seg.off[sg]), n = uniformU32(c.seg.n[sg]);
     const bool active = i - off < n;
+    const uint32_t K = c.nb.stride;
+    uint32_t* out = c.nb.list + (uint64_t)gw * (64ull * K) + (uint32_t)lane;
     uint32_t cnt = 0;
     bool stackFull = false;
-    uint64_t base = 0;
-    bool writable = true;
-    if (FILL) {
-        base = c.nb.waveOff[gw] + (uint64_t)lane;
-        writable = c.nb.waveOff[gw] + 64ull * c.nb.waveMax[gw] <= c.nb.capacity;
-    }
     if (ballot(active) != 0ull && n >= 2u) {
         WaveLds& L = lds[wave];
         float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -63,18 +62,10 @@ __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
             const float4 hu = c.cur.hu[i];
             Hb = supportBound(hu.x, hu.y);
         }
-        L.gp[lane] = make_float4(pi.x, pi.y, pi.z, active ? Hb : -1.0f);
-        float cx, cy, cz, R, HbMax;
-        {
-            const float lx = waveMinF(active ? pi.x : 3.0e38f), hx = waveMaxF(active ? pi.x : -3.0e38f);
-            const float ly = waveMinF(active ? pi.y : 3.0e38f), hy = waveMaxF(active ? pi.y : -3.0e38f);
-            const float lz = waveMinF(active ? pi.z : 3.0e38f), hz = waveMaxF(active ? pi.z : -3.0e38f);
-            cx = 0.5f * (lx + hx); cy = 0.5f * (ly + hy); cz = 0.5f * (lz + hz);
-            const float dx = pi.x - cx, dy = pi.y - cy, dz = pi.z - cz;
-            R = waveMaxF(active ? sqrtf(dx * dx + dy * dy + dz * dz) : 0.0f);
-            R = R * (1.0f + 1.0e-5f) + 1.0e-30f;
-            HbMax = waveMaxF(active ? Hb : 0.0f);
-        }
+        const float lx = waveMinF(active ? pi.x : 3.0e38f), hx = waveMaxF(active ? pi.x : -3.0e38f);
+        const float ly = waveMinF(active ? pi.y : 3.0e38f), hy = waveMaxF(active ? pi.y : -3.0e38f);
+        const float lz = waveMinF(active ? pi.z : 3.0e38f), hz = waveMaxF(active ? pi.z : -3.0e38f);
+        const float HbMax = waveMaxF(active ? Hb : 0.0f);
         const uint64_t lt = (1ull << lane) - 1ull;
         int sp = 1, nl = 0;
         L.stack[0] = (int)off;
@@ -90,28 +81,44 @@ __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
 #pragma unroll
                 for (int side = 0; side < 2; ++side) {
                     const int cc = side == 0 ? ch.x : ch.y;
-                    const bool isLeaf = has && cc < 0;
+                    const bool isInt = has && cc >= 0;
+                    bool isLeaf = has && cc < 0;
+                    float4 mn = make_float4(0.0f, 0.0f, 0.0f, 0.0f), mx = mn, q = mn;
+                    bool maybe = false;
+                    if (isInt) {
+                        mn = c.t.bmin[cc];
+                        mx = c.t.bmax[cc];
+                        const float reach = fmaxf(HbMax, mx.w);
+                        const float gx = fmaxf(0.0f, fmaxf(mn.x - hx, lx - mx.x));
+                        const float gy = fmaxf(0.0f, fmaxf(mn.y - hy, ly - mx.y));
+                        const float gz = fmaxf(0.0f, fmaxf(mn.z - hz, lz - mx.z));
+                        maybe = dist2f(gx, gy, gz) <= reach * reach;
+                    } else if (isLeaf) {
+                        const uint32_t j = (uint32_t)~cc;
+                        const float4 pj = c.cur.pos[j];
+                        const float4 hj = c.cur.hu[j];
+                        q = make_float4(pj.x, pj.y, pj.z, supportBound(hj.x, hj.y));
+                        const float reach = fmaxf(HbMax, q.w);
+                        const float gx = fmaxf(0.0f, fmaxf(lx - q.x, q.x - hx));
+                        const float gy = fmaxf(0.0f, fmaxf(ly - q.y, q.y - hy));
+                        const float gz = fmaxf(0.0f, fmaxf(lz - q.z, q.z - hz));
+                        isLeaf = dist2f(gx, gy, gz) < reach * reach;
+                    }
+                    // exact test of the surviving nodes: lane = particle
                     bool open = false;
-                    if (has && cc >= 0) {
-                        const float4 mn = c.t.bmin[cc], mx = c.t.bmax[cc];
-                        const float Rn = mx.w;
-                        const float ddx = fmaxf(0.0f, fmaxf(mn.x - cx, cx - mx.x));
-                        const float ddy = fmaxf(0.0f, fmaxf(mn.y - cy, cy - mx.y));
-                        const float ddz = fmaxf(0.0f, fmaxf(mn.z - cz, cz - mx.z));
-                        const float db = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
-                        const float reach = fmaxf(HbMax, Rn);
-                        const float slack = 2.0e-6f * (db + R + reach) + 1.0e-30f;
-                        if (db - R - slack <= reach) {
-                            for (int j = 0; j < kWave && !open; ++j) {
-                                const float4 g = L.gp[j];
-                                if (g.w < 0.0f) continue;
-                                const float Rr = fmaxf(g.w, Rn);
-                                open = boxDist2f(g.x, g.y, g.z, mn, mx) <= Rr * Rr;
-                            }
-                        }
+                    uint64_t um = ballot(maybe);
+                    while (um != 0ull) {
+                        const int src = (int)__builtin_ctzll(um);
+                        um &= um - 1ull;
+                        const float4 bmn = make_float4(readLaneF(mn.x, src), readLaneF(mn.y, src), readLaneF(mn.z, src), 0.0f);
+                        const float4 bmx = make_float4(readLaneF(mx.x, src), readLaneF(mx.y, src), readLaneF(mx.z, src), 0.0f);
+                        const float rr = fmaxf(Hb, readLaneF(mx.w, src));
+                        const bool hit = active && boxDist2f(pi.x, pi.y, pi.z, bmn, bmx) <= rr * rr;
+                        const bool any = ballot(hit) != 0ull;
+                        if (lane == src) open = any;
                     }
                     const uint64_t mPush = ballot(open);
-                    int pushAt = sp + __popcll(mPush & lt);
+                    const int pushAt = sp + __popcll(mPush & lt);
                     if (open && pushAt >= kSCap) {
                         open = false;
                         stackFull = true;
@@ -120,40 +127,24 @@ __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
                     const int nPush = __popcll(mPush);
                     sp = sp + nPush < kSCap ? sp + nPush : kSCap;
                     const uint64_t mList = ballot(isLeaf);
-                    if (isLeaf) L.list[nl + __popcll(mList & lt)] = ~cc;
+                    if (isLeaf) {
+                        const int at = nl + __popcll(mList & lt);
+                        L.cand[at] = q;
+                        L.candId[at] = ~cc;
+                    }
                     nl += __popcll(mList);
                 }
                 __builtin_amdgcn_wave_barrier();
             }
-            // candidates: every lane tests its own particle, four records in flight
-            int e = 0;
-            for (; e + 4 <= nl; e += 4) {
-                uint32_t j[4];
-                float4 pj[4], hj[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    j[u] = (uint32_t)uniformInt(L.list[e + u]);
-                    pj[u] = c.cur.pos[j[u]];
-                    hj[u] = c.cur.hu[j[u]];
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const float Rr = fmaxf(Hb, supportBound(hj[u].x, hj[u].y));
-                    const float r2 = dist2f(pi.x - pj[u].x, pi.y - pj[u].y, pi.z - pj[u].z);
-                    if (active && j[u] != i && r2 < Rr * Rr) {
-                        if (FILL && writable) c.nb.list[base + 64ull * cnt] = j[u];
-                        ++cnt;
-                    }
-                }
-            }
-            for (; e < nl; ++e) {
-                const uint32_t jj = (uint32_t)uniformInt(L.list[e]);
-                const float4 pj = c.cur.pos[jj];
-                const float4 hj = c.cur.hu[jj];
-                const float Rr = fmaxf(Hb, supportBound(hj.x, hj.y));
-                const float r2 = dist2f(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
-                if (active && jj != i && r2 < Rr * Rr) {
-                    if (FILL && writable) c.nb.list[base + 64ull * cnt] = jj;
+            // candidates: every lane tests its own particle
+#pragma unroll 4
+            for (int e = 0; e < nl; ++e) {
+                const float4 q = L.cand[e];
+                const uint32_t jj = (uint32_t)L.candId[e];
+                const float rr = fmaxf(Hb, q.w);
+                const float r2 = dist2f(pi.x - q.x, pi.y - q.y, pi.z - q.z);
+                if (active && jj != i && r2 < rr * rr) {
+                    if (cnt < K) out[64ull * cnt] = jj;
                     ++cnt;
                 }
             }
@@ -162,51 +153,24 @@ __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
             if (sp == 0) break;
         }
     }
-    if (!FILL) {
-        c.nb.count[i] = cnt;
-        const uint32_t m = waveMaxU(cnt);
-        if (lane == 0) c.nb.waveMax[gw] = m;
-        if (ballot(stackFull) != 0ull && lane == 0) atomicOr((unsigned long long*)&c.cnt.v[6], 1ull);
-    }
-}
-
-// Exclusive scan of 64 * waveMax over all wavefronts (one block of 1024 threads).
-__global__ __launch_bounds__(1024) void kNeighbourScan(DeviceCtx c, uint32_t waves) {
-    __shared__ unsigned long long part[16];
-    const uint32_t t = threadIdx.x, lane = laneId(), w = t >> 6;
-    const uint32_t per = (waves + 1023u) / 1024u;
-    const uint32_t s0 = t * per, s1 = min(waves, s0 + per);
-    unsigned long long sum = 0;
-    for (uint32_t k = s0; k < s1; ++k) sum += 64ull * c.nb.waveMax[k];
-    unsigned long long inc = sum;
-    for (int o = 1; o < 64; o <<= 1) {
-        const unsigned long long v = __shfl_up(inc, o, 64);
-        if ((int)lane >= o) inc += v;
-    }
-    if (lane == 63) part[w] = inc;
-    __syncthreads();
-    unsigned long long pre = 0, tot = 0;
-    for (uint32_t q = 0; q < 16; ++q) {
-        if (q < w) pre += part[q];
-        tot += part[q];
-    }
-    unsigned long long run = pre + inc - sum;
-    for (uint32_t k = s0; k < s1; ++k) {
-        c.nb.waveOff[k] = run;
-        run += 64ull * c.nb.waveMax[k];
-    }
-    if (t == 0) {
-        c.cnt.v[2] = tot;
-        if (tot > c.nb.capacity) c.cnt.v[3] = 1ull;
+    c.nb.count[i] = cnt < K ? cnt : K;
+    uint32_t sum = cnt;
+    for (int o = 32; o > 0; o >>= 1) sum += (uint32_t)__shfl_xor((int)sum, o, 64);
+    const uint32_t mxc = waveMaxU(cnt);
+    const bool full = ballot(stackFull) != 0ull;
+    if (lane == 0) {
+        c.nb.waveSum[gw] = sum;
+        c.nb.waveMax[gw] = mxc;
+        if (mxc > K) {
+            atomicOr(&c.cnt.v[3], 1ull);
+            atomicMax(&c.cnt.v[7], (unsigned long long)mxc);
+        }
+        if (full) atomicOr(&c.cnt.v[6], 1ull);
     }
 }
 
 }  // namespace
 
-void launchNeighbours(const DeviceCtx& c, hipStream_t s) {
-    kNeighbours<false><<<c.blocks, kBlock, 0, s>>>(c);
-    kNeighbourScan<<<1, 1024, 0, s>>>(c, c.slots / kWave);
-    kNeighbours<true><<<c.blocks, kBlock, 0, s>>>(c);
-}
+void launchNeighbours(const DeviceCtx& c, hipStream_t s) { kNeighbours<<<c.blocks, kBlock, 0, s>>>(c); }
 
 }  // namespace astraea_hip

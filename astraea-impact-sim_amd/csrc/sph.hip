@@ -18,11 +18,6 @@
 namespace astraea_hip {
 namespace {
 
-__device__ __forceinline__ uint32_t tableCount(const DeviceCtx& c, uint32_t i, uint32_t gw) {
-    const uint64_t o = c.nb.waveOff[gw];
-    return (o + 64ull * c.nb.waveMax[gw] <= c.nb.capacity) ? c.nb.count[i] : 0u;
-}
-
 __device__ __forceinline__ float balsaraSwitch(float divV, float cx, float cy, float cz, float cs, float h) {
     const float ad = fabsf(divV), ac = sqrtf(cx * cx + cy * cy + cz * cz);
     const float den = ad + ac + 1.0e-4f * fmaxf(cs, 0.05f) / fmaxf(h, 1.0e-6f);
@@ -50,8 +45,6 @@ template <bool STRENGTH>
 __global__ __launch_bounds__(kBlock) void kPass1(DeviceCtx c) {
     const uint32_t b = xcdBlock(blockIdx.x, gridDim.x);
     const uint32_t i = b * kBlock + threadIdx.x;
-    const uint32_t gw = i >> 6;
-    const int lane = laneId();
     const uint32_t sg = c.seg.slotSeg[i];
     if (i - c.seg.off[sg] >= c.seg.n[sg]) return;
 
@@ -68,8 +61,8 @@ __global__ __launch_bounds__(kBlock) void kPass1(DeviceCtx c) {
     float nsx = 0.0f, nsy = 0.0f, nsz = 0.0f;
     float gxx = 0.0f, gxy = 0.0f, gxz = 0.0f, gyx = 0.0f, gyy = 0.0f, gyz = 0.0f, gzx = 0.0f, gzy = 0.0f, gzz = 0.0f;
 
-    const uint32_t cnt = tableCount(c, i, gw);
-    const uint32_t* lst = c.nb.list + c.nb.waveOff[gw] + (uint64_t)lane;
+    const uint32_t cnt = c.nb.count[i];
+    const uint32_t* lst = tableRow(c.nb, i);
     for (uint32_t k = 0; k < cnt; ++k) {
         const uint32_t j = lst[64ull * k];
         const float4 pj = c.cur.pos[j];
@@ -140,8 +133,6 @@ template <bool STRENGTH>
 __global__ __launch_bounds__(kBlock) void kPass2(DeviceCtx c) {
     const uint32_t b = xcdBlock(blockIdx.x, gridDim.x);
     const uint32_t i = b * kBlock + threadIdx.x;
-    const uint32_t gw = i >> 6;
-    const int lane = laneId();
     const uint32_t sg = c.seg.slotSeg[i];
     if (i - c.seg.off[sg] >= c.seg.n[sg]) return;
 
@@ -174,8 +165,8 @@ __global__ __launch_bounds__(kBlock) void kPass2(DeviceCtx c) {
     float gxx = 0.0f, gxy = 0.0f, gxz = 0.0f, gyx = 0.0f, gyy = 0.0f, gyz = 0.0f, gzx = 0.0f, gzy = 0.0f, gzz = 0.0f;
     uint32_t pairs = 0;
 
-    const uint32_t cnt = tableCount(c, i, gw);
-    const uint32_t* lst = c.nb.list + c.nb.waveOff[gw] + (uint64_t)lane;
+    const uint32_t cnt = c.nb.count[i];
+    const uint32_t* lst = tableRow(c.nb, i);
     for (uint32_t k = 0; k < cnt; ++k) {
         const uint32_t j = lst[64ull * k];
         const float4 pj = c.cur.pos[j];

@@ -155,6 +155,8 @@ def main():
         },
         "energy_drift_rel": drift,
         "interactions_per_particle": cnt["interactions"] / (n * K),
+        "neighbour_candidates_per_particle": cnt2["nbr_entries"] / n,
+        "neighbour_candidates_max": cnt2["nbr_max"],
         "stage_ms": {k: (v[0] / max(v[1], 1)) for k, v in stages.items()},
         "roofline": {
             "bound": "valu",

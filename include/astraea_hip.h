@@ -195,12 +195,21 @@ int astraea_get_dt(astraea_ctx* ctx, float* out_dt);
 #define ASTRAEA_ENERGY_STRIDE 18
 int astraea_energy(astraea_ctx* ctx, double* out);
 
-/* Counters of the last gravity evaluation: `out[0]` = sum over active particles
- * of the particle-node and particle-particle interactions evaluated, `out[1]` =
- * traversal-stack overflows (forced acceptances, sph_gravity.cu:933), `out[2]`
- * = neighbour-table entries, `out[3]` = neighbour-table capacity, `out[4]` =
- * sum of pass-1 neighbour counts (self excluded), `out[5]` = sum of pass-2 pairs. */
+/* Counters: `out[0]` = sum over active particles of the particle-node and
+ * particle-particle interactions of the gravity evaluations since the last call,
+ * `out[1]` = traversal-stack overflows of those (forced acceptances,
+ * sph_gravity.cu:933), `out[2]` = neighbour candidates of the last table (all
+ * particles), `out[3]` = table entries per particle, `out[4]` = sum of pass-1
+ * neighbour counts (self excluded), `out[5]` = sum of pass-2 pairs, `out[6]` =
+ * largest candidate count of one particle in the last table.  Up to `n` values. */
 int astraea_counters(astraea_ctx* ctx, uint64_t* out, int n);
+
+/* Width of the neighbour table, in entries per particle (no counterpart in the
+ * reference, which walks the tree in every pass).  The table takes
+ * 4 * stride * capacity bytes.  A particle with more candidates than this makes
+ * astraea_synchronize report ASTRAEA_E_OVERFLOW for the step and widen the table. */
+#define ASTRAEA_DEFAULT_NEIGHBOUR_STRIDE 320
+int astraea_set_neighbour_stride(astraea_ctx* ctx, uint32_t stride);
 
 /* Tree of segment `seg` from the last sort, in the reference's local encoding
  * (sph_gravity.cuh:26-59): internal nodes 0..n-2, a negative child c is the leaf

@@ -73,6 +73,7 @@ _sig = {
     "or_step": (C.c_int, [_P, C.c_double, C.c_uint32]),
     "or_energy": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "or_counters": (C.c_int, [_P, C.POINTER(C.c_uint64), C.c_int]),
+    "or_neighbour_stats": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
     "or_set_dt": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "or_get_dt": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "or_download_tree": (C.c_int, [_P, C.c_uint32] + [_P] * 9),
@@ -223,8 +224,15 @@ class Oracle:
     def counters(self) -> dict:
         out = (C.c_uint64 * 6)()
         _check(_lib.or_counters(self._h, out, 6))
-        keys = ["interactions", "forced_accepts", "nbr_entries", "nbr_capacity", "pass1_pairs", "pass2_pairs"]
+        keys = ["interactions", "forced_accepts", "nbr_entries", "nbr_stride", "pass1_pairs", "pass2_pairs"]
         return dict(zip(keys, [int(v) for v in out]))
+
+    def neighbour_stats(self) -> dict:
+        """Brute-force size of the neighbour table for the current state: all candidates and the
+        largest count of one particle."""
+        out = (C.c_uint64 * 2)()
+        _check(_lib.or_neighbour_stats(self._h, out))
+        return {"nbr_entries": int(out[0]), "nbr_max": int(out[1])}
 
     def tree(self, seg=0) -> dict:
         n = self.counts[seg]

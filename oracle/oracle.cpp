@@ -1276,6 +1276,37 @@ int or_counters(or_ctx* c, uint64_t* out, int n) {
     return ASTRAEA_OK;
 }
 
+// Neighbour-table size by brute force over all pairs of each segment, in the FP32 arithmetic of
+// the table build: j != i is a candidate of i when dist2f(x_i - x_j) < max(Hb_i, Hb_j)^2 with
+// Hb = supportBoundF(h, h0) of the FP32 state.  out[0] = candidates of all particles,
+// out[1] = the largest count of one particle.
+int or_neighbour_stats(or_ctx* c, uint64_t* out) {
+    if (!c || !out) return ASTRAEA_E_INVALID;
+    uint64_t total = 0, mx = 0;
+    for (auto& s : c->seg) {
+        const uint32_t n = s.n;
+        std::vector<float> xf(n), yf(n), zf(n), hb(n);
+        for (uint32_t k = 0; k < n; ++k) {
+            xf[k] = (float)s.x[k]; yf[k] = (float)s.y[k]; zf[k] = (float)s.z[k];
+            hb[k] = supportBoundF((float)s.h[k], (float)s.h0[k]);
+        }
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : total) reduction(max : mx)
+        for (uint32_t a = 0; a < n; ++a) {
+            uint64_t k = 0;
+            for (uint32_t b = 0; b < n; ++b) {
+                if (b == a) continue;
+                const float r = std::fmax(hb[a], hb[b]);
+                if (dist2f(xf[a] - xf[b], yf[a] - yf[b], zf[a] - zf[b]) < r * r) ++k;
+            }
+            total += k;
+            mx = std::max(mx, k);
+        }
+    }
+    out[0] = total;
+    out[1] = mx;
+    return ASTRAEA_OK;
+}
+
 int or_download_tree(or_ctx* c, uint32_t seg, uint64_t* keys, uint32_t* perm, int32_t* left, int32_t* right,
                      int32_t* pI, int32_t* pL, float* cm, float* mn, float* mx) {
     Seg* s = c ? segOf(c, seg) : nullptr;

@@ -59,6 +59,7 @@ int or_initial_forces(or_ctx* c);
 int or_step(or_ctx* c, double max_dt_psu, uint32_t steps);
 int or_energy(or_ctx* c, double* out);
 int or_counters(or_ctx* c, uint64_t* out, int n);
+int or_neighbour_stats(or_ctx* c, uint64_t* out);
 int or_set_dt(or_ctx* c, const double* dt);
 int or_get_dt(or_ctx* c, double* dt);
 int or_download_tree(or_ctx* c, uint32_t seg, uint64_t* keys, uint32_t* perm,

@@ -350,19 +350,83 @@ def test_conservation_small_config(lib):
     assert abs(drift) < 0.01 and dp < 1e-4 and dl < 0.005, (drift, dp, dl)
 
 
-def test_neighbour_table_overflow_is_reported(lib):
-    """A neighbour table too small for the step is reported at synchronize, not silently used."""
+@pytest.mark.parametrize("which", ["small", "segments"])
+def test_neighbour_table_size_matches_brute_force(lib, orc, which):
+    """The table holds exactly the candidates j != i with |x_i - x_j|^2 < max(Hb_i, Hb_j)^2 (FP32),
+    counted by the oracle over all pairs: same total and same largest count of one particle."""
+    from astraea_amd import scenario as S
+
+    if which == "small":
+        st, counts = perturbed_small_state(), None
+    else:
+        st, counts = S.build_sweep(S.sweep_scenarios(count=4, n_min=300, n_max=4000))
+    counts = counts or [len(st["mass"])]
+    ctx, o = run_pair(lib, orc, st, counts, S.plain_params(), [("sort_and_build",)])
+    g = ctx.counters()
+    r = o.neighbour_stats()
+    print(which, "entries", g["nbr_entries"], r["nbr_entries"], "max", g["nbr_max"], r["nbr_max"])
+    assert g["nbr_entries"] == r["nbr_entries"]
+    assert g["nbr_max"] == r["nbr_max"]
+    assert g["nbr_max"] <= g["nbr_stride"]
+
+
+def test_neighbour_table_widens_at_load(lib):
+    """A table narrower than a particle's candidate count is widened when forces are evaluated at
+    load (a synchronisation point), and the forces equal those of a context that never overflowed."""
     from astraea_amd import scenario as S
 
     st = S.config_small()
     n = len(st["mass"])
+
+    def forces(ctx):
+        ctx.load_state(st)
+        ctx.initial_forces()
+        ctx.synchronize()
+        return np.stack([ctx.download(k) for k in ("acc_x", "acc_y", "acc_z", "du_dt")])
+
+    ref = lib.Context(n)
+    ref.set_params(S.plain_params())
+    ref.set_layout([n])
+    want = forces(ref)
+
     ctx = lib.Context(n)
     ctx.set_params(S.plain_params())
     ctx.set_layout([n])
+    ctx.set_neighbour_stride(16)
+    got = forces(ctx)
+    c = ctx.counters()
+    assert c["nbr_max"] > 16 and c["nbr_stride"] >= c["nbr_max"]
+    assert np.array_equal(got, want)
+
+
+def test_neighbour_table_overflow_in_steps_is_reported(lib):
+    """Smoothing lengths far below equilibrium grow by 8 % per step, and the candidate counts with
+    them; once a particle outgrows the table, synchronize reports the overflow, widens the table and
+    asks for a new force evaluation."""
+    from astraea_amd import scenario as S
+
+    st = S.config_small()
+    st["h"] = (st["h"] * 0.6).astype(np.float32)
+    n = len(st["mass"])
+    ctx = lib.Context(n)
+    ctx.set_params(S.plain_params())
+    ctx.set_layout([n])
+    ctx.set_neighbour_stride(64)
     ctx.load_state(st)
     ctx.initial_forces()
     ctx.synchronize()
-    assert ctx.counters()["nbr_entries"] <= ctx.counters()["nbr_capacity"]
+    assert ctx.counters()["nbr_stride"] == 64
+    with pytest.raises(lib.AstraeaError) as e:
+        for _ in range(30):
+            ctx.step(MAX_DT, 1, graph=False)
+            ctx.synchronize()
+    assert e.value.code == -5 and "neighbour table overflow" in str(e.value)
+    assert ctx.counters()["nbr_stride"] > 64
+    with pytest.raises(lib.AstraeaError):
+        ctx.step(MAX_DT, 1)   # the table was rebuilt empty: forces must be evaluated again
+    ctx.initial_forces()
+    ctx.step(MAX_DT, 2)
+    ctx.synchronize()
 
 
 def test_sim_headless_verify_conservation():

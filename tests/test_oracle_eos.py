@@ -77,16 +77,31 @@ def test_cold_energy_exact_and_monotone(orc, mat):
         prev = v
 
 
+def cold_si_exact(m, rho):
+    """cold_si in 60-digit decimal arithmetic (the log form cancels to ~mu^2/2 near rho0)."""
+    from decimal import Decimal, localcontext
+
+    with localcontext() as ctx:
+        ctx.prec = 60
+        A, B, r0 = Decimal(m["A"]), Decimal(m["B"]), Decimal(m["rho0"])
+        eta = Decimal(rho) / r0
+        ln = eta.ln()
+        return float((A * (ln + 1 / eta - 1) + B * (eta - 1 / eta - 2 * ln)) / r0)
+
+
 def test_cold_energy_series_branch_continuous(orc):
-    """The mu < 1e-5 series (eos.cu:304-310) joins the log form: compare across the switch."""
+    """The mu < 1e-5 series (eos.cu:304-310) joins the log form: both sides of the switch agree
+    with the integral evaluated in 60-digit arithmetic."""
     for mat in (0, 1, 2, 4):
         m = orc.material(mat)
         r0 = m["rho0"] / RHO_UNIT
-        below = orc.cold_energy_psu(mat, r0 * (1 + 0.999e-5))
-        above = orc.cold_energy_psu(mat, r0 * (1 + 1.001e-5))
-        exact_b = cold_si(m, m["rho0"] * (1 + 0.999e-5)) / U_UNIT
-        assert abs(below - exact_b) <= 1e-6 * exact_b
-        assert above > below
+        for f in (0.999e-5, 1.001e-5):
+            v = orc.cold_energy_psu(mat, r0 * (1 + f))
+            exact = cold_si_exact(m, m["rho0"] * (1 + f)) / U_UNIT
+            # the series is exact to O(mu^6); the log form loses ~1e-16 / mu^2 to cancellation
+            tol = 1e-9 if f < 1e-5 else 1e-5
+            assert abs(v - exact) <= tol * exact, (mat, f, v, exact)
+        assert orc.cold_energy_psu(mat, r0 * (1 + 1.001e-5)) > orc.cold_energy_psu(mat, r0 * (1 + 0.999e-5))
 
 
 @pytest.mark.parametrize("mat", [0, 1, 2, 4])
