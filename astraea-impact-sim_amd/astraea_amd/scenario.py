@@ -12,6 +12,7 @@ Seeds follow the benchmark plan: target seed s, impactor seed s + 1
 from __future__ import annotations
 
 import math
+import os
 
 import numpy as np
 
@@ -89,9 +90,21 @@ def sweep_scenarios(count: int = 64, n_min: int = 20_000, n_max: int = 500_000, 
     return out
 
 
-def build_sweep(specs) -> tuple[dict, list[int]]:
-    states = [impact_pair(sp["n_target"], sp["n_impactor"], sp["angle_deg"], sp["v_ratio"], seed=sp["seed"],
-                          mass_ratio=sp["mass_ratio"]) for sp in specs]
+def build_sweep(specs, threads: int | None = None, split: bool = False):
+    """Concatenated state of the scenarios `specs` and their particle counts (segment order).
+    Bodies are built on `threads` host threads (the builder releases the GIL).  With `split`,
+    the per-scenario states are returned instead of the concatenation."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(sp):
+        return impact_pair(sp["n_target"], sp["n_impactor"], sp["angle_deg"], sp["v_ratio"], seed=sp["seed"],
+                           mass_ratio=sp["mass_ratio"])
+
+    threads = threads or min(16, os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=threads) as pool:
+        states = list(pool.map(one, specs))
     counts = [sp["n_target"] + sp["n_impactor"] for sp in specs]
+    if split:
+        return states, counts
     merged = {k: np.ascontiguousarray(np.concatenate([s[k] for s in states])) for k in states[0]}
     return merged, counts

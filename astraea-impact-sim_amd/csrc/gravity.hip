@@ -10,12 +10,13 @@
 //
 // The reference pops one node per iteration, so each wave waits on two dependent loads per node.
 // Here the wave expands up to 64 stack nodes of its LDS stack at once, one per lane: each lane
-// loads its node's children and decides the group opening test for them.  Most decisions come
-// from bounds on the distance between the node's centre of mass and the group's particles (the
-// group's bounding sphere and bounding box): with a relative margin far above FP32 rounding the
-// per-particle FP32 test is then known to accept (or reject) for all particles.  The remaining
-// nodes are tested exactly, one node at a time across the lanes (lane = particle), with the node
-// broadcast from the lane that loaded it.  The interaction set is exactly the reference's.
+// loads its node's children and decides the group opening test for them.  The per-particle FP32
+// test is rewritten as r^2 >= T with T the exact FP32 acceptance threshold of the node, and most
+// decisions come from the squared distances between the node's centre of mass and the group's
+// bounding box (nearest point and farthest corner): with a relative margin far above FP32
+// rounding the test is then known to pass (or fail) for all particles.  The remaining nodes are
+// tested exactly, one node at a time across the lanes (lane = particle), read back from LDS.
+// The interaction set is exactly the reference's.
 //
 // Accepted nodes and leaves are staged in LDS as ready records (x, y, z, G m), loaded by their
 // lanes in parallel; the wave then evaluates them with broadcast LDS reads.  A particle's own
@@ -29,6 +30,7 @@ namespace {
 constexpr int kSCap = 1024;                 // per-wave node stack
 constexpr int kFlushAt = 32;                // evaluate once either list holds more than this
 constexpr int kRCap = kFlushAt + 2 * kWave; // one round lists at most 128 records
+constexpr int kPend = kRCap - kWave;        // undecided nodes of one side, staged past the records
 
 struct WaveLds {
     int stack[kSCap];
@@ -37,6 +39,23 @@ struct WaveLds {
     float leafEps[kRCap];   // eps_j / 2
     uint32_t self[kWave];   // lane l's own leaf was listed
 };
+
+// Least float T with fl(theta2 * T) > e2 and T > 1e-20: for every float r2,
+// (e2 < fl(theta2 * r2)) && (r2 > 1e-20)  <=>  r2 >= T, because rounding is monotone.
+// Starts within a few ulp of e2 / theta2 and steps to the exact boundary.
+__device__ __forceinline__ float acceptThreshold(float e2, float theta2, float invTheta2) {
+    if (!(theta2 > 0.0f)) return __builtin_inff();
+    float x = e2 * invTheta2;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float d = __uint_as_float(__float_as_uint(x) - 1u);
+        if (x > 0.0f && theta2 * d > e2) x = d;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (!(theta2 * x > e2)) x = __uint_as_float(__float_as_uint(x) + 1u);
+    return fmaxf(x, __uint_as_float(__float_as_uint(1.0e-20f) + 1u));
+}
 
 template <bool CHECK_INSIDE>
 __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
@@ -56,7 +75,7 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
     }
     WaveLds& L = lds[wave];
     const float theta2 = c.prm.theta2;
-    const float theta = sqrtf(theta2);
+    const float invTheta2 = theta2 > 0.0f ? 1.0f / theta2 : 0.0f;
     const float soft = c.prm.softFactor;
     float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float epsI = 1.0e-6f;
@@ -68,17 +87,10 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
     const float e2Node = __builtin_fmaf(epsI, epsI, 1.0e-20f);
     L.self[lane] = 0u;
 
-    // group bounds over the active particles: box [l, h], sphere (centre of the box, radius R)
+    // group bounding box over the active particles
     const float lx = waveMinF(active ? pi.x : 3.0e38f), hx = waveMaxF(active ? pi.x : -3.0e38f);
     const float ly = waveMinF(active ? pi.y : 3.0e38f), hy = waveMaxF(active ? pi.y : -3.0e38f);
     const float lz = waveMinF(active ? pi.z : 3.0e38f), hz = waveMaxF(active ? pi.z : -3.0e38f);
-    const float cx = 0.5f * (lx + hx), cy = 0.5f * (ly + hy), cz = 0.5f * (lz + hz);
-    float R;
-    {
-        const float dx = pi.x - cx, dy = pi.y - cy, dz = pi.z - cz;
-        R = waveMaxF(active ? sqrtf(dx * dx + dy * dy + dz * dz) : 0.0f);
-        R = R * (1.0f + 1.0e-5f) + 1.0e-30f;
-    }
 
     float ax = 0.0f, ay = 0.0f, az = 0.0f, phi = 0.0f;
     uint32_t listed = 0, forced = 0;
@@ -103,32 +115,33 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
                 const bool isInt = has && cc >= 0;
                 float4 cm = make_float4(0.0f, 0.0f, 0.0f, 0.0f), mn = cm, mx = cm;
                 float4 rec = cm;
-                float leafHalfEps = 0.0f;
+                float leafHalfEps = 0.0f, T = 0.0f;
                 bool accept = false, undecided = false;
                 if (isInt) {
                     cm = c.t.cm[cc];
                     mn = c.t.bmin[cc];
                     if (CHECK_INSIDE) mx = c.t.bmax[cc];
-                    const float ext = mn.w;
-                    const float ddx = cx - cm.x, ddy = cy - cm.y, ddz = cz - cm.z;
-                    const float d = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
-                    // distance from the com to the group box, and to its farthest corner
-                    const float gx = fmaxf(0.0f, fmaxf(lx - cm.x, cm.x - hx));
-                    const float gy = fmaxf(0.0f, fmaxf(ly - cm.y, cm.y - hy));
-                    const float gz = fmaxf(0.0f, fmaxf(lz - cm.z, cm.z - hz));
-                    const float fx = fmaxf(fabsf(cm.x - lx), fabsf(cm.x - hx));
-                    const float fy = fmaxf(fabsf(cm.y - ly), fabsf(cm.y - hy));
-                    const float fz = fmaxf(fabsf(cm.z - lz), fabsf(cm.z - hz));
-                    const float dBox = sqrtf(gx * gx + gy * gy + gz * gz);
-                    const float dFar = sqrtf(fx * fx + fy * fy + fz * fz);
-                    const float slack = 2.0e-6f * (d + R) + 1.0e-30f;
-                    const float dLo = fmaxf(d - R, dBox) - slack;
-                    const float dHi = fminf(d + R, dFar) + slack;
+                    {
+#pragma clang fp contract(off)
+                        const float e2 = mn.w * mn.w;
+                        T = acceptThreshold(e2, theta2, invTheta2);
+                    }
+                    // squared distance from the com to the group box (g2) and to its farthest
+                    // corner (f2); every particle's FP32 r^2 lies within 1e-6 (relative) of
+                    // [g2, f2], far inside the 4e-5 margins below
+                    const float ux = lx - cm.x, vx = cm.x - hx;
+                    const float uy = ly - cm.y, vy = cm.y - hy;
+                    const float uz = lz - cm.z, vz = cm.z - hz;
+                    const float gx = fmaxf(0.0f, fmaxf(ux, vx)), fx = fmaxf(fabsf(ux), fabsf(vx));
+                    const float gy = fmaxf(0.0f, fmaxf(uy, vy)), fy = fmaxf(fabsf(uy), fabsf(vy));
+                    const float gz = fmaxf(0.0f, fmaxf(uz, vz)), fz = fmaxf(fabsf(uz), fabsf(vz));
+                    const float g2 = gx * gx + gy * gy + gz * gz;
+                    const float f2 = fx * fx + fy * fy + fz * fz;
                     bool apart = true;  // no group particle can lie inside the node's box
                     if (CHECK_INSIDE)
                         apart = hx < mn.x || lx > mx.x || hy < mn.y || ly > mx.y || hz < mn.z || lz > mx.z;
-                    if (apart && dLo > 1.0e-9f && theta * dLo > ext * (1.0f + 1.0e-5f)) accept = true;
-                    else if (!(theta * dHi < ext * (1.0f - 1.0e-5f))) undecided = true;
+                    if (apart && g2 > T * (1.0f + 4.0e-5f)) accept = true;
+                    else if (!(f2 * (1.0f + 4.0e-5f) < T)) undecided = true;
                     rec = make_float4(cm.x, cm.y, cm.z, kGf * cm.w);
                 } else if (isLeaf) {
                     const uint32_t j = (uint32_t)~cc;
@@ -138,29 +151,32 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
                     const uint32_t rel = j - waveBase;
                     if (rel < 64u && ((am >> rel) & 1ull)) L.self[rel] = 1u;
                 }
-                // exact group test of the undecided nodes: lane = particle
-                uint64_t um = ballot(undecided);
-                while (um != 0ull) {
-                    const int src = (int)__builtin_ctzll(um);
-                    um &= um - 1ull;
-                    const float qx = readLaneF(cm.x, src), qy = readLaneF(cm.y, src), qz = readLaneF(cm.z, src);
-                    const float ext = readLaneF(mn.w, src);
-                    bool a;
+                // exact group test of the undecided nodes, one node at a time across the lanes
+                // (lane = particle); the nodes are staged in the free tail of the node list
+                const uint64_t um = ballot(undecided);
+                const int myPend = __popcll(um & lt);
+                if (undecided) L.node[kPend + myPend] = make_float4(cm.x, cm.y, cm.z, T);
+                __builtin_amdgcn_wave_barrier();
+                uint64_t okMask = 0ull, rest = um;
+                for (int u = 0; rest != 0ull; ++u) {
+                    const int src = (int)__builtin_ctzll(rest);
+                    rest &= rest - 1ull;
+                    const float4 q = L.node[kPend + u];
+                    bool fail;
                     {
 #pragma clang fp contract(off)
-                        const float r2 = dist2f(pi.x - qx, pi.y - qy, pi.z - qz);
-                        const float lhs = ext * ext;
-                        const float rhs = theta2 * r2;
-                        a = (lhs < rhs) && (r2 > 1.0e-20f);
+                        const float r2 = dist2f(pi.x - q.x, pi.y - q.y, pi.z - q.z);
+                        fail = !(r2 >= q.w);
                     }
-                    if (CHECK_INSIDE && a) {
+                    if (CHECK_INSIDE && !fail) {
                         const float x0 = readLaneF(mn.x, src), y0 = readLaneF(mn.y, src), z0 = readLaneF(mn.z, src);
                         const float x1 = readLaneF(mx.x, src), y1 = readLaneF(mx.y, src), z1 = readLaneF(mx.z, src);
-                        if (pi.x >= x0 && pi.x <= x1 && pi.y >= y0 && pi.y <= y1 && pi.z >= z0 && pi.z <= z1) a = false;
+                        fail = pi.x >= x0 && pi.x <= x1 && pi.y >= y0 && pi.y <= y1 && pi.z >= z0 && pi.z <= z1;
                     }
-                    const bool all = ballot(active && !a) == 0ull;
-                    if (lane == src) accept = all;
+                    if ((ballot(fail) & am) == 0ull) okMask |= 1ull << u;
                 }
+                if (undecided) accept = ((okMask >> myPend) & 1ull) != 0ull;
+                __builtin_amdgcn_wave_barrier();
                 bool toPush = isInt && !accept;
                 const uint64_t mPush = ballot(toPush);
                 const int nPush = __popcll(mPush);
@@ -190,7 +206,7 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
             const float4 q = L.node[e];
             const float dx = pi.x - q.x, dy = pi.y - q.y, dz = pi.z - q.z;
             const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, e2Node)));
-            const float ir = rsqrtf(d2);
+            const float ir = __builtin_amdgcn_rsqf(d2);  // d2 >= 1e-20: never denormal
             const float t = q.w * ir;
             const float f = t * (ir * ir);
             ax = __builtin_fmaf(-f, dx, ax);
@@ -205,7 +221,7 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
             const float dx = pi.x - q.x, dy = pi.y - q.y, dz = pi.z - q.z;
             const float e2 = __builtin_fmaf(eij, eij, 1.0e-20f);
             const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, e2)));
-            const float ir = rsqrtf(d2);
+            const float ir = __builtin_amdgcn_rsqf(d2);
             const float t = q.w * ir;
             const float f = t * (ir * ir);
             ax = __builtin_fmaf(-f, dx, ax);
@@ -221,7 +237,7 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
     // the particle's own leaf: zero separation, so only its potential term is taken back
     // (the same expression as in the loop: eps_ii = eps_i)
     const bool selfListed = L.self[lane] != 0u;
-    if (active && selfListed) phi += (kGf * pi.w) * rsqrtf(__builtin_fmaf(epsI, epsI, 1.0e-20f));
+    if (active && selfListed) phi += (kGf * pi.w) * __builtin_amdgcn_rsqf(__builtin_fmaf(epsI, epsI, 1.0e-20f));
     if (active) c.d.grav[i] = make_float4(ax, ay, az, phi);
     // interactions = active lanes x listed records - the particles' own leaves
     const uint32_t selfSum = (uint32_t)__popcll(ballot(active && selfListed));

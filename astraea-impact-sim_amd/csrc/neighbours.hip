@@ -32,14 +32,18 @@ struct WaveLds {
     int candId[kCCap];
 };
 
-// Wide walk: the wave expands up to 64 stack nodes per round, one per lane.  A node is opened
-// when any group particle's sphere of radius max(Hb_p, node's max Hb) meets its box.  The box
-// distance between the group's bounding box and the node's box is, in FP32 as in exact arithmetic,
-// a lower bound of every particle's box distance (rounding is monotone), so nodes it puts out of
-// reach are closed for every particle.  The survivors are tested exactly, one node at a time
-// across the lanes (lane = particle), with the node's box broadcast from the lane that loaded it.
-// Leaf children are loaded by their lanes in parallel, filtered by the same box bound, and
-// staged in LDS, where every lane tests its own particle against them.
+// Wide walk: the wave expands up to 64 stack nodes per round, one per lane.  A node must be opened
+// when some group particle's sphere of radius max(Hb_p, node's max Hb) meets its box.  The group is
+// cut into four sub-groups of 16 Morton-consecutive particles.  In FP32 as in exact arithmetic
+// (rounding is monotone) the box distance between a sub-group's box and the node's box is a lower
+// bound of the box distance of each of its particles, so a node out of reach of every sub-group
+// is closed for every particle.  A compact sub-group (box sides within twice its largest support)
+// opens the nodes within its reach directly: the leaves are tested pair by pair afterwards, so the
+// table stays exact.  A sub-group that straddles a jump of the Morton curve has a box far larger
+// than its particles' spheres; nodes that only such a sub-group reaches are tested exactly, one
+// node at a time across the lanes (lane = particle), with the node's box broadcast from the lane
+// that loaded it.  Leaf children are loaded by their lanes in parallel, kept when within reach of
+// the group's box, and staged in LDS, where every lane tests its own particle against them.
 __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
     __shared__ WaveLds lds[kWavesPerBlock];
     const uint32_t b = xcdBlock(blockIdx.x, gridDim.x);
@@ -66,6 +70,29 @@ __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
         const float ly = waveMinF(active ? pi.y : 3.0e38f), hy = waveMaxF(active ? pi.y : -3.0e38f);
         const float lz = waveMinF(active ? pi.z : 3.0e38f), hz = waveMaxF(active ? pi.z : -3.0e38f);
         const float HbMax = waveMaxF(active ? Hb : 0.0f);
+        // sub-group boxes (lanes 16q .. 16q+15); a sub-group without particles gets an empty box
+        float sl[4][3], sh[4][3], sH[4];
+        bool compact[4];
+        {
+            float v[7] = {active ? pi.x : 3.0e38f, active ? pi.y : 3.0e38f, active ? pi.z : 3.0e38f,
+                          active ? pi.x : -3.0e38f, active ? pi.y : -3.0e38f, active ? pi.z : -3.0e38f,
+                          active ? Hb : 0.0f};
+            for (int o = 1; o < 16; o <<= 1) {
+                for (int a = 0; a < 3; ++a) v[a] = fminf(v[a], __shfl_xor(v[a], o, 64));
+                for (int a = 3; a < 7; ++a) v[a] = fmaxf(v[a], __shfl_xor(v[a], o, 64));
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    sl[q][a] = readLaneF(v[a], 16 * q);
+                    sh[q][a] = readLaneF(v[3 + a], 16 * q);
+                }
+                sH[q] = readLaneF(v[6], 16 * q);
+                const float side = fmaxf(fmaxf(sh[q][0] - sl[q][0], sh[q][1] - sl[q][1]), sh[q][2] - sl[q][2]);
+                compact[q] = side <= 2.0f * sH[q];  // an empty sub-group is compact and reaches nothing
+            }
+        }
         const uint64_t lt = (1ull << lane) - 1ull;
         int sp = 1, nl = 0;
         L.stack[0] = (int)off;
@@ -83,16 +110,22 @@ __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
                     const int cc = side == 0 ? ch.x : ch.y;
                     const bool isInt = has && cc >= 0;
                     bool isLeaf = has && cc < 0;
-                    float4 mn = make_float4(0.0f, 0.0f, 0.0f, 0.0f), mx = mn, q = mn;
-                    bool maybe = false;
+                    float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f), mn = q, mx = q;
+                    bool open = false, undecided = false;
                     if (isInt) {
                         mn = c.t.bmin[cc];
                         mx = c.t.bmax[cc];
-                        const float reach = fmaxf(HbMax, mx.w);
-                        const float gx = fmaxf(0.0f, fmaxf(mn.x - hx, lx - mx.x));
-                        const float gy = fmaxf(0.0f, fmaxf(mn.y - hy, ly - mx.y));
-                        const float gz = fmaxf(0.0f, fmaxf(mn.z - hz, lz - mx.z));
-                        maybe = dist2f(gx, gy, gz) <= reach * reach;
+#pragma unroll
+                        for (int g = 0; g < 4; ++g) {
+                            const float reach = fmaxf(sH[g], mx.w);
+                            const float gx = fmaxf(0.0f, fmaxf(mn.x - sh[g][0], sl[g][0] - mx.x));
+                            const float gy = fmaxf(0.0f, fmaxf(mn.y - sh[g][1], sl[g][1] - mx.y));
+                            const float gz = fmaxf(0.0f, fmaxf(mn.z - sh[g][2], sl[g][2] - mx.z));
+                            const bool hit = dist2f(gx, gy, gz) <= reach * reach;
+                            if (compact[g]) open = open || hit;
+                            else undecided = undecided || hit;
+                        }
+                        undecided = undecided && !open;
                     } else if (isLeaf) {
                         const uint32_t j = (uint32_t)~cc;
                         const float4 pj = c.cur.pos[j];
@@ -104,9 +137,8 @@ __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
                         const float gz = fmaxf(0.0f, fmaxf(lz - q.z, q.z - hz));
                         isLeaf = dist2f(gx, gy, gz) < reach * reach;
                     }
-                    // exact test of the surviving nodes: lane = particle
-                    bool open = false;
-                    uint64_t um = ballot(maybe);
+                    // exact test of the nodes only straddling sub-groups reach: lane = particle
+                    uint64_t um = ballot(undecided);
                     while (um != 0ull) {
                         const int src = (int)__builtin_ctzll(um);
                         um &= um - 1ull;
