@@ -84,6 +84,32 @@ __device__ __forceinline__ const uint32_t* tableRow(const Neighbours& nb, uint32
     return nb.list + (uint64_t)(i >> 6) * (64ull * nb.stride) + (i & 63u);
 }
 
+// Walks the table row of slot i in batches of U entries, in table order.  The indices of the next
+// batch and the neighbour records of the current one are requested together, so a lane keeps
+// about 2U loads in flight instead of one dependent chain per entry.  load(j, rec) gathers the
+// record of neighbour j, pair(rec) consumes it.
+template <int U, class Rec, class Load, class Pair>
+__device__ __forceinline__ void forEachNeighbour(const Neighbours& nb, uint32_t i, Load&& load, Pair&& pair) {
+    const uint32_t cnt = nb.count[i];
+    const uint32_t* lst = tableRow(nb, i);
+    uint32_t jn[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) jn[u] = (uint32_t)u < cnt ? lst[64ull * u] : 0u;
+    for (uint32_t k = 0; k < cnt; k += U) {
+        Rec rec[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t j = jn[u];
+            const uint32_t kn = k + U + u;
+            jn[u] = kn < cnt ? lst[64ull * kn] : 0u;
+            if (k + u < cnt) load(j, rec[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (k + u < cnt) pair(rec[u]);
+    }
+}
+
 struct Segments {
     uint32_t count;       // number of segments
     uint32_t* off;        // [count+1] padded slot offsets
@@ -183,6 +209,27 @@ __device__ __forceinline__ float wendlandGrad(float r, float H) {
     if (r >= H || H <= 0.0f) return 0.0f;
     const float q = r / H, o = 1.0f - q, H2 = H * H;
     return -(210.0f / (kPiF * H2 * H2 * H)) * o * o * o;
+}
+
+// The same kernels for the pair loops, from the per-particle constants of a support H:
+// invH = 1/H, wNorm = 21/(2 pi H^3), gNorm = -210/(pi H^5).  They use the hardware
+// reciprocal and square root (about 1 ulp), and vanish for r >= H as above.
+struct Support {
+    float invH, wNorm, gNorm;
+};
+__device__ __forceinline__ float fastRcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fastSqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ Support supportOf(float H) {
+    const float iv = fastRcp(H), iv2 = iv * iv;
+    return Support{iv, (21.0f / (2.0f * kPiF)) * iv2 * iv, (-210.0f / kPiF) * iv2 * iv2 * iv};
+}
+__device__ __forceinline__ float wendlandS(float r, const Support& s) {
+    const float q = r * s.invH, o = fmaxf(0.0f, 1.0f - q), o2 = o * o;
+    return s.wNorm * (o2 * o2) * (1.0f + 4.0f * q);
+}
+__device__ __forceinline__ float wendlandGradS(float r, const Support& s) {
+    const float o = fmaxf(0.0f, 1.0f - r * s.invH);
+    return s.gNorm * (o * o * o);
 }
 
 // Strength table (engine/sph_gravity.cu:35-73) gated by the params' strength mask.

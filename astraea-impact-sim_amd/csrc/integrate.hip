@@ -180,25 +180,33 @@ __global__ __launch_bounds__(kBlock) void kDrift(DeviceCtx c) {
         const float hi = fmaxf(c.cur.hu[i].x, 1.0e-6f), Hi = 2.0f * hi;
         const float rhoI = fmaxf(vi.w, 1.0e-9f);
         float xs = 0.0f, ys = 0.0f, zs = 0.0f;
-        const uint32_t cnt = c.nb.count[i];
-        const uint32_t* lst = tableRow(c.nb, i);
-        for (uint32_t k = 0; k < cnt; ++k) {
-            const uint32_t j = lst[64ull * k];
-            const float4 pj = c.cur.pos[j];
-            const float Hj = 2.0f * fmaxf(c.d.nb[j].x, 1.0e-6f);
-            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            const float r2 = dist2f(dx, dy, dz);
-            const float Hm = fmaxf(Hi, Hj);
-            if (!(r2 < Hm * Hm)) continue;
-            const float4 vj = c.cur.vel[j];
-            const float r = sqrtf(r2);
-            const float wb = 0.5f * (wendland(r, Hi) + wendland(r, Hj));
-            const float rb = 0.5f * (rhoI + fmaxf(vj.w, 1.0e-9f));
-            const float co = (pj.w / rb) * wb;
-            xs += co * (vj.x - vi.x);
-            ys += co * (vj.y - vi.y);
-            zs += co * (vj.z - vi.z);
-        }
+        const Support sI = supportOf(Hi);
+        struct Rec {
+            float4 p, v;
+            float h;
+        };
+        forEachNeighbour<4, Rec>(
+            c.nb, i,
+            [&](uint32_t j, Rec& r) {
+                r.p = c.cur.pos[j];
+                r.v = c.cur.vel[j];
+                r.h = c.d.nb[j].x;
+            },
+            [&](const Rec& n) {
+                const float4 pj = n.p, vj = n.v;
+                const float Hj = 2.0f * fmaxf(n.h, 1.0e-6f);
+                const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                const float r2 = dist2f(dx, dy, dz);
+                const float Hm = fmaxf(Hi, Hj);
+                if (!(r2 < Hm * Hm)) return;
+                const float r = fastSqrt(r2);
+                const float wb = 0.5f * (wendlandS(r, sI) + wendlandS(r, supportOf(Hj)));
+                const float rb = 0.5f * (rhoI + fmaxf(vj.w, 1.0e-9f));
+                const float co = (pj.w * fastRcp(rb)) * wb;
+                xs += co * (vj.x - vi.x);
+                ys += co * (vj.y - vi.y);
+                zs += co * (vj.z - vi.z);
+            });
         const float dt = c.ss.dt[sg];
         if (FP64) {
             const size_t cap = (size_t)c.stride64;

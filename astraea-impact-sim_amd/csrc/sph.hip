@@ -61,31 +61,37 @@ __global__ __launch_bounds__(kBlock) void kPass1(DeviceCtx c) {
     float nsx = 0.0f, nsy = 0.0f, nsz = 0.0f;
     float gxx = 0.0f, gxy = 0.0f, gxz = 0.0f, gyx = 0.0f, gyy = 0.0f, gyz = 0.0f, gzx = 0.0f, gzy = 0.0f, gzz = 0.0f;
 
-    const uint32_t cnt = c.nb.count[i];
-    const uint32_t* lst = tableRow(c.nb, i);
-    for (uint32_t k = 0; k < cnt; ++k) {
-        const uint32_t j = lst[64ull * k];
-        const float4 pj = c.cur.pos[j];
-        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const float r2 = dist2f(dx, dy, dz);
-        if (r2 < Hi2) {
+    const Support sI = supportOf(Hi);
+    struct Rec {
+        float4 p, v;
+    };
+    forEachNeighbour<4, Rec>(
+        c.nb, i,
+        [&](uint32_t j, Rec& r) {
+            r.p = c.cur.pos[j];
+            r.v = c.cur.vel[j];
+        },
+        [&](const Rec& n) {
+            const float4 pj = n.p, vj = n.v;
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float r2 = dist2f(dx, dy, dz);
+            if (!(r2 < Hi2)) return;
             ++ngb;
-            const float4 vj = c.cur.vel[j];
-            const float r = sqrtf(r2);
-            const float volJ = pj.w / fmaxf(vj.w, 1.0e-9f);
-            const float w = wendland(r, Hi);
-            shep += volJ * w;
-            nsx -= volJ * dx * w;
-            nsy -= volJ * dy * w;
-            nsz -= volJ * dz * w;
-            const float gc = wendlandGrad(r, Hi);
+            const float r = fastSqrt(r2);
+            const float volJ = pj.w * fastRcp(fmaxf(vj.w, 1.0e-9f));
+            const float w = wendlandS(r, sI);
+            const float vw = volJ * w;
+            shep += vw;
+            nsx -= vw * dx;
+            nsy -= vw * dy;
+            nsz -= vw * dz;
+            const float gc = volJ * wendlandGradS(r, sI);
             const float gx = gc * dx, gy = gc * dy, gz = gc * dz;
             const float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
-            gxx += volJ * dvx * gx; gxy += volJ * dvx * gy; gxz += volJ * dvx * gz;
-            gyx += volJ * dvy * gx; gyy += volJ * dvy * gy; gyz += volJ * dvy * gz;
-            gzx += volJ * dvz * gx; gzy += volJ * dvz * gy; gzz += volJ * dvz * gz;
-        }
-    }
+            gxx += dvx * gx; gxy += dvx * gy; gxz += dvx * gz;
+            gyx += dvy * gx; gyy += dvy * gy; gyz += dvy * gz;
+            gzx += dvz * gx; gzy += dvz * gy; gzz += dvz * gz;
+        });
 
     const float normNs = sqrtf(nsx * nsx + nsy * nsy + nsz * nsz);
     float fs = fminf(1.0f, fmaxf(0.0f, 1.6f * fmaxf(1.0f - shep, normNs / hi) - 0.15f));
@@ -165,76 +171,89 @@ __global__ __launch_bounds__(kBlock) void kPass2(DeviceCtx c) {
     float gxx = 0.0f, gxy = 0.0f, gxz = 0.0f, gyx = 0.0f, gyy = 0.0f, gyz = 0.0f, gzx = 0.0f, gzy = 0.0f, gzz = 0.0f;
     uint32_t pairs = 0;
 
-    const uint32_t cnt = c.nb.count[i];
-    const uint32_t* lst = tableRow(c.nb, i);
-    for (uint32_t k = 0; k < cnt; ++k) {
-        const uint32_t j = lst[64ull * k];
-        const float4 pj = c.cur.pos[j];
-        const float4 nbj = c.d.nb[j];
-        const float hj = fmaxf(nbj.x, 1.0e-6f), Hj = 2.0f * hj;
-        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        const float r2 = dist2f(dx, dy, dz);
-        const float Hm = fmaxf(Hi, Hj);
-        if (!(r2 < Hm * Hm)) continue;
-        ++pairs;
-        const float4 vj = c.cur.vel[j];
-        const uint32_t matJ = c.cur.meta[j] & 0xffu;
-        const float mj = pj.w, rhoJ = fmaxf(vj.w, 1.0e-9f);
-        const float pr2J = nbj.y, csJ = nbj.z, fBalJ = nbj.w;
-        const float dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
-        const float vDotR = dvx * dx + dvy * dy + dvz * dz;
-        const float r = sqrtf(r2);
-        const float gI = wendlandGrad(r, Hi), gJ = wendlandGrad(r, Hj);
-        const float gb = 0.5f * (gI + gJ);
-        const float gvx = gb * dx, gvy = gb * dy, gvz = gb * dz;
-        const float vdg = dvx * gvx + dvy * gvy + dvz * gvz;
-        const float volJ = mj / rhoJ;
-        dCont += volJ * vdg;
-        if (matJ == matI) {
-            const float rdg = -(dx * gvx + dy * gvy + dz * gvz);
-            dDiff += volJ * (2.0f * (rhoJ - rhoI) / (r2 + 0.01f * Hi * Hi)) * rdg;
-        }
-        float piIJ = 0.0f;
-        if (vDotR < 0.0f) {
-            const float hBar = 0.5f * (hi + hj), cBar = 0.5f * (csI + csJ), rhoBar = 0.5f * (rhoI + rhoJ);
-            const float mu = (hBar * vDotR) / (r2 + 0.01f * hBar * hBar);
-            piIJ = ((-cBar * mu + 2.0f * mu * mu) / rhoBar) * (0.5f * (fBalI + fBalJ));
-        }
-        float tens = 0.0f;
-        const float Rj = pr2J < 0.0f ? -0.3f * pr2J : 0.0f;
-        if (Ri + Rj > 0.0f) {
-            const float Hbar = 0.5f * (Hi + Hj);
-            if (r < Hbar) {
-                const float q = r / Hbar, o = 1.0f - q, o2 = o * o;
-                const float fr = (o2 * o2 * (1.0f + 4.0f * q)) * (1.0f / 0.6328125f);
-                const float fr2 = fr * fr;
-                tens = (Ri + Rj) * (fr2 * fr2);
+    const Support sI = supportOf(Hi);
+    const float dampI = 0.01f * Hi * Hi;
+    struct Rec {
+        float4 p, n, v, s0;
+        float2 s1;
+        uint32_t mat;
+    };
+    forEachNeighbour<STRENGTH ? 2 : 4, Rec>(
+        c.nb, i,
+        [&](uint32_t j, Rec& r) {
+            r.p = c.cur.pos[j];
+            r.n = c.d.nb[j];
+            r.v = c.cur.vel[j];
+            r.mat = c.cur.meta[j] & 0xffu;
+            if (STRENGTH) {
+                r.s0 = c.d.sr0[j];
+                r.s1 = c.d.sr1[j];
             }
-        }
-        const float iso = -mj * (pr2I + pr2J + piIJ + tens);
-        ax += iso * gvx;
-        ay += iso * gvy;
-        az += iso * gvz;
-        duP += mj * vdg;
-        duAv += 0.5f * mj * piIJ * vdg;
-        duT += 0.5f * mj * tens * vdg;
-        if (STRENGTH) {
-            const float4 sj0 = c.d.sr0[j];
-            const float2 sj1 = c.d.sr1[j];
-            const float sxx = srI0.x + sj0.x, syy = srI0.y + sj0.y, szz = srI0.z + sj0.z;
-            const float sxy = srI0.w + sj0.w, sxz = srI1.x + sj1.x, syz = srI1.y + sj1.y;
-            ax += mj * (sxx * gvx + sxy * gvy + sxz * gvz);
-            ay += mj * (sxy * gvx + syy * gvy + syz * gvz);
-            az += mj * (sxz * gvx + syz * gvy + szz * gvz);
-            const float wx = -dvx, wy = -dvy, wz = -dvz;
-            duS += mj * ((sI0.x * wx + sI0.w * wy + sI1.x * wz) * gvx + (sI0.w * wx + sI0.y * wy + sI1.y * wz) * gvy +
-                         (sI1.x * wx + sI1.y * wy + sI0.z * wz) * gvz);
-            const float gx = gI * dx, gy = gI * dy, gz = gI * dz;
-            gxx += volJ * wx * gx; gxy += volJ * wx * gy; gxz += volJ * wx * gz;
-            gyx += volJ * wy * gx; gyy += volJ * wy * gy; gyz += volJ * wy * gz;
-            gzx += volJ * wz * gx; gzy += volJ * wz * gy; gzz += volJ * wz * gz;
-        }
-    }
+        },
+        [&](const Rec& n) {
+            const float4 pj = n.p, nbj = n.n, vj = n.v;
+            const float hj = fmaxf(nbj.x, 1.0e-6f), Hj = 2.0f * hj;
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float r2 = dist2f(dx, dy, dz);
+            const float Hm = fmaxf(Hi, Hj);
+            if (!(r2 < Hm * Hm)) return;
+            ++pairs;
+            const float mj = pj.w, rhoJ = fmaxf(vj.w, 1.0e-9f);
+            const float pr2J = nbj.y, csJ = nbj.z, fBalJ = nbj.w;
+            const float dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
+            const float vDotR = dvx * dx + dvy * dy + dvz * dz;
+            const float r = fastSqrt(r2);
+            const float gI = wendlandGradS(r, sI), gJ = wendlandGradS(r, supportOf(Hj));
+            const float gb = 0.5f * (gI + gJ);
+            const float gvx = gb * dx, gvy = gb * dy, gvz = gb * dz;
+            const float vdg = dvx * gvx + dvy * gvy + dvz * gvz;
+            const float volJ = mj * fastRcp(rhoJ);
+            dCont += volJ * vdg;
+            if (n.mat == matI) {
+                const float rdg = -(dx * gvx + dy * gvy + dz * gvz);
+                dDiff += volJ * (2.0f * (rhoJ - rhoI) * fastRcp(r2 + dampI)) * rdg;
+            }
+            float piIJ = 0.0f;
+            if (vDotR < 0.0f) {
+                const float hBar = 0.5f * (hi + hj), cBar = 0.5f * (csI + csJ), rhoBar = 0.5f * (rhoI + rhoJ);
+                const float mu = (hBar * vDotR) * fastRcp(r2 + 0.01f * hBar * hBar);
+                piIJ = ((-cBar * mu + 2.0f * mu * mu) * fastRcp(rhoBar)) * (0.5f * (fBalI + fBalJ));
+            }
+            float tens = 0.0f;
+            const float Rj = pr2J < 0.0f ? -0.3f * pr2J : 0.0f;
+            if (Ri + Rj > 0.0f) {
+                const float Hbar = 0.5f * (Hi + Hj);
+                if (r < Hbar) {
+                    const float q = r * fastRcp(Hbar), o = fmaxf(0.0f, 1.0f - q), o2 = o * o;
+                    const float fr = (o2 * o2 * (1.0f + 4.0f * q)) * (1.0f / 0.6328125f);
+                    const float fr2 = fr * fr;
+                    tens = (Ri + Rj) * (fr2 * fr2);
+                }
+            }
+            const float iso = -mj * (pr2I + pr2J + piIJ + tens);
+            ax += iso * gvx;
+            ay += iso * gvy;
+            az += iso * gvz;
+            duP += mj * vdg;
+            duAv += 0.5f * mj * piIJ * vdg;
+            duT += 0.5f * mj * tens * vdg;
+            if (STRENGTH) {
+                const float4 sj0 = n.s0;
+                const float2 sj1 = n.s1;
+                const float sxx = srI0.x + sj0.x, syy = srI0.y + sj0.y, szz = srI0.z + sj0.z;
+                const float sxy = srI0.w + sj0.w, sxz = srI1.x + sj1.x, syz = srI1.y + sj1.y;
+                ax += mj * (sxx * gvx + sxy * gvy + sxz * gvz);
+                ay += mj * (sxy * gvx + syy * gvy + syz * gvz);
+                az += mj * (sxz * gvx + syz * gvy + szz * gvz);
+                const float wx = -dvx, wy = -dvy, wz = -dvz;
+                duS += mj * ((sI0.x * wx + sI0.w * wy + sI1.x * wz) * gvx + (sI0.w * wx + sI0.y * wy + sI1.y * wz) * gvy +
+                             (sI1.x * wx + sI1.y * wy + sI0.z * wz) * gvz);
+                const float gx = gI * dx, gy = gI * dy, gz = gI * dz;
+                gxx += volJ * wx * gx; gxy += volJ * wx * gy; gxz += volJ * wx * gz;
+                gyx += volJ * wy * gx; gyy += volJ * wy * gy; gyz += volJ * wy * gz;
+                gzx += volJ * wz * gx; gzy += volJ * wz * gy; gzz += volJ * wz * gz;
+            }
+        });
 
     const float4 g = c.d.grav[i];
     const float4 p1 = c.d.p1a[i];
