@@ -488,6 +488,7 @@ int astraea_create(astraea_ctx** out, int device, uint32_t capacity, uint32_t ma
     A(devAlloc(c, d.t.parentI, C));
     A(devAlloc(c, d.t.parentL, C));
     A(devAlloc(c, d.t.visited, C));
+    A(devAlloc(c, d.t.other, C));
     A(devAlloc(c, d.t.cm, C));
     A(devAlloc(c, d.t.bmin, C));
     A(devAlloc(c, d.t.bmax, C));

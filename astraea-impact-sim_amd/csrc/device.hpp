@@ -65,7 +65,8 @@ struct Tree {
     int2* child;          // [slot of internal node] (left, right)
     int32_t* parentI;     // parent of internal node
     int32_t* parentL;     // parent of leaf slot
-    uint32_t* visited;    // bottom-up arrival counters
+    uint32_t* visited;    // bottom-up arrival counters (nodes whose leaves span two moment chunks)
+    int32_t* other;       // slot of the far end of an internal node's leaf range (the near end is the node's own index)
     float4* cm;           // (com x, y, z, mass)
     float4* bmin;         // (min x, y, z, extent = max side)
     float4* bmax;         // (max x, y, z, max support bound of the subtree)

@@ -147,9 +147,12 @@ __global__ __launch_bounds__(kBlock) void kPermuteKarras(DeviceCtx c) {
         if (lLeaf) c.t.parentL[o + gamma] = (int)i; else c.t.parentI[o + gamma] = (int)i;
         if (rLeaf) c.t.parentL[o + gamma + 1] = (int)i; else c.t.parentI[o + gamma + 1] = (int)i;
         c.t.visited[i] = 0u;
+        c.t.other[i] = o + j;
         if (local == 0u) c.t.parentI[i] = -1;
     }
 }
+
+constexpr int kChunk = 1024;  // leaf slots per block of the moments climb
 
 struct NodeRec {
     float4 cm, mn, mx;
@@ -192,44 +195,94 @@ __device__ __forceinline__ NodeRec loadChild(const DeviceCtx& c, int ch) {
     return r;
 }
 
-// Leaf-to-root climb over the permuted state (the host has swapped the sets: `cur` is sorted).
-// The first thread to reach a node stops; the second combines both children.  A thread's
-// record stores complete (s_waitcnt) before its arrival increment, so the second arriver's
-// coherent loads see them.
-__global__ __launch_bounds__(kBlock) void kMoments(DeviceCtx c) {
+// Combine of two children (the FP32 replica the oracle reproduces bit for bit).
+__device__ __forceinline__ NodeRec combine(const NodeRec& L, const NodeRec& R) {
 #pragma clang fp contract(off)
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    NodeRec o;
+    o.mn.x = fminf(L.mn.x, R.mn.x); o.mn.y = fminf(L.mn.y, R.mn.y); o.mn.z = fminf(L.mn.z, R.mn.z);
+    o.mx.x = fmaxf(L.mx.x, R.mx.x); o.mx.y = fmaxf(L.mx.y, R.mx.y); o.mx.z = fmaxf(L.mx.z, R.mx.z);
+    o.mx.w = fmaxf(L.mx.w, R.mx.w);
+    const float mS = L.cm.w + R.cm.w;
+    if (mS > 0.0f) {
+        const float invM = __fdiv_rn(1.0f, mS);
+        o.cm.x = __builtin_fmaf(L.cm.w, L.cm.x, R.cm.w * R.cm.x) * invM;
+        o.cm.y = __builtin_fmaf(L.cm.w, L.cm.y, R.cm.w * R.cm.y) * invM;
+        o.cm.z = __builtin_fmaf(L.cm.w, L.cm.z, R.cm.w * R.cm.z) * invM;
+    } else {
+        o.cm.x = 0.5f * (L.cm.x + R.cm.x);
+        o.cm.y = 0.5f * (L.cm.y + R.cm.y);
+        o.cm.z = 0.5f * (L.cm.z + R.cm.z);
+    }
+    o.cm.w = mS;
+    o.mn.w = fmaxf(o.mx.x - o.mn.x, fmaxf(o.mx.y - o.mn.y, o.mx.z - o.mn.z));
+    return o;
+}
+
+__device__ __forceinline__ NodeRec leafRec(const DeviceCtx& c, int j) {
+    const float4 p = c.cur.pos[j];
+    const float4 hu = c.cur.hu[j];
+    return NodeRec{p, make_float4(p.x, p.y, p.z, 0.0f), make_float4(p.x, p.y, p.z, supportBound(hu.x, hu.y))};
+}
+
+__device__ __forceinline__ bool inChunk(const DeviceCtx& c, int p, int base) {
+    const int o = c.t.other[p];
+    const int lo = p < o ? p : o, hi = p < o ? o : p;
+    return lo >= base && hi < base + kChunk;
+}
+
+// Leaf-to-root climb over the permuted state (the host has swapped the sets: `cur` is sorted).
+// The first thread to reach a node stops; the second combines both children.
+//
+// A block takes a chunk of kChunk consecutive leaf slots.  A node whose leaves all lie in the
+// chunk (its index is one end of its range, so it is in the chunk too) is reached only by the
+// block's own threads: its arrival counter and record live in LDS.  Only the nodes whose range
+// spans two chunks (a few percent) use the global protocol: records written and read with
+// agent-scope relaxed atomics (global stores/loads with the sc1 coherence bit, visible across the
+// XCD-private L2 caches without a whole-L2 write-back), and a thread's record stores completed
+// (s_waitcnt) before its arrival increment.
+__global__ __launch_bounds__(kChunk) void kMoments(DeviceCtx c) {
+    __shared__ uint32_t arrive[kChunk];
+    __shared__ NodeRec rec[kChunk];
+    const int base = (int)(blockIdx.x * kChunk);
+    const uint32_t i = (uint32_t)base + threadIdx.x;
+    arrive[threadIdx.x] = 0u;
+    __syncthreads();
+    if (i >= c.slots) return;
     const uint32_t sg = c.seg.slotSeg[i];
     const uint32_t n = c.seg.n[sg];
     if (i - c.seg.off[sg] >= n || n < 2u) return;
     int p = c.t.parentL[i];
+    bool local = true;
     while (p >= 0) {
-        __builtin_amdgcn_s_waitcnt(0);
-        const uint32_t prior = __hip_atomic_fetch_add(&c.t.visited[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (prior == 0u) return;
+        local = local && inChunk(c, p, base);
         const int2 ch = c.t.child[p];
-        const NodeRec L = loadChild(c, ch.x), R = loadChild(c, ch.y);
-        float4 mn, mx, cm;
-        mn.x = fminf(L.mn.x, R.mn.x); mn.y = fminf(L.mn.y, R.mn.y); mn.z = fminf(L.mn.z, R.mn.z);
-        mx.x = fmaxf(L.mx.x, R.mx.x); mx.y = fmaxf(L.mx.y, R.mx.y); mx.z = fmaxf(L.mx.z, R.mx.z);
-        mx.w = fmaxf(L.mx.w, R.mx.w);
-        const float mS = L.cm.w + R.cm.w;
-        if (mS > 0.0f) {
-            const float invM = __fdiv_rn(1.0f, mS);
-            cm.x = __builtin_fmaf(L.cm.w, L.cm.x, R.cm.w * R.cm.x) * invM;
-            cm.y = __builtin_fmaf(L.cm.w, L.cm.y, R.cm.w * R.cm.y) * invM;
-            cm.z = __builtin_fmaf(L.cm.w, L.cm.z, R.cm.w * R.cm.z) * invM;
+        NodeRec out;
+        if (local) {
+            const uint32_t prior =
+                __hip_atomic_fetch_add(&arrive[p - base], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (prior == 0u) return;
+            const NodeRec L = ch.x < 0 ? leafRec(c, ~ch.x) : rec[ch.x - base];
+            const NodeRec R = ch.y < 0 ? leafRec(c, ~ch.y) : rec[ch.y - base];
+            out = combine(L, R);
+            rec[p - base] = out;
         } else {
-            cm.x = 0.5f * (L.cm.x + R.cm.x);
-            cm.y = 0.5f * (L.cm.y + R.cm.y);
-            cm.z = 0.5f * (L.cm.z + R.cm.z);
+            __builtin_amdgcn_s_waitcnt(0);
+            const uint32_t prior = __hip_atomic_fetch_add(&c.t.visited[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (prior == 0u) return;
+            out = combine(loadChild(c, ch.x), loadChild(c, ch.y));
         }
-        cm.w = mS;
-        mn.w = fmaxf(mx.x - mn.x, fmaxf(mx.y - mn.y, mx.z - mn.z));
-        storeAgent(&c.t.cm[p], cm);
-        storeAgent(&c.t.bmin[p], mn);
-        storeAgent(&c.t.bmax[p], mx);
-        p = c.t.parentI[p];
+        const int pp = c.t.parentI[p];
+        if (pp >= 0 && !(local && inChunk(c, pp, base))) {
+            // the parent spans chunks: another block may combine it
+            storeAgent(&c.t.cm[p], out.cm);
+            storeAgent(&c.t.bmin[p], out.mn);
+            storeAgent(&c.t.bmax[p], out.mx);
+        } else {
+            c.t.cm[p] = out.cm;
+            c.t.bmin[p] = out.mn;
+            c.t.bmax[p] = out.mx;
+        }
+        p = pp;
     }
 }
 
@@ -267,6 +320,8 @@ void launchBoundsReduce(const DeviceCtx& c, hipStream_t s) { kBoundsReduce<<<c.s
 void launchPrepareSort(const DeviceCtx& c, hipStream_t s) { kPrepareSort<<<c.blocks, kBlock, 0, s>>>(c); }
 void launchMorton(const DeviceCtx& c, hipStream_t s) { kMorton<<<c.blocks, kBlock, 0, s>>>(c); }
 void launchPermuteKarras(const DeviceCtx& c, hipStream_t s) { kPermuteKarras<<<c.blocks, kBlock, 0, s>>>(c); }
-void launchMoments(const DeviceCtx& c, hipStream_t s) { kMoments<<<c.blocks, kBlock, 0, s>>>(c); }
+void launchMoments(const DeviceCtx& c, hipStream_t s) {
+    kMoments<<<(c.slots + kChunk - 1) / kChunk, kChunk, 0, s>>>(c);
+}
 
 }  // namespace astraea_hip
