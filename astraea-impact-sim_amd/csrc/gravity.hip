@@ -157,23 +157,50 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
                 const int myPend = __popcll(um & lt);
                 if (undecided) L.node[kPend + myPend] = make_float4(cm.x, cm.y, cm.z, T);
                 __builtin_amdgcn_wave_barrier();
-                uint64_t okMask = 0ull, rest = um;
-                for (int u = 0; rest != 0ull; ++u) {
-                    const int src = (int)__builtin_ctzll(rest);
-                    rest &= rest - 1ull;
-                    const float4 q = L.node[kPend + u];
-                    bool fail;
-                    {
+                uint64_t okMask = 0ull;
+                if (CHECK_INSIDE) {
+                    uint64_t rest = um;
+                    for (int u = 0; rest != 0ull; ++u) {
+                        const int src = (int)__builtin_ctzll(rest);
+                        rest &= rest - 1ull;
+                        const float4 q = L.node[kPend + u];
+                        bool fail;
+                        {
 #pragma clang fp contract(off)
-                        const float r2 = dist2f(pi.x - q.x, pi.y - q.y, pi.z - q.z);
-                        fail = !(r2 >= q.w);
+                            const float r2 = dist2f(pi.x - q.x, pi.y - q.y, pi.z - q.z);
+                            fail = !(r2 >= q.w);
+                        }
+                        if (!fail) {
+                            const float x0 = readLaneF(mn.x, src), y0 = readLaneF(mn.y, src), z0 = readLaneF(mn.z, src);
+                            const float x1 = readLaneF(mx.x, src), y1 = readLaneF(mx.y, src), z1 = readLaneF(mx.z, src);
+                            fail = pi.x >= x0 && pi.x <= x1 && pi.y >= y0 && pi.y <= y1 && pi.z >= z0 && pi.z <= z1;
+                        }
+                        if ((ballot(fail) & am) == 0ull) okMask |= 1ull << u;
                     }
-                    if (CHECK_INSIDE && !fail) {
-                        const float x0 = readLaneF(mn.x, src), y0 = readLaneF(mn.y, src), z0 = readLaneF(mn.z, src);
-                        const float x1 = readLaneF(mx.x, src), y1 = readLaneF(mx.y, src), z1 = readLaneF(mx.z, src);
-                        fail = pi.x >= x0 && pi.x <= x1 && pi.y >= y0 && pi.y <= y1 && pi.z >= z0 && pi.z <= z1;
+                } else {
+                    // four nodes per iteration: their LDS reads overlap; bits past the pending
+                    // count are never read
+                    const int nu = __popcll(um);
+                    for (int u = 0; u < nu; u += 4) {
+                        float4 q[4];
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const int at = kPend + u + v;  // slots past nu hold stale records
+                            q[v] = L.node[at < kRCap ? at : kRCap - 1];
+                        }
+                        uint32_t nib = 0u;
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            bool fail;
+                            {
+#pragma clang fp contract(off)
+                                const float r2 = dist2f(pi.x - q[v].x, pi.y - q[v].y, pi.z - q[v].z);
+                                fail = !(r2 >= q[v].w);
+                            }
+                            nib |= ((ballot(fail) & am) == 0ull ? 1u : 0u) << v;
+                        }
+                        okMask |= (uint64_t)nib << u;
                     }
-                    if ((ballot(fail) & am) == 0ull) okMask |= 1ull << u;
                 }
                 if (undecided) accept = ((okMask >> myPend) & 1ull) != 0ull;
                 __builtin_amdgcn_wave_barrier();
