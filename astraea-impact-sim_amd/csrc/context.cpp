@@ -898,6 +898,7 @@ int astraea_set_neighbour_stride(astraea_ctx* c, uint32_t stride) {
     if (!c) return fail(ASTRAEA_E_INVALID, "null context");
     if (stride == 0 || stride > kMaxNeighbourStride)
         return fail(ASTRAEA_E_INVALID, "neighbour stride %u outside [1, %u]", stride, kMaxNeighbourStride);
+    stride = roundUp(stride, 4);  // the table is kept in chunks of four entries
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));
     TRY(allocNeighbourList(c, stride));

@@ -76,37 +76,36 @@ struct Neighbours {
     uint32_t* count;      // entries of each slot (at most stride)
     uint32_t* waveSum;    // entries wanted by each wavefront's particles (uncapped)
     uint32_t* waveMax;    // largest entry count one particle of the wavefront wanted
-    uint32_t* list;       // entry k of slot (w*64 + l) at w*64*stride + k*64 + l
-    uint32_t stride;      // entries per slot
+    uint32_t* list;       // see tableChunks
+    uint32_t stride;      // entries per slot, a multiple of 4
 };
 
-// First table entry of slot i; entry k is at 64*k past it.
-__device__ __forceinline__ const uint32_t* tableRow(const Neighbours& nb, uint32_t i) {
-    return nb.list + (uint64_t)(i >> 6) * (64ull * nb.stride) + (i & 63u);
+// Table layout: entry k of slot i = 64 w + l is word (k & 3) of chunk 64 (k >> 2) + l of the
+// wavefront's block, which starts at word 64 w stride.  A lane's entries come in 16-byte chunks,
+// so the build writes one chunk per four entries and a pass reads four with one load; chunk r of
+// the 64 lanes is one contiguous kilobyte.
+__device__ __forceinline__ uint4* tableChunks(const Neighbours& nb, uint32_t i) {
+    return reinterpret_cast<uint4*>(nb.list + (uint64_t)(i >> 6) * (64ull * nb.stride)) + (i & 63u);
 }
 
-// Walks the table row of slot i in batches of U entries, in table order.  The indices of the next
-// batch and the neighbour records of the current one are requested together, so a lane keeps
-// about 2U loads in flight instead of one dependent chain per entry.  load(j, rec) gathers the
-// record of neighbour j, pair(rec) consumes it.
-template <int U, class Rec, class Load, class Pair>
+// Walks the table row of slot i four entries at a time, in table order.  The next chunk of
+// indices and the neighbour records of the current one are requested together, so a lane keeps
+// about eight loads in flight instead of one dependent chain per entry.  load(j, rec) gathers
+// the record of neighbour j, pair(rec) consumes it.
+template <class Rec, class Load, class Pair>
 __device__ __forceinline__ void forEachNeighbour(const Neighbours& nb, uint32_t i, Load&& load, Pair&& pair) {
     const uint32_t cnt = nb.count[i];
-    const uint32_t* lst = tableRow(nb, i);
-    uint32_t jn[U];
+    const uint4* ch = tableChunks(nb, i);
+    uint4 jn = cnt > 0u ? ch[0] : make_uint4(0u, 0u, 0u, 0u);
+    for (uint32_t k = 0; k < cnt; k += 4u) {
+        const uint32_t js[4] = {jn.x, jn.y, jn.z, jn.w};
+        jn = k + 4u < cnt ? ch[64ull * ((k >> 2) + 1u)] : make_uint4(0u, 0u, 0u, 0u);
+        Rec rec[4];
 #pragma unroll
-    for (int u = 0; u < U; ++u) jn[u] = (uint32_t)u < cnt ? lst[64ull * u] : 0u;
-    for (uint32_t k = 0; k < cnt; k += U) {
-        Rec rec[U];
+        for (int u = 0; u < 4; ++u)
+            if (k + u < cnt) load(js[u], rec[u]);
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint32_t j = jn[u];
-            const uint32_t kn = k + U + u;
-            jn[u] = kn < cnt ? lst[64ull * kn] : 0u;
-            if (k + u < cnt) load(j, rec[u]);
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
+        for (int u = 0; u < 4; ++u)
             if (k + u < cnt) pair(rec[u]);
     }
 }

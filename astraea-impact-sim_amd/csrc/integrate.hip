@@ -185,7 +185,7 @@ __global__ __launch_bounds__(kBlock) void kDrift(DeviceCtx c) {
             float4 p, v;
             float h;
         };
-        forEachNeighbour<4, Rec>(
+        forEachNeighbour<Rec>(
             c.nb, i,
             [&](uint32_t j, Rec& r) {
                 r.p = c.cur.pos[j];

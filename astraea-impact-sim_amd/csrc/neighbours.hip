@@ -13,10 +13,9 @@
 // sets are exactly the reference's.  Each lane then loops over its own list: the pair
 // physics runs only for real pairs.
 //
-// Layout: slot i owns `stride` entries; entry k of slot i = 64 w + l is at
-// 64 w stride + 64 k + l, so a pass reading entry k on every lane issues one coalesced load.
-// The table is written in a single walk; a slot that wants more than `stride` entries keeps the
-// first `stride`, and the overflow flag and the largest count wanted are left for the host.
+// Layout (tableChunks): slot i owns `stride` entries, in 16-byte chunks of four.  The table is
+// written in a single walk, a chunk at a time; a slot that wants more than `stride` entries keeps
+// the first `stride`, and the overflow flag and the largest count wanted are left for the host.
 #include "launch.hpp"
 
 namespace astraea_hip {
@@ -28,7 +27,7 @@ constexpr int kFlushAt = kCCap - 2 * kWave;  // one round lists at most 128 leav
 
 struct WaveLds {
     int stack[kSCap];
-    float4 cand[kCCap];  // candidate particle (x, y, z, support bound)
+    float4 cand[kCCap];  // candidate particle (x, y, z, squared support bound)
     int candId[kCCap];
 };
 
@@ -54,8 +53,8 @@ __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
     const uint32_t off = uniformU32(c.seg.off[sg]), n = uniformU32(c.seg.n[sg]);
     const bool active = i - off < n;
     const uint32_t K = c.nb.stride;
-    uint32_t* out = c.nb.list + (uint64_t)gw * (64ull * K) + (uint32_t)lane;
-    uint32_t cnt = 0;
+    uint4* out = tableChunks(c.nb, i);
+    uint32_t cnt = 0, b0 = 0, b1 = 0, b2 = 0, b3 = 0;  // the chunk being filled
     bool stackFull = false;
     if (ballot(active) != 0ull && n >= 2u) {
         WaveLds& L = lds[wave];
@@ -66,6 +65,7 @@ __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
             const float4 hu = c.cur.hu[i];
             Hb = supportBound(hu.x, hu.y);
         }
+        const float Hb2 = Hb * Hb;
         const float lx = waveMinF(active ? pi.x : 3.0e38f), hx = waveMaxF(active ? pi.x : -3.0e38f);
         const float ly = waveMinF(active ? pi.y : 3.0e38f), hy = waveMaxF(active ? pi.y : -3.0e38f);
         const float lz = waveMinF(active ? pi.z : 3.0e38f), hz = waveMaxF(active ? pi.z : -3.0e38f);
@@ -130,8 +130,9 @@ __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
                         const uint32_t j = (uint32_t)~cc;
                         const float4 pj = c.cur.pos[j];
                         const float4 hj = c.cur.hu[j];
-                        q = make_float4(pj.x, pj.y, pj.z, supportBound(hj.x, hj.y));
-                        const float reach = fmaxf(HbMax, q.w);
+                        const float hbj = supportBound(hj.x, hj.y);
+                        q = make_float4(pj.x, pj.y, pj.z, hbj * hbj);  // max(a, b)^2 = max(a^2, b^2) in FP32 too
+                        const float reach = fmaxf(HbMax, hbj);
                         const float gx = fmaxf(0.0f, fmaxf(lx - q.x, q.x - hx));
                         const float gy = fmaxf(0.0f, fmaxf(ly - q.y, q.y - hy));
                         const float gz = fmaxf(0.0f, fmaxf(lz - q.z, q.z - hz));
@@ -173,10 +174,14 @@ __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
             for (int e = 0; e < nl; ++e) {
                 const float4 q = L.cand[e];
                 const uint32_t jj = (uint32_t)L.candId[e];
-                const float rr = fmaxf(Hb, q.w);
                 const float r2 = dist2f(pi.x - q.x, pi.y - q.y, pi.z - q.z);
-                if (active && jj != i && r2 < rr * rr) {
-                    if (cnt < K) out[64ull * cnt] = jj;
+                if (active && jj != i && r2 < fmaxf(Hb2, q.w)) {
+                    const uint32_t s = cnt & 3u;
+                    b0 = s == 0u ? jj : b0;
+                    b1 = s == 1u ? jj : b1;
+                    b2 = s == 2u ? jj : b2;
+                    b3 = s == 3u ? jj : b3;
+                    if (s == 3u && cnt < K) out[64ull * (cnt >> 2)] = make_uint4(b0, b1, b2, b3);
                     ++cnt;
                 }
             }
@@ -185,6 +190,7 @@ __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
             if (sp == 0) break;
         }
     }
+    if ((cnt & 3u) != 0u && cnt < K) out[64ull * (cnt >> 2)] = make_uint4(b0, b1, b2, b3);
     c.nb.count[i] = cnt < K ? cnt : K;
     uint32_t sum = cnt;
     for (int o = 32; o > 0; o >>= 1) sum += (uint32_t)__shfl_xor((int)sum, o, 64);

@@ -65,7 +65,7 @@ __global__ __launch_bounds__(kBlock) void kPass1(DeviceCtx c) {
     struct Rec {
         float4 p, v;
     };
-    forEachNeighbour<4, Rec>(
+    forEachNeighbour<Rec>(
         c.nb, i,
         [&](uint32_t j, Rec& r) {
             r.p = c.cur.pos[j];
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(kBlock) void kPass2(DeviceCtx c) {
         float2 s1;
         uint32_t mat;
     };
-    forEachNeighbour<STRENGTH ? 2 : 4, Rec>(
+    forEachNeighbour<Rec>(
         c.nb, i,
         [&](uint32_t j, Rec& r) {
             r.p = c.cur.pos[j];

@@ -204,9 +204,9 @@ int astraea_energy(astraea_ctx* ctx, double* out);
  * largest candidate count of one particle in the last table.  Up to `n` values. */
 int astraea_counters(astraea_ctx* ctx, uint64_t* out, int n);
 
-/* Width of the neighbour table, in entries per particle (no counterpart in the
- * reference, which walks the tree in every pass).  The table takes
- * 4 * stride * capacity bytes.  A particle with more candidates than this makes
+/* Width of the neighbour table, in entries per particle, rounded up to a multiple
+ * of 4 (no counterpart in the reference, which walks the tree in every pass).  The
+ * table takes 4 * stride * capacity bytes.  A particle with more candidates than this makes
  * astraea_synchronize report ASTRAEA_E_OVERFLOW for the step and widen the table. */
 #define ASTRAEA_DEFAULT_NEIGHBOUR_STRIDE 320
 int astraea_set_neighbour_stride(astraea_ctx* ctx, uint32_t stride);
