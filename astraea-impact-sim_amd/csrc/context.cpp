@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -140,6 +141,11 @@ Params toDeviceParams(const astraea_params& p) {
     Params d;
     const float th = std::max(0.0f, p.theta);
     d.theta2 = th * th;
+    {
+        int e = 0;
+        d.theta2Pow2 = d.theta2 > 0.0f && std::isfinite(d.theta2) && std::frexp(d.theta2, &e) == 0.5f &&
+                       std::isnormal(d.theta2) ? 1u : 0u;
+    }
     d.softFactor = p.soft_factor;
     d.cfl = p.cfl_factor;
     d.gammaHHe = p.gamma_hhe;

@@ -137,6 +137,7 @@ struct Counters {
 
 struct Params {
     float theta2;
+    uint32_t theta2Pow2;  // theta^2 is a power of two: multiplying by it is exact
     float softFactor;
     float cfl;
     float gammaHHe;

@@ -43,9 +43,14 @@ struct WaveLds {
 // Least float T with fl(theta2 * T) > e2 and T > 1e-20: for every float r2,
 // (e2 < fl(theta2 * r2)) && (r2 > 1e-20)  <=>  r2 >= T, because rounding is monotone.
 // Starts within a few ulp of e2 / theta2 and steps to the exact boundary.
-__device__ __forceinline__ float acceptThreshold(float e2, float theta2, float invTheta2) {
+__device__ __forceinline__ float acceptThreshold(float e2, float theta2, float invTheta2, bool pow2) {
     if (!(theta2 > 0.0f)) return __builtin_inff();
+    const float floor20 = __uint_as_float(__float_as_uint(1.0e-20f) + 1u);
     float x = e2 * invTheta2;
+    if (pow2 && x < 3.0e38f) {
+        // theta^2 * x is exact, so the boundary is e2 / theta^2 itself and T is the next float
+        return fmaxf(__uint_as_float(__float_as_uint(x) + 1u), floor20);
+    }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const float d = __uint_as_float(__float_as_uint(x) - 1u);
@@ -54,7 +59,7 @@ __device__ __forceinline__ float acceptThreshold(float e2, float theta2, float i
 #pragma unroll
     for (int k = 0; k < 4; ++k)
         if (!(theta2 * x > e2)) x = __uint_as_float(__float_as_uint(x) + 1u);
-    return fmaxf(x, __uint_as_float(__float_as_uint(1.0e-20f) + 1u));
+    return fmaxf(x, floor20);
 }
 
 template <bool CHECK_INSIDE>
@@ -76,6 +81,7 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
     WaveLds& L = lds[wave];
     const float theta2 = c.prm.theta2;
     const float invTheta2 = theta2 > 0.0f ? 1.0f / theta2 : 0.0f;
+    const bool pow2 = c.prm.theta2Pow2 != 0u;
     const float soft = c.prm.softFactor;
     float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float epsI = 1.0e-6f;
@@ -124,7 +130,7 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
                     {
 #pragma clang fp contract(off)
                         const float e2 = mn.w * mn.w;
-                        T = acceptThreshold(e2, theta2, invTheta2);
+                        T = acceptThreshold(e2, theta2, invTheta2, pow2);
                     }
                     // squared distance from the com to the group box (g2) and to its farthest
                     // corner (f2); every particle's FP32 r^2 lies within 1e-6 (relative) of

@@ -199,6 +199,25 @@ def test_force_parity_strength(lib, orc):
     assert rel_rms(g, r) < 1e-5
 
 
+@pytest.mark.parametrize("theta", [0.45, 0.7])
+def test_gravity_parity_other_theta(lib, orc, theta):
+    """theta^2 not a power of two (the acceptance threshold is found by ulp steps), and
+    theta^2 > 1/3, where the AABB-inside test applies (sph_gravity.cu:855): the interaction
+    count equals the oracle's and the accelerations are within 1e-5 of FP64."""
+    from astraea_amd import scenario as S
+
+    st = perturbed_small_state()
+    n = len(st["mass"])
+    ctx, o = run_pair(lib, orc, st, [n], S.plain_params(theta=theta), [("sort_and_build",), ("gravity",)])
+    cg, co = ctx.counters(), o.counters()
+    assert cg["interactions"] == co["interactions"], (cg, co)
+    assert cg["forced_accepts"] == co["forced_accepts"] == 0
+    ag, ar = vec_by_id(ctx, o, ["acc_grav_x", "acc_grav_y", "acc_grav_z"])
+    assert rel_rms(ag, ar) < 1e-5
+    pg, pr = by_id(ctx, o, "phi", f64=True)
+    assert rel_rms(pg, pr) < 1e-5
+
+
 def test_force_parity_segments(lib, orc):
     from astraea_amd import scenario as S
 
