@@ -29,6 +29,7 @@ struct WaveLds {
     int stack[kSCap];
     float4 cand[kCCap];  // candidate particle (x, y, z, squared support bound)
     int candId[kCCap];
+    uint4 chunk[kWave];  // each lane's table chunk being filled
 };
 
 // Wide walk: the wave expands up to 64 stack nodes per round, one per lane.  A node must be opened
@@ -42,7 +43,9 @@ struct WaveLds {
 // than its particles' spheres; nodes that only such a sub-group reaches are tested exactly, one
 // node at a time across the lanes (lane = particle), with the node's box broadcast from the lane
 // that loaded it.  Leaf children are loaded by their lanes in parallel, kept when within reach of
-// the group's box, and staged in LDS, where every lane tests its own particle against them.
+// the group's box, and staged in LDS, where every lane tests its own particle against them:
+// 32 candidates into a hit mask, then the lane appends its hits in candidate order, so the
+// bookkeeping of an entry runs once per hit rather than once per candidate.
 __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
     __shared__ WaveLds lds[kWavesPerBlock];
     const uint32_t b = xcdBlock(blockIdx.x, gridDim.x);
@@ -54,10 +57,11 @@ __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
     const bool active = i - off < n;
     const uint32_t K = c.nb.stride;
     uint4* out = tableChunks(c.nb, i);
-    uint32_t cnt = 0, b0 = 0, b1 = 0, b2 = 0, b3 = 0;  // the chunk being filled
+    uint32_t cnt = 0;
     bool stackFull = false;
+    WaveLds& L = lds[wave];
+    uint32_t* chunk = reinterpret_cast<uint32_t*>(&L.chunk[lane]);
     if (ballot(active) != 0ull && n >= 2u) {
-        WaveLds& L = lds[wave];
         float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float Hb = 0.0f;
         if (active) {
@@ -169,19 +173,26 @@ __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
                 }
                 __builtin_amdgcn_wave_barrier();
             }
-            // candidates: every lane tests its own particle
-#pragma unroll 4
-            for (int e = 0; e < nl; ++e) {
-                const float4 q = L.cand[e];
-                const uint32_t jj = (uint32_t)L.candId[e];
-                const float r2 = dist2f(pi.x - q.x, pi.y - q.y, pi.z - q.z);
-                if (active && jj != i && r2 < fmaxf(Hb2, q.w)) {
-                    const uint32_t s = cnt & 3u;
-                    b0 = s == 0u ? jj : b0;
-                    b1 = s == 1u ? jj : b1;
-                    b2 = s == 2u ? jj : b2;
-                    b3 = s == 3u ? jj : b3;
-                    if (s == 3u && cnt < K) out[64ull * (cnt >> 2)] = make_uint4(b0, b1, b2, b3);
+            // candidates: every lane tests its own particle (r^2 < max(Hb_i, Hb_j)^2), 32 at a time;
+            // slots past nl hold stale records and are masked off
+            for (int c0 = 0; c0 < nl; c0 += 32) {
+                uint32_t m = 0u;
+#pragma unroll
+                for (int u = 0; u < 32; ++u) {
+                    const float4 q = L.cand[c0 + u];
+                    const float r2 = dist2f(pi.x - q.x, pi.y - q.y, pi.z - q.z);
+                    m |= (r2 < Hb2 || r2 < q.w) ? (1u << u) : 0u;
+                }
+                const int left = nl - c0;
+                if (left < 32) m &= (1u << left) - 1u;
+                if (!active) m = 0u;
+                while (m != 0u) {
+                    const int u = __builtin_ctz(m);
+                    m &= m - 1u;
+                    const uint32_t jj = (uint32_t)L.candId[c0 + u];
+                    if (jj == i) continue;
+                    chunk[cnt & 3u] = jj;
+                    if ((cnt & 3u) == 3u && cnt < K) out[64ull * (cnt >> 2)] = L.chunk[lane];
                     ++cnt;
                 }
             }
@@ -190,7 +201,7 @@ __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
             if (sp == 0) break;
         }
     }
-    if ((cnt & 3u) != 0u && cnt < K) out[64ull * (cnt >> 2)] = make_uint4(b0, b1, b2, b3);
+    if ((cnt & 3u) != 0u && cnt < K) out[64ull * (cnt >> 2)] = L.chunk[lane];
     c.nb.count[i] = cnt < K ? cnt : K;
     uint32_t sum = cnt;
     for (int o = 32; o > 0; o >>= 1) sum += (uint32_t)__shfl_xor((int)sum, o, 64);
