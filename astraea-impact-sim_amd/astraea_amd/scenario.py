@@ -16,7 +16,7 @@ import os
 
 import numpy as np
 
-from . import PREC_FP32, build_body, default_params, place_two_bodies, splitmix64
+from . import PREC_FP32, PREC_FP64_ACCUM, build_body, default_params, place_two_bodies, splitmix64
 
 R_EARTH_M = 6.371e6
 IRON, GRANITE, WATER_ICE, HHE, REGOLITH = 0, 1, 2, 3, 4
@@ -64,6 +64,15 @@ def plain_params(**kw):
 def config_small(seed: int = 42) -> dict:
     """Two 8,192-particle Fe/granite bodies, 45 deg, 1.1 v_esc (the CPU-runnable case)."""
     return impact_pair(8192, 8192, 45.0, 1.1, seed=seed)
+
+
+def config_strength(n_target: int = 3_600_000, n_impactor: int = 400_000, seed: int = 42, sep_radii: float = 2.0):
+    """Precision form: Fe 30 % / granite 60 % / water ice 10 % bodies with Weibull flaws (m = 9,
+    k = 4e29 per m^3), 30 deg, 1.5 v_esc.  Returns (state, params): strength and damage on for
+    granite and ice (Y0 = 3.5 GPa for granite in the strength table), FP64 leapfrog accumulators."""
+    st = impact_pair(n_target, n_impactor, 30.0, 1.5, seed=seed, layers=FE_GRANITE_ICE, sep_radii=sep_radii,
+                     flaws=True)
+    return st, plain_params(strength_mask=(1 << GRANITE) | (1 << WATER_ICE), precision=PREC_FP64_ACCUM)
 
 
 def config_giant(n_target: int = 900_000, n_impactor: int = 100_000, seed: int = 42) -> dict:

@@ -343,6 +343,35 @@ def test_graph_replay_matches_eager(lib):
         assert np.array_equal(out[0][k], out[1][k]), k
 
 
+def test_strength_damage_mixed_vs_fp64(lib, orc):
+    """Precision form (three materials, strength and Grady-Kipp damage on for granite and ice,
+    Weibull flaws, 30 deg, 1.5 v_esc), scaled to 6,600 particles and started at contact: 300 steps
+    of the mixed-precision GPU step against the FP64 oracle (damage starts after ~200 steps).
+    Bars: the energy drifts agree to 1e-4 (relative to |E0|), and the D^(1/3) fields by original
+    id differ by at most 0.01 in L1 (mean |D_gpu - D_fp64|; the growth switch is a threshold on
+    the strain, so single particles may flip)."""
+    from astraea_amd import scenario as S
+
+    st, p = S.config_strength(6000, 600, sep_radii=1.02)
+    n = len(st["mass"])
+    ctx, o = run_pair(lib, orc, st, [n], p, [("initial_forces",)])
+    tot = lambda e: e[:, 1] + e[:, 2] + e[:, 3] + e[:, 10]
+    g0, r0 = tot(ctx.energy()), tot(o.energy())
+    ctx.step(MAX_DT, 300, graph=True)
+    o.step(MAX_DT, 300)
+    ctx.synchronize()
+    g1, r1 = tot(ctx.energy()), tot(o.energy())
+    dg = (g1 - g0) / np.maximum(np.abs(g0), 1.0)
+    dr = (r1 - r0) / np.maximum(np.abs(r0), 1.0)
+    dG, dR = by_id(ctx, o, "damage_cbrt", f64=True)
+    l1 = float(np.mean(np.abs(dG - dR)))
+    print("drift gpu", dg, "fp64", dr, "D^(1/3) L1", l1, "max", dG.max(), dR.max(),
+          "damaged (D^(1/3) > 0.1)", int((dG > 0.1).sum()), int((dR > 0.1).sum()))
+    assert dR.max() > 0.2  # the case exercises damage
+    assert float(np.max(np.abs(dg - dr))) < 1e-4
+    assert l1 < 0.01
+
+
 def test_conservation_small_config(lib):
     """engine/headless_main.cpp:263-281 gates over 60 steps of the 2 x 8,192 impact."""
     from astraea_amd import scenario as S
