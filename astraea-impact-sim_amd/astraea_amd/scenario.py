@@ -22,6 +22,8 @@ R_EARTH_M = 6.371e6
 IRON, GRANITE, WATER_ICE, HHE, REGOLITH = 0, 1, 2, 3, 4
 FE_GRANITE = [(IRON, 0.3, 0.0), (GRANITE, 0.7, 0.0)]
 FE_GRANITE_ICE = [(IRON, 0.3, 0.0), (GRANITE, 0.6, 0.0), (WATER_ICE, 0.1, 0.0)]
+GAS_GIANT = [(IRON, 0.075, 0.0), (WATER_ICE, 0.075, 0.0), (HHE, 0.85, 0.0)]
+R_GAS_GIANT_M = 7.0e7  # surface of the n = 1 polytrope envelope (its mass, ~0.47 M_Jupiter, follows)
 
 
 def body_radius(mass_ratio: float, r_target_m: float) -> float:
@@ -73,6 +75,59 @@ def config_strength(n_target: int = 3_600_000, n_impactor: int = 400_000, seed: 
     st = impact_pair(n_target, n_impactor, 30.0, 1.5, seed=seed, layers=FE_GRANITE_ICE, sep_radii=sep_radii,
                      flaws=True)
     return st, plain_params(strength_mask=(1 << GRANITE) | (1 << WATER_ICE), precision=PREC_FP64_ACCUM)
+
+
+def uniform_body_radius(mass_kg: float, layers) -> float:
+    """Radius of a body of uniform-density layers with the given mass: such a body's mass scales
+    as R^3, so one small trial build fixes it."""
+    r0 = 1.0e6
+    m0 = build_body(64, r0, layers)["mass_kg"]
+    return r0 * (mass_kg / m0) ** (1.0 / 3.0)
+
+
+def gas_giant_pair(n_target: int, n_impactor: int, angle_deg: float, v_ratio: float, seed: int = 42,
+                   mass_ratio: float = 0.05, sep_radii: float = 2.0) -> dict:
+    """Gas-giant form: a target with an H/He envelope (85 % of the mass, n = 1 polytrope) around an
+    iron + water-ice core, hit by an iron/granite impactor of `mass_ratio` times its mass."""
+    t = build_body(n_target, R_GAS_GIANT_M, GAS_GIANT, seed=seed)
+    r_i = uniform_body_radius(mass_ratio * t["mass_kg"], FE_GRANITE)
+    i = build_body(n_impactor, r_i, FE_GRANITE, seed=seed + 1)
+    n = n_target + n_impactor
+
+    def cat(k):
+        return np.ascontiguousarray(np.concatenate([t[k], i[k]]).astype(np.float32))
+
+    st = {
+        "pos_x": cat("x"), "pos_y": cat("y"), "pos_z": cat("z"),
+        "vel_x": np.zeros(n, np.float32), "vel_y": np.zeros(n, np.float32), "vel_z": np.zeros(n, np.float32),
+        "mass": cat("m"), "h": cat("h"), "h0": cat("h"), "rho": cat("rho"), "u": cat("u"),
+        "eps_fail": cat("eps_fail"),
+        "material": np.ascontiguousarray(np.concatenate([t["mat"], i["mat"]])),
+        "body": np.concatenate([np.zeros(n_target, np.uint8), np.ones(n_impactor, np.uint8)]),
+    }
+    place_two_bodies(st, n_target, n_impactor, R_GAS_GIANT_M / 1.0e6, r_i / 1.0e6, v_ratio,
+                     math.sin(math.radians(angle_deg)), sep_radii)
+    return st
+
+
+def gas_giant_params(**kw):
+    """Gas-giant physics: H/He gamma = 5/3, surface cooling only above 2,000 K, FP64 accumulators."""
+    base = dict(gamma_hhe=5.0 / 3.0, cooling_min_temp_k=2000.0, precision=PREC_FP64_ACCUM)
+    base.update(kw)
+    return plain_params(**base)
+
+
+def gas_giant_sweep(count: int = 16, n_per: int = 500_000, master_seed: int = 42):
+    """Batched gas-giant cases: per scenario angle U[0, 60] deg, v U[1, 2] v_esc, impactor 0.05 of
+    the target's mass with n/20 particles, seeds splitmix64(master + k)."""
+    out = []
+    for k in range(count):
+        s = splitmix64(master_seed + k)
+        rng = np.random.default_rng(s)
+        n_i = max(16, n_per // 20)
+        out.append(dict(n_target=n_per - n_i, n_impactor=n_i, angle_deg=rng.uniform(0.0, 60.0),
+                        v_ratio=rng.uniform(1.0, 2.0), seed=int(s % (2**63))))
+    return out
 
 
 def config_giant(n_target: int = 900_000, n_impactor: int = 100_000, seed: int = 42) -> dict:

@@ -8,6 +8,14 @@
 // are put on the Keplerian approach orbit of Simulation::loadScenario
 // (engine/simulation.cpp:1389-1436) and shifted to zero COM and momentum
 // (simulation.cpp:919-944).
+//
+// A body whose outermost layer is H/He gets a gas envelope instead: the cold part of the H/He
+// EOS is the n = 1 polytrope P = K rho^2 (eos.cu:331), so the envelope in hydrostatic equilibrium
+// around uniform-density core layers solves d rho / dr = -G M(r) / (2 K r^2).  The total mass and
+// the density at the core's top are found by shooting, so that the envelope holds its mass
+// fraction and ends (rho = 0) at the requested radius.  Particles have equal masses; a uniform
+// lattice ball is stretched radially so that each particle's rank in radius matches its
+// enclosed mass, and u is the cold energy K rho (no thermal part, P = K rho^2 exactly).
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -75,6 +83,247 @@ double energyForPressure(int mat, double rhoPsu, double pSi, double tSurface, do
     return 0.5 * (lo + hi);
 }
 
+struct LatticePoint {
+    double x, y, z, r2;
+    uint64_t id;
+};
+
+// The n points of a seed-jittered HCP lattice of spacing s nearest to the origin, re-centred on
+// their centroid, in lattice order.  Returns false if the over-generated ball holds fewer than n.
+bool latticeBall(uint32_t n, double R, double jitter, uint64_t seed, std::vector<LatticePoint>& pts, double& s) {
+    const double V = 4.0 / 3.0 * kPi * R * R * R;
+    s = std::cbrt(std::sqrt(2.0) * V / (double)n);
+    const double dzL = s * std::sqrt(2.0 / 3.0), dyR = s * std::sqrt(3.0) / 2.0;
+    const double Rg = 1.15 * R + 2.0 * s;
+    const int kz = (int)std::ceil(Rg / dzL) + 1, jy = (int)std::ceil(Rg / dyR) + 1, ix = (int)std::ceil(Rg / s) + 1;
+    uint64_t rng = seed * 0x2545F4914F6CDD1DULL + 0x9E3779B97F4A7C15ULL;
+    pts.clear();
+    pts.reserve((size_t)(1.6 * n) + 64);
+    const double jit = jitter * s;
+    uint64_t id = 0;
+    for (int k = -kz; k <= kz; ++k) {
+        const bool odd = (k & 1) != 0;
+        const double ox = odd ? 0.5 * s : 0.0, oy = odd ? s / (2.0 * std::sqrt(3.0)) : 0.0;
+        const double z0 = k * dzL;
+        for (int j = -jy; j <= jy; ++j) {
+            const double y0 = j * dyR + oy;
+            const double xo = ((j & 1) ? 0.5 * s : 0.0) + ox;
+            for (int i = -ix; i <= ix; ++i) {
+                const double x0 = i * s + xo;
+                ++id;
+                if (x0 * x0 + y0 * y0 + z0 * z0 > Rg * Rg) continue;
+                uint64_t h = rng ^ (id * 0xD1B54A32D192ED03ULL);
+                const double jx = (2.0 * uniform01(h) - 1.0) * jit;
+                const double jy2 = (2.0 * uniform01(h) - 1.0) * jit;
+                const double jz = (2.0 * uniform01(h) - 1.0) * jit;
+                const double x = x0 + jx, y = y0 + jy2, z = z0 + jz;
+                pts.push_back({x, y, z, x * x + y * y + z * z, id});
+            }
+        }
+    }
+    if (pts.size() < n) return false;
+    std::nth_element(pts.begin(), pts.begin() + (n - 1), pts.end(), [](const LatticePoint& a, const LatticePoint& b) {
+        return a.r2 < b.r2 || (a.r2 == b.r2 && a.id < b.id);
+    });
+    pts.resize(n);
+    std::sort(pts.begin(), pts.end(), [](const LatticePoint& a, const LatticePoint& b) { return a.id < b.id; });
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    for (const LatticePoint& q : pts) { gx += q.x; gy += q.y; gz += q.z; }
+    gx /= n; gy /= n; gz /= n;
+    for (LatticePoint& q : pts) { q.x -= gx; q.y -= gy; q.z -= gz; q.r2 = q.x * q.x + q.y * q.y + q.z * q.z; }
+    return true;
+}
+
+// Envelope of a gas giant: RK4 on (rho, M) outward from the core's top, SI units.
+struct Envelope {
+    std::vector<double> r, m, rho;  // samples from the core's top to the surface
+    double rSurface = 0.0, mEnvelope = 0.0;
+};
+
+Envelope integrateEnvelope(double K, double rCore, double mCore, double rhoTop, double dr, bool keep) {
+    Envelope e;
+    double r = rCore, m = mCore, rho = rhoTop;
+    auto f = [&](double rr, double rh, double mm, double& dRho, double& dM) {
+        dRho = -kGsi * mm / (2.0 * K * rr * rr);
+        dM = 4.0 * kPi * rr * rr * std::max(rh, 0.0);
+    };
+    if (keep) { e.r.push_back(r); e.m.push_back(m); e.rho.push_back(rho); }
+    for (int it = 0; it < 4000000; ++it) {
+        double a1, b1, a2, b2, a3, b3, a4, b4;
+        f(r, rho, m, a1, b1);
+        f(r + 0.5 * dr, rho + 0.5 * dr * a1, m + 0.5 * dr * b1, a2, b2);
+        f(r + 0.5 * dr, rho + 0.5 * dr * a2, m + 0.5 * dr * b2, a3, b3);
+        f(r + dr, rho + dr * a3, m + dr * b3, a4, b4);
+        const double rn = rho + dr / 6.0 * (a1 + 2.0 * a2 + 2.0 * a3 + a4);
+        const double mn = m + dr / 6.0 * (b1 + 2.0 * b2 + 2.0 * b3 + b4);
+        if (rn <= 0.0) {
+            const double t = rho / (rho - rn);
+            e.rSurface = r + t * dr;
+            e.mEnvelope = m + t * (mn - m) - mCore;
+            if (keep) { e.r.push_back(e.rSurface); e.m.push_back(m + t * (mn - m)); e.rho.push_back(0.0); }
+            return e;
+        }
+        rho = rn; m = mn; r += dr;
+        if (keep) { e.r.push_back(r); e.m.push_back(m); e.rho.push_back(rho); }
+    }
+    e.rSurface = r;
+    e.mEnvelope = m - mCore;
+    return e;
+}
+
+int buildGasGiant(const astraea_body_spec* spec, uint32_t n, astraea_body_out* out, double* outMass, double* outSpacing) {
+    const int nl = (int)spec->num_layers;
+    const int nc = nl - 1;  // core layers
+    const double K = astraea_hip::material(3).kH2;
+    const double R = spec->radius_m;
+    double fsum = 0.0, rho[ASTRAEA_MAX_LAYERS], f[ASTRAEA_MAX_LAYERS];
+    for (int k = 0; k < nl; ++k) fsum += spec->layer_mass_frac[k];
+    if (!(fsum > 0.0)) return ASTRAEA_E_INVALID;
+    for (int k = 0; k < nl; ++k) {
+        f[k] = spec->layer_mass_frac[k] / fsum;
+        rho[k] = spec->layer_density_kgm3[k] > 0.0 ? spec->layer_density_kgm3[k]
+                                                   : astraea_hip::material(spec->layer_material[k]).rho0;
+    }
+    const double fEnv = f[nl - 1];
+    if (!(fEnv > 0.0 && fEnv < 1.0)) return ASTRAEA_E_INVALID;
+    const double rPure = kPi * std::sqrt(K / (2.0 * kPi * kGsi));
+    const double dr = rPure / 20000.0;
+    auto coreTop = [&](double M, double* radii) {
+        double v = 0.0;
+        for (int k = 0; k < nc; ++k) {
+            v += f[k] * M / rho[k];
+            if (radii) radii[k] = std::cbrt(3.0 * v / (4.0 * kPi));
+        }
+        return std::cbrt(3.0 * v / (4.0 * kPi));
+    };
+    // density at the core's top that puts the envelope's mass fraction inside its surface
+    auto solveTop = [&](double M) {
+        const double rc = coreTop(M, nullptr), mc = (1.0 - fEnv) * M;
+        double lo = std::log(1.0e-8), hi = std::log(1.0e8);
+        for (int it = 0; it < 90; ++it) {
+            const double mid = 0.5 * (lo + hi);
+            if (integrateEnvelope(K, rc, mc, std::exp(mid), dr, false).mEnvelope < fEnv * M) lo = mid; else hi = mid;
+        }
+        return std::exp(0.5 * (lo + hi));
+    };
+    // total mass with the surface at R (the surface radius grows with the mass: the core pushes out)
+    auto surfaceOf = [&](double M) {
+        const double top = solveTop(M);
+        return integrateEnvelope(K, coreTop(M, nullptr), (1.0 - fEnv) * M, top, dr, false).rSurface;
+    };
+    double lo = std::log(1.0e18), hi = std::log(1.0e31);
+    if (!(surfaceOf(std::exp(lo)) < R && surfaceOf(std::exp(hi)) > R)) return ASTRAEA_E_INVALID;
+    for (int it = 0; it < 70; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (surfaceOf(std::exp(mid)) < R) lo = mid; else hi = mid;
+    }
+    const double M = std::exp(0.5 * (lo + hi));
+    double rCore[ASTRAEA_MAX_LAYERS];
+    const double rc = coreTop(M, rCore);
+    const double rhoTop = solveTop(M);
+    const Envelope env = integrateEnvelope(K, rc, (1.0 - fEnv) * M, rhoTop, dr, true);
+
+    // core layers: uniform density, pressure from the envelope's base pressure K rho_top^2 inward
+    std::vector<Layer> L(nc);
+    double mEnc = 0.0, rPrev = 0.0;
+    for (int k = 0; k < nc; ++k) {
+        L[k] = {rPrev, rCore[k], rho[k], mEnc, spec->layer_material[k]};
+        mEnc += 4.0 / 3.0 * kPi * rho[k] * (rCore[k] * rCore[k] * rCore[k] - rPrev * rPrev * rPrev);
+        rPrev = rCore[k];
+    }
+    std::vector<double> pTop(nc, 0.0);
+    pTop[nc - 1] = K * rhoTop * rhoTop;
+    for (int k = nc - 2; k >= 0; --k) {
+        const Layer& l = L[k + 1];
+        const double kk = 4.0 / 3.0 * kPi * l.rho;
+        const double c = l.mEncIn - kk * l.rIn * l.rIn * l.rIn;
+        pTop[k] = pTop[k + 1] + kGsi * l.rho * (c * (1.0 / l.rIn - 1.0 / l.rOut) + 0.5 * kk * (l.rOut * l.rOut - l.rIn * l.rIn));
+    }
+
+    std::vector<LatticePoint> pts;
+    double s0 = 0.0;
+    if (!latticeBall(n, 1.0, spec->jitter, spec->seed, pts, s0)) return ASTRAEA_E_INVALID;
+    // rank in radius -> enclosed mass -> new radius
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return pts[a].r2 < pts[b].r2 || (pts[a].r2 == pts[b].r2 && pts[a].id < pts[b].id);
+    });
+    const double mP = M / (double)n;
+    const double mCoreTot = (1.0 - fEnv) * M;
+    std::vector<double> rNew(n);
+    for (uint32_t rank = 0; rank < n; ++rank) {
+        const double mt = ((double)rank + 0.5) * mP;
+        double r;
+        if (mt <= mCoreTot) {
+            // invert the piecewise-uniform core
+            int k = 0;
+            while (k + 1 < nc && mt > L[k + 1].mEncIn) ++k;
+            const double a = L[k].rIn;
+            r = std::cbrt(a * a * a + (mt - L[k].mEncIn) / (4.0 / 3.0 * kPi * L[k].rho));
+        } else {
+            const auto it = std::lower_bound(env.m.begin(), env.m.end(), mt);
+            const size_t j = (size_t)std::min<std::ptrdiff_t>(std::max<std::ptrdiff_t>(it - env.m.begin(), 1),
+                                                              (std::ptrdiff_t)env.m.size() - 1);
+            const double t = (mt - env.m[j - 1]) / std::max(env.m[j] - env.m[j - 1], 1.0e-300);
+            r = env.r[j - 1] + t * (env.r[j] - env.r[j - 1]);
+        }
+        rNew[order[rank]] = r;
+    }
+    const double rhoFloor = 1.0e-3 * rhoTop;
+    uint64_t frng = spec->seed ^ 0x5851F42D4C957F2DULL;
+    std::vector<double> px(n), py(n), pz(n);
+    double cx = 0.0, cy = 0.0, cz = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const double r0 = std::sqrt(pts[i].r2);
+        const double sc = r0 > 0.0 ? rNew[i] / r0 : 0.0;
+        px[i] = pts[i].x * sc; py[i] = pts[i].y * sc; pz[i] = pts[i].z * sc;
+        cx += px[i]; cy += py[i]; cz += pz[i];
+    }
+    cx /= n; cy /= n; cz /= n;
+    for (uint32_t i = 0; i < n; ++i) {
+        const double r = rNew[i];
+        int mat;
+        double rhoSi, u;
+        if (r <= rc && nc > 0) {
+            int k = 0;
+            while (k + 1 < nc && r > L[k].rOut) ++k;
+            mat = L[k].mat;
+            rhoSi = L[k].rho;
+            const double tS = spec->surface_temp_k > 0.0 ? spec->surface_temp_k : astraea_hip::material(mat).tRef;
+            u = energyForPressure(mat, rhoSi / astraea_hip::kRhoUnit, pressureAt(L, pTop, r), tS, 5.0 / 3.0);
+        } else {
+            mat = 3;
+            const auto it = std::lower_bound(env.r.begin(), env.r.end(), r);
+            const size_t j = (size_t)std::min<std::ptrdiff_t>(std::max<std::ptrdiff_t>(it - env.r.begin(), 1),
+                                                              (std::ptrdiff_t)env.r.size() - 1);
+            const double t = (r - env.r[j - 1]) / std::max(env.r[j] - env.r[j - 1], 1.0e-300);
+            rhoSi = std::max(env.rho[j - 1] + t * (env.rho[j] - env.rho[j - 1]), rhoFloor);
+            u = astraea_hip::coldEnergyPsu(3, rhoSi / astraea_hip::kRhoUnit);  // P = K rho^2, no thermal part
+        }
+        const double vCell = mP / rhoSi;
+        if (out->x) out->x[i] = (float)((px[i] - cx) / kLUnit);
+        if (out->y) out->y[i] = (float)((py[i] - cy) / kLUnit);
+        if (out->z) out->z[i] = (float)((pz[i] - cz) / kLUnit);
+        if (out->m) out->m[i] = (float)(mP / kMUnit);
+        if (out->h) out->h[i] = (float)(spec->h_over_spacing * std::cbrt(std::sqrt(2.0) * vCell) / kLUnit);
+        if (out->rho) out->rho[i] = (float)(rhoSi / astraea_hip::kRhoUnit);
+        if (out->u) out->u[i] = (float)u;
+        if (out->mat) out->mat[i] = (uint8_t)mat;
+        if (out->eps_fail) {
+            double e = 1.0;
+            if (spec->strength_flaws && mat != 3) {
+                const double U = std::max(uniform01(frng), 1.0e-300);
+                e = std::pow(-std::log(U) / (spec->weibull_k * vCell), 1.0 / spec->weibull_m);
+            }
+            out->eps_fail[i] = (float)e;
+        }
+    }
+    if (outMass) *outMass = M;
+    if (outSpacing) *outSpacing = std::cbrt(std::sqrt(2.0) * mP / rho[0]);
+    return ASTRAEA_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -107,6 +356,9 @@ int astraea_build_body(const astraea_body_spec* spec, uint32_t n, astraea_body_o
         !(spec->radius_m > 0.0))
         return ASTRAEA_E_INVALID;
     const int nl = (int)spec->num_layers;
+    for (int k = 0; k + 1 < nl; ++k)
+        if (spec->layer_material[k] == 3) return ASTRAEA_E_INVALID;  // H/He only as the outermost layer
+    if (spec->layer_material[nl - 1] == 3) return nl >= 2 ? buildGasGiant(spec, n, out, outMass, outSpacing) : ASTRAEA_E_INVALID;
     const double R = spec->radius_m;
     // Layer radii from mass fractions and densities: r_k^3 = R^3 * (sum_{j<=k} f_j/rho_j) / (sum_j f_j/rho_j)
     double fsum = 0.0, vsum = 0.0, rho[ASTRAEA_MAX_LAYERS];
@@ -136,54 +388,15 @@ int astraea_build_body(const astraea_body_spec* spec, uint32_t n, astraea_body_o
         pTop[k] = pTop[k + 1] + kGsi * l.rho * (c * (1.0 / l.rIn - 1.0 / l.rOut) + 0.5 * kk * (l.rOut * l.rOut - l.rIn * l.rIn));
     }
 
-    // HCP lattice: number density sqrt(2)/s^3; over-generate within 1.15 R and keep the n innermost.
+    // HCP lattice: number density sqrt(2)/s^3; the n points nearest the centre, centred; layers are
+    // assigned by radius, then the body is shifted to its centre of mass and the hydrostatic
+    // pressure is evaluated at the final radius
+    std::vector<LatticePoint> pts;
+    double s = 0.0;
+    if (!latticeBall(n, R, spec->jitter, spec->seed, pts, s)) return ASTRAEA_E_INVALID;
     const double V = 4.0 / 3.0 * kPi * R * R * R;
-    const double s = std::cbrt(std::sqrt(2.0) * V / (double)n);
-    const double dzL = s * std::sqrt(2.0 / 3.0), dyR = s * std::sqrt(3.0) / 2.0;
-    const double Rg = 1.15 * R + 2.0 * s;
-    const int kz = (int)std::ceil(Rg / dzL) + 1, jy = (int)std::ceil(Rg / dyR) + 1, ix = (int)std::ceil(Rg / s) + 1;
-    uint64_t rng = spec->seed * 0x2545F4914F6CDD1DULL + 0x9E3779B97F4A7C15ULL;
-    struct P { double x, y, z, r2; uint64_t id; };
-    std::vector<P> pts;
-    pts.reserve((size_t)(1.6 * n) + 64);
-    const double jit = spec->jitter * s;
-    uint64_t id = 0;
-    for (int k = -kz; k <= kz; ++k) {
-        const bool odd = (k & 1) != 0;
-        const double ox = odd ? 0.5 * s : 0.0, oy = odd ? s / (2.0 * std::sqrt(3.0)) : 0.0;
-        const double z0 = k * dzL;
-        for (int j = -jy; j <= jy; ++j) {
-            const double y0 = j * dyR + oy;
-            const double xo = ((j & 1) ? 0.5 * s : 0.0) + ox;
-            for (int i = -ix; i <= ix; ++i) {
-                const double x0 = i * s + xo;
-                ++id;
-                if (x0 * x0 + y0 * y0 + z0 * z0 > Rg * Rg) continue;
-                uint64_t h = rng ^ (id * 0xD1B54A32D192ED03ULL);
-                const double jx = (2.0 * uniform01(h) - 1.0) * jit;
-                const double jy2 = (2.0 * uniform01(h) - 1.0) * jit;
-                const double jz = (2.0 * uniform01(h) - 1.0) * jit;
-                const double x = x0 + jx, y = y0 + jy2, z = z0 + jz;
-                pts.push_back({x, y, z, x * x + y * y + z * z, id});
-            }
-        }
-    }
-    if (pts.size() < n) return ASTRAEA_E_INVALID;
-    std::nth_element(pts.begin(), pts.begin() + (n - 1), pts.end(),
-                     [](const P& a, const P& b) { return a.r2 < b.r2 || (a.r2 == b.r2 && a.id < b.id); });
-    pts.resize(n);
-    std::sort(pts.begin(), pts.end(), [](const P& a, const P& b) { return a.id < b.id; });
-
     const double vCell = V / (double)n;
     const double hPsu = spec->h_over_spacing * s / kLUnit;
-    // centre the selection, assign layers by radius, then shift to the centre of mass; the
-    // hydrostatic pressure is evaluated at the final radius
-    {
-        double gx = 0.0, gy = 0.0, gz = 0.0;
-        for (const P& q : pts) { gx += q.x; gy += q.y; gz += q.z; }
-        gx /= n; gy /= n; gz /= n;
-        for (P& q : pts) { q.x -= gx; q.y -= gy; q.z -= gz; q.r2 = q.x * q.x + q.y * q.y + q.z * q.z; }
-    }
     std::vector<double> mass(n);
     std::vector<int> lay(n);
     double cx = 0.0, cy = 0.0, cz = 0.0, mt = 0.0;

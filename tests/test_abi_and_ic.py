@@ -171,3 +171,53 @@ def test_two_body_placement(lib):
     # angular momentum of the relative orbit: |d x v| = rc * v_contact * b, b = sin 45 deg
     lrel = np.linalg.norm(np.cross(ci - ct, vi - vt))
     assert abs(lrel - rc * 1.1 * vesc * math.sin(math.radians(45))) < 1e-4 * lrel
+
+
+def test_gas_giant_envelope_hydrostatic(lib, orc):
+    """H/He envelope (85 % of the mass) around an iron + water-ice core: the n = 1 polytrope
+    P = K rho^2 of the H/He EOS in hydrostatic equilibrium, re-solved here independently
+    (scipy's RK45 and a root find on the density at the core's top) for the mass the builder
+    returned: the surface lands at the requested radius, and every envelope particle's density
+    matches the profile at its radius, with u = K rho (no thermal part, so P_EOS = K rho^2)."""
+    from scipy.integrate import solve_ivp
+    from scipy.optimize import brentq
+    from astraea_amd import scenario as S
+
+    n, R = 12000, S.R_GAS_GIANT_M
+    b = lib.build_body(n, R, S.GAS_GIANT, seed=5)
+    G, K = 6.6743e-11, 2.08e5
+    M = b["mass_kg"]
+    m = b["m"].astype(np.float64)
+    assert abs(m.sum() * 1e24 - M) <= 1e-5 * M
+    counts = np.bincount(b["mat"], minlength=4)
+    assert abs(counts[0] - 0.075 * n) <= 1 and abs(counts[2] - 0.075 * n) <= 1 and abs(counts[3] - 0.85 * n) <= 1
+    rc = (3 / (4 * math.pi) * (0.075 * M / 7800.0 + 0.075 * M / 1000.0)) ** (1 / 3)
+
+    def envelope(rho_top):
+        def rhs(r, y):
+            return [-G * y[1] / (2 * K * r * r), 4 * math.pi * r * r * max(y[0], 0.0)]
+        surf = lambda r, y: y[0]
+        surf.terminal = True
+        return solve_ivp(rhs, [rc, 3 * R], [rho_top, 0.15 * M], events=surf, rtol=1e-10, atol=[1e-9, 1e10],
+                         dense_output=True)
+
+    top = brentq(lambda x: envelope(math.exp(x)).y_events[0][0][1] - M, math.log(1.0), math.log(1e5), xtol=1e-12)
+    sol = envelope(math.exp(top))
+    r_surface = sol.t_events[0][0]
+    assert abs(r_surface - R) <= 1e-3 * R
+    env = b["mat"] == 3
+    r = np.sqrt(b["x"] ** 2 + b["y"] ** 2 + b["z"] ** 2).astype(np.float64) * 1e6
+    inner = env & (r < 0.95 * R)  # the outer rim holds the density floor of the builder
+    rho_ref = np.array([sol.sol(x)[0] for x in r[inner]])
+    rho = b["rho"].astype(np.float64)[inner] * 1e6  # PSU density unit: 1e24 kg / (1e6 m)^3
+    # the COM shift moves particles by a fraction of the spacing (measured: median 4.5e-5, max 6.3e-4)
+    assert np.median(np.abs(rho / rho_ref - 1)) < 2e-4
+    assert np.max(np.abs(rho / rho_ref - 1)) < 3e-3
+    for k in np.flatnonzero(env)[::97]:
+        u_cold = orc.cold_energy_psu(3, float(b["rho"][k]))
+        assert abs(b["u"][k] - u_cold) <= 1e-6 * u_cold
+        p, _, t, _ = orc.eos_eval(3, float(b["rho"][k]), float(b["u"][k]), 0.0, gamma=5 / 3)
+        assert abs(p - K * float(b["rho"][k]) ** 2) <= 1e-5 * p and t == 3.0
+    # core: uniform density at each material's reference density, inside the core radius
+    core = ~env
+    assert r[core].max() <= rc * 1.01 and r[env].min() >= rc * 0.99
