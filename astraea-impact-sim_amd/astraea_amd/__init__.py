@@ -37,6 +37,11 @@ STACK_DEPTH = 192
 ENERGY_STRIDE = 18
 EN = dict(mass=0, kinetic=1, potential=2, internal=3, mom_x=4, mom_y=5, mom_z=6, ang_x=7, ang_y=8, ang_z=9,
           radiated=10, rad_power_w=11, max_temp=12, max_pressure=13, com_x=14, com_y=15, com_z=16, sim_time=17)
+CLASS_STRIDE = 36
+CL = dict(body0_mass=0, body0_com=slice(1, 4), body0_vel=slice(4, 7), body1_mass=7, body1_com=slice(8, 11),
+          body1_vel=slice(11, 14), seed=slice(14, 20), rem_mass=20, rem_pos=slice(21, 24), rem_vel=slice(24, 27),
+          rem_radius=27, bound_planet=28, disk=29, ejecta=30, impactor_bound=31, spin_l=32, spin_i=33,
+          max_damage=34, iterations=35)
 STAGES = ["dt", "kick1", "drift", "morton", "sort", "permute_tree", "moments", "neighbours", "pass1", "gravity",
           "pass2", "kick2"]
 
@@ -139,6 +144,7 @@ _sig = {
     "astraea_synchronize": (C.c_int, [_P]),
     "astraea_get_dt": (C.c_int, [_P, C.POINTER(C.c_float)]),
     "astraea_energy": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "astraea_classify": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "astraea_counters": (C.c_int, [_P, C.POINTER(C.c_uint64), C.c_int]),
     "astraea_set_neighbour_stride": (C.c_int, [_P, C.c_uint32]),
     "astraea_download_tree": (C.c_int, [_P, C.c_uint32] + [_P] * 9),
@@ -292,6 +298,16 @@ class Context:
         out = np.zeros(len(self.counts) * ENERGY_STRIDE, dtype=np.float64)
         _check(_lib.astraea_energy(self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
         return out.reshape(len(self.counts), ENERGY_STRIDE)
+
+    def classify(self, target_radius) -> np.ndarray:
+        """Largest-remnant classification per segment (ASTRAEA_CL_* columns, see CL);
+        target_radius: one PSU radius per segment, or a scalar for all."""
+        ns = len(self.counts)
+        r = np.ascontiguousarray(np.broadcast_to(np.asarray(target_radius, np.float64), (ns,)))
+        out = np.zeros(ns * CLASS_STRIDE, dtype=np.float64)
+        dp = C.POINTER(C.c_double)
+        _check(_lib.astraea_classify(self._h, r.ctypes.data_as(dp), out.ctypes.data_as(dp)))
+        return out.reshape(ns, CLASS_STRIDE)
 
     def counters(self) -> dict:
         out = (C.c_uint64 * 7)()

@@ -64,6 +64,9 @@ struct astraea_ctx {
     DeviceCtx dc{};
     std::vector<void*> allocs;
     double* dEnergy = nullptr;
+    double* dClass = nullptr;              // ASTRAEA_CLASS_STRIDE per segment
+    double* dRem = nullptr;                // 3 x 8 per segment: remnant guess | sums | body 0
+    unsigned long long* dSeed = nullptr;   // remnant seed key per segment
     bool loaded = false, treeValid = false, listValid = false, dtRawValid = false, posNewPending = false;
     int parity = 0;  // number of cur/alt swaps mod 2
     // graph
@@ -525,6 +528,9 @@ int astraea_create(astraea_ctx** out, int device, uint32_t capacity, uint32_t ma
     A(devAlloc(c, d.sb.digitTotal, 256));
     d.sb.tilesCap = tiles;
     A(devAlloc(c, c->dEnergy, (size_t)ASTRAEA_ENERGY_STRIDE * maxSegments));
+    A(devAlloc(c, c->dClass, (size_t)ASTRAEA_CLASS_STRIDE * maxSegments));
+    A(devAlloc(c, c->dRem, (size_t)24 * maxSegments));
+    A(devAlloc(c, c->dSeed, maxSegments));
     if (r == ASTRAEA_OK) r = allocNeighbourList(c, ASTRAEA_DEFAULT_NEIGHBOUR_STRIDE);
     if (r != ASTRAEA_OK) return cleanup(r);
     d.stride64 = C;
@@ -896,6 +902,91 @@ int astraea_energy(astraea_ctx* c, double* out) {
             for (int k = ASTRAEA_EN_COM_X; k <= ASTRAEA_EN_COM_Z; ++k) o[k] /= m;
         o[ASTRAEA_EN_RADIATED] = erad[s];
         o[ASTRAEA_EN_SIM_TIME] = t[s];
+    }
+    return ASTRAEA_OK;
+}
+
+int astraea_classify(astraea_ctx* c, const double* targetRadius, double* out) {
+    TRY(requireLoaded(c));
+    if (!targetRadius || !out) return fail(ASTRAEA_E_INVALID, "null argument");
+    const uint32_t S = c->numSeg;
+    for (uint32_t s = 0; s < S; ++s)
+        if (!(targetRadius[s] >= 0.0) || !std::isfinite(targetRadius[s]))
+            return fail(ASTRAEA_E_INVALID, "target radius of segment %u is %g", s, targetRadius[s]);
+    HIP_TRY(hipSetDevice(c->device));
+    constexpr int W = ASTRAEA_CLASS_STRIDE;
+    double* dGuess = c->dRem;
+    double* dAcc = c->dRem + 8 * (size_t)S;
+    double* dBody = c->dRem + 16 * (size_t)S;
+    launchClassBodies(c->dc, c->stream, c->dClass, c->dSeed);
+    TRY(checkLaunch(c));
+    std::vector<double> first((size_t)W * S);
+    HIP_TRY(hipMemcpyAsync(first.data(), c->dClass, sizeof(double) * W * S, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+
+    // Remnant guess (simulation.cpp:1115-1123): the seed's state, mass max(m0, M/2), radius
+    // max(R_target, 0.25).  rem[0] < 0 marks a segment that stopped iterating.
+    std::vector<double> rem(8 * (size_t)S), guess(8 * (size_t)S), acc(8 * (size_t)S), body(8 * (size_t)S, 0.0);
+    std::vector<int> iters(S, 0);
+    std::vector<bool> active(S);
+    for (uint32_t s = 0; s < S; ++s) {
+        double* o = first.data() + (size_t)s * W;
+        const double m0 = o[ASTRAEA_CL_BODY0_MASS], m1 = o[ASTRAEA_CL_BODY1_MASS];
+        for (int b = 0; b < 2; ++b) {
+            const double m = b ? m1 : m0;
+            double* v = o + (b ? ASTRAEA_CL_BODY1_COM : ASTRAEA_CL_BODY0_COM);
+            for (int k = 0; k < 6; ++k) v[k] = m > 0.0 ? v[k] / m : 0.0;
+        }
+        double* r = rem.data() + 8 * (size_t)s;
+        r[0] = std::max(m0, 0.5 * (m0 + m1));
+        for (int k = 0; k < 6; ++k) r[1 + k] = o[ASTRAEA_CL_SEED_X + k];
+        r[7] = std::max(targetRadius[s], 0.25);
+        active[s] = c->counts[s] > 0;
+        double* b0 = body.data() + 8 * (size_t)s;
+        for (int k = 0; k < 6; ++k) b0[k] = o[ASTRAEA_CL_BODY0_COM + k];
+        b0[6] = targetRadius[s];
+    }
+    // Five iterations (simulation.cpp:1125-1178), all segments per launch.
+    for (int it = 0; it < 5; ++it) {
+        bool any = false;
+        for (uint32_t s = 0; s < S; ++s) {
+            for (int k = 0; k < 8; ++k) guess[8 * (size_t)s + k] = rem[8 * (size_t)s + k];
+            if (!active[s]) guess[8 * (size_t)s] = -1.0;
+            any = any || active[s];
+        }
+        if (!any) break;
+        HIP_TRY(hipMemcpyAsync(dGuess, guess.data(), sizeof(double) * 8 * S, hipMemcpyHostToDevice, c->stream));
+        launchRemnant(c->dc, c->stream, dGuess, dAcc);
+        TRY(checkLaunch(c));
+        HIP_TRY(hipMemcpyAsync(acc.data(), dAcc, sizeof(double) * 8 * S, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        for (uint32_t s = 0; s < S; ++s) {
+            if (!active[s]) continue;
+            const double* a = acc.data() + 8 * (size_t)s;
+            double* r = rem.data() + 8 * (size_t)s;
+            if (!(a[0] > 0.0)) {
+                active[s] = false;
+                continue;
+            }
+            r[0] = a[0];
+            for (int k = 1; k < 7; ++k) r[k] = a[k] / a[0];
+            r[7] = std::max(0.5 * targetRadius[s], std::sqrt(5.0 / 3.0) * std::sqrt(a[7] / a[0]));
+            iters[s] += 1;
+        }
+    }
+    // Final split (simulation.cpp:1180-1227) and the spin of body 0 (1094-1112).
+    HIP_TRY(hipMemcpyAsync(dGuess, rem.data(), sizeof(double) * 8 * S, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(dBody, body.data(), sizeof(double) * 8 * S, hipMemcpyHostToDevice, c->stream));
+    launchClassify(c->dc, c->stream, dGuess, dBody, c->dClass);
+    TRY(checkLaunch(c));
+    HIP_TRY(hipMemcpyAsync(out, c->dClass, sizeof(double) * W * S, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (uint32_t s = 0; s < S; ++s) {
+        double* o = out + (size_t)s * W;
+        const double* f = first.data() + (size_t)s * W;
+        for (int k = 0; k < ASTRAEA_CL_REM_MASS; ++k) o[k] = f[k];
+        for (int k = 0; k < 8; ++k) o[ASTRAEA_CL_REM_MASS + k] = rem[8 * (size_t)s + k];
+        o[ASTRAEA_CL_ITERATIONS] = iters[s];
     }
     return ASTRAEA_OK;
 }

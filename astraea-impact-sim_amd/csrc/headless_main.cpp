@@ -13,49 +13,12 @@
 #include <string>
 #include <vector>
 
+#include "../../include/astraea/presets.hpp"
 #include "../../include/astraea/simulation.hpp"
 
 using namespace astraea;
 
 namespace {
-
-BodyInitSpec body(PlanetArchetype a, double massKg, double surfaceK, double coreK, double spin) {
-    return BodyInitSpec{a, massKg, surfaceK, coreK, spin, 80};
-}
-
-// Scenario presets of engine/presets.cpp:128-210 with the archetype defaults of presets.cpp:9-88.
-CollisionScenarioConfig preset(const std::string& name, uint32_t particles) {
-    CollisionScenarioConfig c;
-    c.presetName = name;
-    c.totalParticles = particles;
-    c.tracerGridRes = 16;
-    c.randomSeed = 42;
-    const double ME = units::M_EARTH_SI;
-    if (name == "EarthTheiaMoonForming") {
-        c.target = body(PlanetArchetype::LivingTerra, 1.0 * ME, 288.0, 5500.0, 7.292115e-5);
-        c.impactor = body(PlanetArchetype::ProtoTheia, 0.13 * ME, 240.0, 2200.0, 7.09e-5);
-        c.impactVelocityRatioVesc = 1.05; c.impactParamB = 0.72; c.initialSepRadii = 3.0;
-    } else if (name == "MercuryMantleStrip") {
-        c.target = body(PlanetArchetype::ProtoTheia, 0.45 * ME, 440.0, 2200.0, 7.09e-5);
-        c.impactor = body(PlanetArchetype::IronCore, 0.22 * ME, 440.0, 3200.0, 1.24e-6);
-        c.impactVelocityRatioVesc = 2.80; c.impactParamB = 0.35; c.initialSepRadii = 3.2;
-    } else if (name == "OceanWorldSteamPlume") {
-        c.target = body(PlanetArchetype::OceanIce, 0.75 * ME, 275.0, 3800.0, 5.0e-5);
-        c.impactor = body(PlanetArchetype::ProtoTheia, 0.15 * ME, 240.0, 2200.0, 7.09e-5);
-        c.impactVelocityRatioVesc = 1.35; c.impactParamB = 0.25; c.initialSepRadii = 3.2;
-    } else if (name == "GasGiantCoreImpact") {
-        c.target = body(PlanetArchetype::GasGiant, 20.0 * ME, 165.0, 18000.0, 1.76e-4);
-        c.impactor = body(PlanetArchetype::IronCore, 1.0 * ME, 440.0, 3200.0, 1.24e-6);
-        c.impactVelocityRatioVesc = 1.40; c.impactParamB = 0.15; c.initialSepRadii = 2.8;
-    } else if (name == "LivingGaiaAsteroidCataclysm") {
-        c.target = body(PlanetArchetype::LivingTerra, 1.0 * ME, 288.0, 5500.0, 7.292115e-5);
-        c.impactor = body(PlanetArchetype::ProtoTheia, 0.025 * ME, 240.0, 1800.0, 0.0);
-        c.impactVelocityRatioVesc = 1.65; c.impactParamB = 0.30; c.initialSepRadii = 3.2;
-    } else {
-        throw std::invalid_argument("Unknown preset name: " + name);
-    }
-    return c;
-}
 
 // Radius of a body of uniform material densities (the builder's bodies), metres.
 double bodyRadius(const BodyInitSpec& b) {
@@ -129,9 +92,8 @@ int main(int argc, char** argv) {
             const std::string a = argv[i];
             if (a == "--help" || a == "-h") { usage(argv[0]); return 0; }
             if (a == "--list-presets") {
-                for (const char* p : {"EarthTheiaMoonForming", "MercuryMantleStrip", "OceanWorldSteamPlume",
-                                      "GasGiantCoreImpact", "LivingGaiaAsteroidCataclysm"})
-                    std::cout << p << '\n';
+                for (int k = 0; k <= (int)ScenarioPresetId::LivingGaiaAsteroidCataclysm; ++k)
+                    std::cout << presetToString((ScenarioPresetId)k) << '\n';
                 return 0;
             }
             if (a == "--preset") presetName = need(i, "--preset");
@@ -156,7 +118,7 @@ int main(int argc, char** argv) {
             else if (a == "--verify-conservation") verify = true;
             else throw std::invalid_argument("Unknown option: " + a);
         }
-        CollisionScenarioConfig cfg = preset(presetName, particles);
+        CollisionScenarioConfig cfg = makePresetConfig(parsePresetName(presetName), particles);
         if (relaxOverride) cfg.target.relaxSteps = cfg.impactor.relaxSteps = relaxSteps;
         const double mTot = cfg.target.massKg + std::max(0.0, cfg.impactor.massKg);
         const double rScale = std::max(bodyRadius(cfg.target) + (cfg.impactor.massKg > 0 ? bodyRadius(cfg.impactor) : 0.0), 1.0);

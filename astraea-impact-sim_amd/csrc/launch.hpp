@@ -72,4 +72,10 @@ void launchInitIntegrator(const DeviceCtx& c, hipStream_t s); // radPrev = radSu
 void launchClear(const DeviceCtx& c, hipStream_t s);
 void launchEnergy(const DeviceCtx& c, hipStream_t s, double* dOut);
 
+// telemetry.hip: largest-remnant classification.  out holds ASTRAEA_CLASS_STRIDE doubles per
+// segment, seed one key per segment; rem, acc and body hold 8 doubles per segment.
+void launchClassBodies(const DeviceCtx& c, hipStream_t s, double* out, unsigned long long* seed);
+void launchRemnant(const DeviceCtx& c, hipStream_t s, const double* rem, double* acc);
+void launchClassify(const DeviceCtx& c, hipStream_t s, const double* rem, const double* body, double* out);
+
 }  // namespace astraea_hip

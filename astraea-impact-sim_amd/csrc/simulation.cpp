@@ -9,9 +9,13 @@
 // reference's exceptions: std::invalid_argument for bad configuration, std::runtime_error for
 // device failures and call order.
 //
+// The largest-remnant classification (bound / disk / ejecta mass fractions, living-body centre
+// and spin, impactor centre) comes from the device reductions of astraea_classify, and the
+// collision outcome from the same rules as simulation.cpp:1229-1281.
+//
 // Bodies come from the lattice builder (astraea_build_body: uniform density per layer, the
 // archetype's layer mass fractions, hydrostatic u) instead of planet_builder.cpp, relaxation is
-// not run, and render frames, tracers and the bound-mass classifier are not produced.
+// not run, and render frames and tracers are not produced.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,6 +29,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/astraea/presets.hpp"
 #include "../../include/astraea/simulation.hpp"
 #include "../../include/astraea_hip.h"
 
@@ -34,27 +39,6 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 
-struct Layer {
-    MaterialId mat;
-    double frac;
-};
-
-// Archetype compositions of engine/presets.cpp:9-88 (innermost layer first).
-std::vector<Layer> archetypeLayers(PlanetArchetype a) {
-    switch (a) {
-        case PlanetArchetype::IronCore: return {{MaterialId::Iron, 0.680}, {MaterialId::Granite, 0.320}};
-        case PlanetArchetype::OceanIce:
-            return {{MaterialId::Iron, 0.150}, {MaterialId::Granite, 0.450}, {MaterialId::WaterIce, 0.400}};
-        case PlanetArchetype::LavaIo: return {{MaterialId::Iron, 0.250}, {MaterialId::Granite, 0.750}};
-        case PlanetArchetype::GasGiant:
-            return {{MaterialId::Iron, 0.050}, {MaterialId::Granite, 0.100}, {MaterialId::HydrogenHelium, 0.850}};
-        case PlanetArchetype::ProtoTheia:
-            return {{MaterialId::Iron, 0.220}, {MaterialId::Granite, 0.580}, {MaterialId::Regolith, 0.200}};
-        case PlanetArchetype::LivingTerra:
-        default:
-            return {{MaterialId::Iron, 0.320}, {MaterialId::Granite, 0.675}, {MaterialId::WaterIce, 0.005}};
-    }
-}
 
 double rho0Of(MaterialId m) {
     switch (m) {
@@ -128,20 +112,22 @@ struct Body {
 };
 
 Body buildBody(const BodyInitSpec& spec, uint32_t n, uint64_t seed) {
-    const std::vector<Layer> layers = archetypeLayers(spec.archetype);
+    const ArchetypeComposition& comp = getArchetypeComposition(spec.archetype);
     astraea_body_spec bs;
     astraea_default_body_spec(&bs);
     double vPerKg = 0.0;
     bs.num_layers = 0;
-    for (const Layer& l : layers) {
-        if (l.frac <= 0.0) continue;
-        if (l.mat == MaterialId::HydrogenHelium)
-            throw std::invalid_argument("loadScenario: H/He envelopes need a polytrope profile, which the lattice builder does not provide");
-        bs.layer_material[bs.num_layers] = (uint8_t)l.mat;
-        bs.layer_mass_frac[bs.num_layers] = l.frac;
+    for (uint32_t k = 0; k < comp.layerCount; ++k) {
+        const PlanetLayerDef& l = comp.layers[k];
+        if (l.massFraction <= 0.0) continue;
+        if (l.material == MaterialId::HydrogenHelium)
+            throw std::invalid_argument("loadScenario: a body of a given mass with an H/He envelope is not built here "
+                                        "(the polytrope builder fixes the radius, not the mass)");
+        bs.layer_material[bs.num_layers] = (uint8_t)l.material;
+        bs.layer_mass_frac[bs.num_layers] = l.massFraction;
         bs.layer_density_kgm3[bs.num_layers] = 0.0;
         ++bs.num_layers;
-        vPerKg += l.frac / rho0Of(l.mat);
+        vPerKg += l.massFraction / rho0Of(l.material);
     }
     const double volume = std::max(spec.massKg, 1.0) * vPerKg;
     bs.radius_m = std::cbrt(3.0 * volume / (4.0 * kPi));
@@ -197,6 +183,8 @@ struct Simulation::Impl {
     double lastGpuStepMs = 0.0;
     double qRatioQrd = 0.0;
     double targetRadiusPsu = 1.0;
+    double impactorRadiusPsu = 0.0;
+    bool hasImpacted = false;
     uint32_t count = 0;
     bool livingBodyPresent = false;
     TelemetrySnapshot lastSnapshot{};
@@ -256,8 +244,64 @@ struct Simulation::Impl {
         s.maxTempK = e[ASTRAEA_EN_MAX_TEMP];
         s.maxPressureGPa = e[ASTRAEA_EN_MAX_PRESSURE] * 1000.0;
         s.livingBodyRadiusKm = livingBodyPresent ? targetRadiusPsu * 1000.0 : 0.0;
+        classify(s, e);
         lastSnapshot = s;
         return s;
+    }
+
+    // Bound-mass classification and collision outcome (simulation.cpp:1043-1060, 1087-1118,
+    // 1229-1281) from the device reductions.
+    void classify(TelemetrySnapshot& s, const double* e) {
+        double c[ASTRAEA_CLASS_STRIDE];
+        check(astraea_classify(ctx, &targetRadiusPsu, c), "astraea_classify");
+        const double m0 = c[ASTRAEA_CL_BODY0_MASS], m1 = c[ASTRAEA_CL_BODY1_MASS];
+        const double total = e[ASTRAEA_EN_MASS];
+        const double* sys = e + ASTRAEA_EN_COM_X;
+        const double* com0 = c + ASTRAEA_CL_BODY0_COM;
+        const double* com1 = c + ASTRAEA_CL_BODY1_COM;
+        if (livingBodyPresent) {
+            for (int a = 0; a < 3; ++a) s.livingBodyComKm[a] = ((m0 > 0.0 ? com0[a] : sys[a]) - sys[a]) * 1000.0;
+            const double iyy = c[ASTRAEA_CL_SPIN_I];
+            s.livingBodySpinRadS = iyy > 0.0 ? c[ASTRAEA_CL_SPIN_L] / iyy / units::T_UNIT_S : config.target.spinOmegaRadS;
+        }
+        if (m1 > 0.0)
+            for (int a = 0; a < 3; ++a) s.impactorComKm[a] = (com1[a] - sys[a]) * 1000.0;
+        if (total > 0.0) {
+            s.boundMassFrac = c[ASTRAEA_CL_BOUND_PLANET] / total;
+            s.diskMassFrac = c[ASTRAEA_CL_DISK] / total;
+            s.ejectaMassFrac = c[ASTRAEA_CL_EJECTA] / total;
+        }
+        if (m1 <= 0.0) {
+            s.outcome = CollisionOutcome::Merging;
+            return;
+        }
+        double r[3], v[3];
+        for (int a = 0; a < 3; ++a) {
+            r[a] = com1[a] - com0[a];
+            v[a] = c[ASTRAEA_CL_BODY1_VEL + a] - c[ASTRAEA_CL_BODY0_VEL + a];
+        }
+        const double dSep = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        const double rDotV = r[0] * v[0] + r[1] * v[1] + r[2] * v[2];
+        const double contact = targetRadiusPsu + impactorRadiusPsu;
+        if (dSep <= 1.05 * contact || c[ASTRAEA_CL_MAX_DAMAGE] > 0.25 || rDotV >= 0.0) hasImpacted = true;
+        if (!hasImpacted && dSep > 1.05 * contact && rDotV < 0.0) {
+            s.outcome = CollisionOutcome::Approaching;
+            return;
+        }
+        const double impactorBoundFrac = c[ASTRAEA_CL_IMPACTOR_BOUND] / m1;
+        const double bAbs = std::abs(config.impactParamB);
+        const double twoBodyE = 0.5 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) -
+                                units::G_PSU * (m0 + m1) / std::max(dSep, 1.0e-3);
+        if (s.boundMassFrac < 0.50 || (qRatioQrd >= 1.0 && (s.ejectaMassFrac >= 0.15 || s.boundMassFrac < 0.72)))
+            s.outcome = CollisionOutcome::CatastrophicDisruption;
+        else if (rDotV > 0.0 && dSep > 1.05 * contact && (impactorBoundFrac < 0.50 || twoBodyE > 0.0))
+            s.outcome = CollisionOutcome::HitAndRun;
+        else if (bAbs >= 0.50 && config.impactVelocityRatioVesc > 1.25 && impactorBoundFrac < 0.45)
+            s.outcome = CollisionOutcome::HitAndRun;
+        else if (bAbs >= 0.45)
+            s.outcome = CollisionOutcome::GrazeAndMerge;
+        else
+            s.outcome = CollisionOutcome::Merging;
     }
 };
 
@@ -277,6 +321,8 @@ void Simulation::loadScenario(const CollisionScenarioConfig& config) {
     I.initialTotalEnergyJ = 0.0;
     I.lastGpuStepMs = 0.0;
     I.history.clear();
+    I.hasImpacted = false;
+    I.impactorRadiusPsu = 0.0;
 
     // particle split proportional to mass (simulation.cpp:1313-1324)
     const bool hasImpactor = config.impactor.massKg > 0.0 && config.totalParticles >= 2u;
@@ -305,6 +351,7 @@ void Simulation::loadScenario(const CollisionScenarioConfig& config) {
     I.qRatioQrd = 0.0;
     if (hasImpactor) {
         Body b = buildBody(config.impactor, nI, config.randomSeed + 1ULL);
+        I.impactorRadiusPsu = b.radiusPsu;
         const double omI = config.impactor.spinOmegaRadS * units::T_UNIT_S;
         for (uint32_t i = 0; i < nI; ++i) {
             x.push_back(b.x[i]); y.push_back(b.y[i]); z.push_back(b.z[i]); m.push_back(b.m[i]);

@@ -195,6 +195,39 @@ int astraea_get_dt(astraea_ctx* ctx, float* out_dt);
 #define ASTRAEA_ENERGY_STRIDE 18
 int astraea_energy(astraea_ctx* ctx, double* out);
 
+/* Largest-remnant classification of the telemetry, per segment, as FP64 reductions on
+ * the device (replaces the host particle loops of Simulation::computeTelemetryFromDevice,
+ * simulation.cpp:1000-1013, 1036-1039, 1094-1112 and 1115-1227): the sums of body 0 and
+ * of the other bodies; the remnant seed, the particle of lowest finite phi (lowest
+ * ORIGINAL_ID on ties); up to five iterations of the particles bound to the remnant
+ * guess, each stopping the segment when none is bound; and the final split into bound
+ * planet / disk / ejecta, with the bound mass of bodies other than 0, the largest damage
+ * and the Y spin of body 0 within 1.5 target radii of its centre.  `target_radius_psu`
+ * holds one radius per segment.  `out` receives ASTRAEA_CLASS_STRIDE doubles per segment
+ * in the order of the ASTRAEA_CL_* indices (PSU units; centres and velocities already
+ * divided by their mass, zero for a body without mass).  Synchronous. */
+#define ASTRAEA_CL_BODY0_MASS 0
+#define ASTRAEA_CL_BODY0_COM 1       /* x, y, z */
+#define ASTRAEA_CL_BODY0_VEL 4       /* vx, vy, vz */
+#define ASTRAEA_CL_BODY1_MASS 7      /* every body other than 0 */
+#define ASTRAEA_CL_BODY1_COM 8
+#define ASTRAEA_CL_BODY1_VEL 11
+#define ASTRAEA_CL_SEED_X 14         /* x, y, z, vx, vy, vz of the seed particle */
+#define ASTRAEA_CL_REM_MASS 20       /* remnant after the iterations */
+#define ASTRAEA_CL_REM_POS 21
+#define ASTRAEA_CL_REM_VEL 24
+#define ASTRAEA_CL_REM_RADIUS 27
+#define ASTRAEA_CL_BOUND_PLANET 28   /* masses of the final split */
+#define ASTRAEA_CL_DISK 29
+#define ASTRAEA_CL_EJECTA 30
+#define ASTRAEA_CL_IMPACTOR_BOUND 31 /* bound mass of bodies other than 0 */
+#define ASTRAEA_CL_SPIN_L 32         /* sum m (dz dvx - dx dvz) about body 0's centre */
+#define ASTRAEA_CL_SPIN_I 33         /* sum m (dx^2 + dz^2) */
+#define ASTRAEA_CL_MAX_DAMAGE 34
+#define ASTRAEA_CL_ITERATIONS 35     /* remnant iterations that found bound mass */
+#define ASTRAEA_CLASS_STRIDE 36
+int astraea_classify(astraea_ctx* ctx, const double* target_radius_psu, double* out);
+
 /* Counters: `out[0]` = sum over active particles of the particle-node and
  * particle-particle interactions of the gravity evaluations since the last call,
  * `out[1]` = traversal-stack overflows of those (forced acceptances,

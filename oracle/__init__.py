@@ -72,6 +72,7 @@ _sig = {
     "or_initial_forces": (C.c_int, [_P]),
     "or_step": (C.c_int, [_P, C.c_double, C.c_uint32]),
     "or_energy": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "or_classify_arrays": (None, [C.c_uint32] + [_P] * 10 + [C.c_double, _P]),
     "or_counters": (C.c_int, [_P, C.POINTER(C.c_uint64), C.c_int]),
     "or_neighbour_stats": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
     "or_set_dt": (C.c_int, [_P, C.POINTER(C.c_double)]),
@@ -251,6 +252,24 @@ def eos_eval(mat, rho, u, damage=0.0, gamma=1.4):
     ph = C.c_uint8()
     _lib.or_eos_eval(mat, rho, u, damage, gamma, C.byref(p), C.byref(cs), C.byref(t), C.byref(ph))
     return p.value, cs.value, t.value, ph.value
+
+
+CLASS_STRIDE = 36
+
+
+def classify_arrays(x, y, z, vx, vy, vz, m, phi, body, damage, target_radius: float) -> np.ndarray:
+    """Largest-remnant classification of one body pair's arrays in input order
+    (engine/simulation.cpp:1000-1227); returns CLASS_STRIDE doubles in ASTRAEA_CL_* order."""
+    n = len(m)
+    d = [np.ascontiguousarray(a, np.float64) for a in (x, y, z, vx, vy, vz, m)]
+    ph = np.ascontiguousarray(phi, np.float32)
+    b = np.ascontiguousarray(body, np.uint8)
+    dm = np.ascontiguousarray(damage, np.float32)
+    assert all(len(a) == n for a in d + [ph, b, dm])
+    out = np.zeros(CLASS_STRIDE, np.float64)
+    _lib.or_classify_arrays(n, *[a.ctypes.data for a in d], ph.ctypes.data, b.ctypes.data, dm.ctypes.data,
+                            float(target_radius), out.ctypes.data)
+    return out
 
 
 def cold_energy_psu(mat, rho_psu):

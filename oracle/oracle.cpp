@@ -20,6 +20,11 @@
 //                                form (tests/cpp/test_gravity_sph.cpp:646-670); Wendland normalisation.
 //   kick / dt                    closed form per particle.
 //   step                         conservation gates of engine/headless_main.cpp:263-281.
+//   classify_arrays              hand-solvable configurations: a test particle on circular,
+//                                parabolic-plus and eccentric orbits about a point remnant (disk /
+//                                ejecta / planet by the vis-viva and periapsis closed forms), rigid
+//                                rotation (spin = omega), the seed at the phi minimum, first-index
+//                                ties, and the stop rule when nothing is bound.
 
 #include "oracle.h"
 
@@ -1356,6 +1361,104 @@ double or_material_param(int mat, int k) {
     const double v[19] = {p.rho0, p.a, p.b, p.A, p.B, p.E0, p.Eiv, p.Ecv, p.alpha, p.beta,
                           p.cv, p.tMelt, p.tRef, p.mu, p.Y0, p.epsMin, p.kH2, p.gamma, p.bulk};
     return (k >= 0 && k < 19) ? v[k] : 0.0;
+}
+
+// Largest-remnant classification of the telemetry on one body pair's arrays, in input order:
+// Simulation::computeTelemetryFromDevice, engine/simulation.cpp:1000-1013 (body sums),
+// 1036-1039 (seed = first particle of lowest finite phi), 1094-1112 (spin of body 0 within
+// 1.5 target radii), 1115-1178 (five remnant iterations), 1180-1227 (bound planet / disk /
+// ejecta).  out: 36 doubles in the order of the ASTRAEA_CL_* indices.
+void or_classify_arrays(uint32_t n, const double* x, const double* y, const double* z, const double* vx,
+                        const double* vy, const double* vz, const double* m, const float* phi, const uint8_t* body,
+                        const float* damage, double targetRadius, double* out) {
+    std::fill(out, out + ASTRAEA_CLASS_STRIDE, 0.0);
+    double m0 = 0, m1 = 0, s0[6] = {}, s1[6] = {};
+    uint32_t seed = 0;
+    double minPhi = INFINITY;
+    for (uint32_t i = 0; i < n; ++i) {
+        const double v[6] = {x[i], y[i], z[i], vx[i], vy[i], vz[i]};
+        double* acc = body[i] == 0 ? s0 : s1;
+        (body[i] == 0 ? m0 : m1) += m[i];
+        for (int k = 0; k < 6; ++k) acc[k] += m[i] * v[k];
+        const double ph = (double)phi[i];
+        if (std::isfinite(ph) && ph < minPhi) {
+            minPhi = ph;
+            seed = i;
+        }
+    }
+    for (int k = 0; k < 6; ++k) {
+        if (m0 > 0) s0[k] /= m0;
+        if (m1 > 0) s1[k] /= m1;
+    }
+    out[ASTRAEA_CL_BODY0_MASS] = m0;
+    out[ASTRAEA_CL_BODY1_MASS] = m1;
+    for (int k = 0; k < 6; ++k) {
+        out[ASTRAEA_CL_BODY0_COM + k] = s0[k];
+        out[ASTRAEA_CL_BODY1_COM + k] = s1[k];
+    }
+    if (n == 0) return;
+    double rem[7] = {x[seed], y[seed], z[seed], vx[seed], vy[seed], vz[seed], 0.0};
+    for (int k = 0; k < 6; ++k) out[ASTRAEA_CL_SEED_X + k] = rem[k];
+    double remMass = std::max(m0, 0.5 * (m0 + m1));
+    double remRadius = std::max(targetRadius, 0.25);
+    int iters = 0;
+    for (int it = 0; it < 5; ++it) {
+        double nm = 0, nx[6] = {}, mr2 = 0;
+        const double rSoft = std::max(0.5 * remRadius, 0.1);
+        for (uint32_t i = 0; i < n; ++i) {
+            const double dx = x[i] - rem[0], dy = y[i] - rem[1], dz = z[i] - rem[2];
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const double dvx = vx[i] - rem[3], dvy = vy[i] - rem[4], dvz = vz[i] - rem[5];
+            const double e = 0.5 * (dvx * dvx + dvy * dvy + dvz * dvz) - kG * remMass / std::max(std::sqrt(r2), rSoft);
+            if (e < 0.0) {
+                const double v[6] = {x[i], y[i], z[i], vx[i], vy[i], vz[i]};
+                nm += m[i];
+                for (int k = 0; k < 6; ++k) nx[k] += m[i] * v[k];
+                mr2 += m[i] * r2;
+            }
+        }
+        if (!(nm > 0.0)) break;
+        remMass = nm;
+        for (int k = 0; k < 6; ++k) rem[k] = nx[k] / nm;
+        remRadius = std::max(0.5 * targetRadius, std::sqrt(5.0 / 3.0) * std::sqrt(mr2 / nm));
+        ++iters;
+    }
+    out[ASTRAEA_CL_REM_MASS] = remMass;
+    for (int k = 0; k < 3; ++k) {
+        out[ASTRAEA_CL_REM_POS + k] = rem[k];
+        out[ASTRAEA_CL_REM_VEL + k] = rem[3 + k];
+    }
+    out[ASTRAEA_CL_REM_RADIUS] = remRadius;
+    out[ASTRAEA_CL_ITERATIONS] = iters;
+    const double rSoft = std::max(0.5 * remRadius, 0.1), mu = kG * remMass;
+    const double lim2 = (1.5 * targetRadius) * (1.5 * targetRadius);
+    for (uint32_t i = 0; i < n; ++i) {
+        out[ASTRAEA_CL_MAX_DAMAGE] = std::max(out[ASTRAEA_CL_MAX_DAMAGE], (double)damage[i]);
+        const double dx = x[i] - rem[0], dy = y[i] - rem[1], dz = z[i] - rem[2];
+        const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+        const double dvx = vx[i] - rem[3], dvy = vy[i] - rem[4], dvz = vz[i] - rem[5];
+        const double e = 0.5 * (dvx * dvx + dvy * dvy + dvz * dvz) - mu / std::max(r, rSoft);
+        if (e >= 0.0) {
+            out[ASTRAEA_CL_EJECTA] += m[i];
+        } else {
+            if (body[i] != 0) out[ASTRAEA_CL_IMPACTOR_BOUND] += m[i];
+            bool disk = false;
+            if (r > remRadius && mu > 0.0) {
+                const double jx = dy * dvz - dz * dvy, jy = dz * dvx - dx * dvz, jz = dx * dvy - dy * dvx;
+                const double j2 = jx * jx + jy * jy + jz * jz;
+                const double ecc = std::sqrt(std::max(0.0, 1.0 + 2.0 * e * j2 / (mu * mu)));
+                disk = j2 / (mu * (1.0 + ecc)) > remRadius;
+            }
+            out[disk ? ASTRAEA_CL_DISK : ASTRAEA_CL_BOUND_PLANET] += m[i];
+        }
+        if (body[i] == 0) {
+            const double ex = x[i] - s0[0], ey = y[i] - s0[1], ez = z[i] - s0[2];
+            if (ex * ex + ey * ey + ez * ez <= lim2) {
+                out[ASTRAEA_CL_SPIN_L] += m[i] * (ez * (vx[i] - s0[3]) - ex * (vz[i] - s0[5]));
+                out[ASTRAEA_CL_SPIN_I] += m[i] * (ex * ex + ez * ez);
+            }
+        }
+    }
 }
 
 uint64_t or_morton_key(float x, float y, float z, const float* b) {

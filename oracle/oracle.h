@@ -79,6 +79,12 @@ double or_material_param(int mat, int k);
 /* Morton key of one point for given FP32 bounds {minx,miny,minz,maxx,maxy,maxz} */
 uint64_t or_morton_key(float x, float y, float z, const float* bounds6);
 
+/* Largest-remnant classification of one body pair's arrays (engine/simulation.cpp:1000-1227),
+ * in input order; `out` receives ASTRAEA_CLASS_STRIDE doubles (ASTRAEA_CL_* order). */
+void or_classify_arrays(uint32_t n, const double* x, const double* y, const double* z, const double* vx,
+                        const double* vy, const double* vz, const double* m, const float* phi, const uint8_t* body,
+                        const float* damage, double target_radius, double* out);
+
 #ifdef __cplusplus
 }
 #endif
