@@ -109,6 +109,20 @@ class BodySpec(C.Structure):
     ]
 
 
+class PlanetSpec(C.Structure):
+    _fields_ = [
+        ("mass_kg", C.c_double), ("num_layers", C.c_uint32), ("layer_material", C.c_uint8 * MAX_LAYERS),
+        ("layer_mass_frac", C.c_double * MAX_LAYERS), ("surface_temp_k", C.c_double), ("core_temp_k", C.c_double),
+        ("surface_pressure_pa", C.c_double), ("gamma_hhe", C.c_double), ("h_factor", C.c_double),
+        ("seed", C.c_uint64),
+    ]
+
+
+class PlanetInfo(C.Structure):
+    _fields_ = [("mass_kg", C.c_double), ("radius_m", C.c_double), ("central_pressure_pa", C.c_double),
+                ("profile_samples", C.c_uint32)]
+
+
 class BodyOut(C.Structure):
     _fields_ = [(n, C.POINTER(C.c_float)) for n in ("x", "y", "z", "m", "h", "rho", "u", "eps_fail")] + [
         ("mat", C.POINTER(C.c_uint8))
@@ -145,12 +159,18 @@ _sig = {
     "astraea_get_dt": (C.c_int, [_P, C.POINTER(C.c_float)]),
     "astraea_energy": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "astraea_classify": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "astraea_relax": (C.c_int, [_P, C.c_uint32, _P, _P, _P]),
     "astraea_counters": (C.c_int, [_P, C.POINTER(C.c_uint64), C.c_int]),
     "astraea_set_neighbour_stride": (C.c_int, [_P, C.c_uint32]),
     "astraea_download_tree": (C.c_int, [_P, C.c_uint32] + [_P] * 9),
     "astraea_enable_stage_timing": (C.c_int, [_P, C.c_int]),
     "astraea_stage_times": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "astraea_default_body_spec": (None, [C.POINTER(BodySpec)]),
+    "astraea_default_planet_spec": (None, [C.POINTER(PlanetSpec)]),
+    "astraea_build_planet": (C.c_int, [C.POINTER(PlanetSpec), C.c_uint32, C.POINTER(BodyOut), _P,
+                                       C.POINTER(PlanetInfo)]),
+    "astraea_planet_profile": (C.c_int, [C.POINTER(PlanetSpec), C.c_uint32] + [_P] * 6 + [C.POINTER(PlanetInfo)]),
+    "astraea_body_radius_for_mass": (C.c_int, [C.POINTER(BodySpec), C.c_double, C.POINTER(C.c_double)]),
     "astraea_build_body": (C.c_int, [C.POINTER(BodySpec), C.c_uint32, C.POINTER(BodyOut), C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]),
     "astraea_place_two_bodies": (C.c_int, [C.c_uint32, C.c_uint32] + [_P] * 7 +
@@ -309,6 +329,21 @@ class Context:
         _check(_lib.astraea_classify(self._h, r.ctypes.data_as(dp), out.ctypes.data_as(dp)))
         return out.reshape(ns, CLASS_STRIDE)
 
+    def relax(self, steps: int, nominal_radius, anchor_temp=None) -> np.ndarray:
+        """Damped relaxation, one body per segment; returns v_rms / v_esc per segment.
+        anchor_temp: K per particle by original id, or None for the EOS temperatures."""
+        ns = len(self.counts)
+        r = np.ascontiguousarray(np.broadcast_to(np.asarray(nominal_radius, np.float64), (ns,)))
+        out = np.zeros(ns, np.float64)
+        a = None
+        if anchor_temp is not None:
+            a = np.ascontiguousarray(anchor_temp, np.float32)
+            if len(a) != int(sum(self.counts)):
+                raise ValueError("anchor_temp needs one value per particle")
+        _check(_lib.astraea_relax(self._h, steps, None if a is None else a.ctypes.data, r.ctypes.data,
+                                  out.ctypes.data))
+        return out
+
     def counters(self) -> dict:
         out = (C.c_uint64 * 7)()
         _check(_lib.astraea_counters(self._h, out, 7))
@@ -376,6 +411,62 @@ def build_body(n: int, radius_m: float, layers, h_over_spacing=1.3, jitter=0.1, 
     arrs["mass_kg"] = mass.value
     arrs["spacing_m"] = spacing.value
     return arrs
+
+
+def planet_spec(mass_kg: float, layers, surface_temp_k: float, core_temp_k: float, surface_pressure_pa: float = 0.0,
+                gamma_hhe: float = 1.4, h_factor: float = 1.2, seed: int = 42) -> PlanetSpec:
+    """`layers`: [(material, mass_fraction), ...] innermost first."""
+    s = PlanetSpec()
+    _lib.astraea_default_planet_spec(C.byref(s))
+    s.mass_kg = mass_kg
+    s.num_layers = len(layers)
+    for k, (mat, frac) in enumerate(layers):
+        s.layer_material[k] = mat
+        s.layer_mass_frac[k] = frac
+    s.surface_temp_k, s.core_temp_k, s.surface_pressure_pa = surface_temp_k, core_temp_k, surface_pressure_pa
+    s.gamma_hhe, s.h_factor, s.seed = gamma_hhe, h_factor, seed
+    return s
+
+
+def planet_profile(spec: PlanetSpec) -> dict:
+    """Solved hydrostatic profile of `spec` (astraea_planet_profile)."""
+    info = PlanetInfo()
+    _check(_lib.astraea_planet_profile(C.byref(spec), 0, None, None, None, None, None, None, C.byref(info)))
+    n = info.profile_samples
+    out = {k: np.zeros(n) for k in ("r", "rho", "p", "m", "t")}
+    out["mat"] = np.zeros(n, np.uint8)
+    _check(_lib.astraea_planet_profile(C.byref(spec), n, *[out[k].ctypes.data for k in ("r", "rho", "p", "m", "t", "mat")],
+                                       C.byref(info)))
+    out.update(mass_kg=info.mass_kg, radius_m=info.radius_m, central_pressure_pa=info.central_pressure_pa)
+    return out
+
+
+def build_planet(spec: PlanetSpec, n: int) -> dict:
+    """A planet of `n` particles in hydrostatic equilibrium (astraea_build_planet); arrays x, y, z,
+    m, h, rho, u, eps_fail (PSU), mat, temp_k and the profile's mass, radius, central pressure."""
+    arrs = {k: np.zeros(n, np.float32) for k in ("x", "y", "z", "m", "h", "rho", "u", "eps_fail", "temp_k")}
+    arrs["mat"] = np.zeros(n, np.uint8)
+    out = BodyOut(*[arrs[k].ctypes.data_as(C.POINTER(C.c_float)) for k in ("x", "y", "z", "m", "h", "rho", "u", "eps_fail")],
+                  arrs["mat"].ctypes.data_as(C.POINTER(C.c_uint8)))
+    info = PlanetInfo()
+    _check(_lib.astraea_build_planet(C.byref(spec), n, C.byref(out), arrs["temp_k"].ctypes.data, C.byref(info)))
+    arrs.update(mass_kg=info.mass_kg, radius_m=info.radius_m, central_pressure_pa=info.central_pressure_pa)
+    return arrs
+
+
+def body_radius_for_mass(layers, mass_kg: float) -> float:
+    """Outer radius (m) of a body of `layers` ([(material, mass_fraction, density or 0), ...],
+    innermost first) holding `mass_kg` (astraea_body_radius_for_mass)."""
+    spec = BodySpec()
+    _lib.astraea_default_body_spec(C.byref(spec))
+    spec.num_layers = len(layers)
+    for k, (mat, frac, dens) in enumerate(layers):
+        spec.layer_material[k] = mat
+        spec.layer_mass_frac[k] = frac
+        spec.layer_density_kgm3[k] = dens
+    r = C.c_double()
+    _check(_lib.astraea_body_radius_for_mass(C.byref(spec), C.c_double(mass_kg), C.byref(r)))
+    return r.value
 
 
 def place_two_bodies(state: dict, n_t: int, n_i: int, r_t_psu: float, r_i_psu: float, v_ratio: float,

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/astraea_hip.h"
+#include "errors.hpp"
 #include "launch.hpp"
 
 using namespace astraea_hip;
@@ -53,6 +54,16 @@ struct FieldLoc {
 
 }  // namespace
 
+int astraea_hip::reportError(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    gLastError = buf;
+    return code;
+}
+
 struct astraea_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -67,6 +78,7 @@ struct astraea_ctx {
     double* dClass = nullptr;              // ASTRAEA_CLASS_STRIDE per segment
     double* dRem = nullptr;                // 3 x 8 per segment: remnant guess | sums | body 0
     unsigned long long* dSeed = nullptr;   // remnant seed key per segment
+    float* dAnchor = nullptr;              // relaxation anchor temperatures by original id (capSlots)
     bool loaded = false, treeValid = false, listValid = false, dtRawValid = false, posNewPending = false;
     int parity = 0;  // number of cur/alt swaps mod 2
     // graph
@@ -328,8 +340,15 @@ int transferField(astraea_ctx* c, int f, void* hostBuf, bool upload) {
             forEachParticle(c, [&](size_t h, size_t s) { d[s] = (double)((const float*)hostBuf)[h]; });
             TRY(copyArray(c, base, b64, sizeof(double), false));
         }
-        c->treeValid = c->listValid = c->dtRawValid = false;
-        c->posNewPending = false;
+        // positions, masses and smoothing lengths shape the tree, its moments and the neighbour
+        // table; the other fields leave them valid
+        const bool shapesTree = (f >= ASTRAEA_F_POS_X && f <= ASTRAEA_F_POS_Z) || f == ASTRAEA_F_MASS ||
+                                f == ASTRAEA_F_H || f == ASTRAEA_F_H0 || (f >= ASTRAEA_F_POS64_X && f <= ASTRAEA_F_POS64_Z);
+        if (shapesTree) {
+            c->treeValid = c->listValid = false;
+            c->posNewPending = false;
+        }
+        c->dtRawValid = false;
     }
     return ASTRAEA_OK;
 }
@@ -988,6 +1007,90 @@ int astraea_classify(astraea_ctx* c, const double* targetRadius, double* out) {
         for (int k = 0; k < 8; ++k) o[ASTRAEA_CL_REM_MASS + k] = rem[8 * (size_t)s + k];
         o[ASTRAEA_CL_ITERATIONS] = iters[s];
     }
+    return ASTRAEA_OK;
+}
+
+// One relaxation step (simulation.cpp:603-615): the KDK sequence with dt_max 0.02 PSU and the
+// thermal clamp in both kicks.  The tree is fitted (the table widened if needed) before its use.
+static int enqueueRelaxKdk(astraea_ctx* c) {
+    hipStream_t s = c->stream;
+    launchStepBegin(c->dc, s, 0.02f);
+    launchKick(c->dc, s, false, true);
+    launchDrift(c->dc, s);
+    enqueueTree(c, false);
+    TRY(checkLaunch(c));
+    TRY(fitNeighbourTable(c));
+    launchPass1(c->dc, s);
+    launchGravity(c->dc, s);
+    launchPass2(c->dc, s);
+    launchKick(c->dc, s, true, true);
+    launchStepEnd(c->dc, s);
+    return checkLaunch(c);
+}
+
+int astraea_relax(astraea_ctx* c, uint32_t steps, const float* anchorTempK, const double* nominalRadius,
+                  double* out) {
+    TRY(requireLoaded(c));
+    if (!nominalRadius || !out) return fail(ASTRAEA_E_INVALID, "null argument");
+    HIP_TRY(hipSetDevice(c->device));
+    const uint32_t S = c->numSeg;
+    uint32_t total = 0;
+    for (uint32_t k = 0; k < S; ++k) total += c->counts[k];
+    if (!c->dAnchor) TRY(devAlloc(c, c->dAnchor, c->capSlots));
+    hipStream_t s = c->stream;
+    double* sums = c->dRem;
+    double* stats = c->dRem + 8 * (size_t)S;
+    TRY(astraea_evaluate_initial_forces(c));
+    if (anchorTempK)
+        HIP_TRY(hipMemcpyAsync(c->dAnchor, anchorTempK, sizeof(float) * total, hipMemcpyHostToDevice, s));
+    else
+        launchAnchorFromEos(c->dc, s, c->dAnchor, total);
+    for (uint32_t k = 0; k < steps; ++k) {
+        TRY(enqueueRelaxKdk(c));
+        // centre, damp, re-anchor, clear stress and damage (simulation.cpp:637-700)
+        launchRelaxCorrect(c->dc, s, sums, c->dAnchor, total, 0.85);
+        if (k + 1 < steps) {  // forces of the corrected state (simulation.cpp:702-706)
+            launchEos(c->dc, s);
+            launchGravity(c->dc, s);
+            launchPass2(c->dc, s);
+            launchDtCandidates(c->dc, s);
+        }
+        TRY(checkLaunch(c));
+    }
+    if (steps > 0) {  // four undamped steps (simulation.cpp:709-726)
+        launchResetBounds(c->dc, s);
+        launchPrepareSort(c->dc, s);
+        enqueueTree(c, false);
+        TRY(checkLaunch(c));
+        TRY(fitNeighbourTable(c));
+        launchPass1(c->dc, s);
+        launchGravity(c->dc, s);
+        launchPass2(c->dc, s);
+        launchDtCandidates(c->dc, s);
+        for (int k = 0; k < 4; ++k) TRY(enqueueRelaxKdk(c));
+        launchEos(c->dc, s);
+    }
+    launchRelaxStats(c->dc, s, stats);
+    TRY(checkLaunch(c));
+    std::vector<double> st(8 * (size_t)S);
+    HIP_TRY(hipMemcpyAsync(st.data(), stats, sizeof(double) * 8 * S, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (uint32_t k = 0; k < S; ++k) {  // simulation.cpp:729-754
+        const double* q = st.data() + 8 * (size_t)k;
+        const double vRms = q[0] > 0.0 ? std::sqrt(q[1] / q[0]) : 0.0;
+        const double rScale = std::max(std::max(nominalRadius[k], std::sqrt(q[2])), 1.0e-3);
+        const double vEsc = std::sqrt(2.0 * 66.7430 * q[0] / rScale);
+        out[k] = vEsc > 0.0 ? vRms / vEsc : 0.0;
+    }
+    if (steps > 0) {  // re-anchor and clear once more (simulation.cpp:757-780)
+        launchReanchor(c->dc, s, c->dAnchor, total);
+        launchEos(c->dc, s);
+        TRY(checkLaunch(c));
+    }
+    // the forces are those of the state before the last corrections: evaluate them again to step
+    c->listValid = false;
+    c->dtRawValid = false;
+    HIP_TRY(hipStreamSynchronize(s));
     return ASTRAEA_OK;
 }
 

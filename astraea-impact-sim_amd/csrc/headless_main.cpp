@@ -1,8 +1,8 @@
 // headless_main.cpp — sim_headless: the reference's command-line runner (engine/headless_main.cpp:98-288)
 // on the gfx950 Simulation.  Same flags, summary line and --verify-conservation gates
 // (|dE|/|E0| < 0.01, |dP|/(M v_esc) < 1e-4, |dL|/|L0| < 0.005; exit 1 on failure or non-finite
-// telemetry, 2 on an exception).  The mutual escape speed uses the bodies' own radii (uniform
-// density per layer) instead of PlanetBuilder::solveHydrostatic1D.
+// telemetry, 2 on an exception).  The mutual escape speed uses the radii of the bodies'
+// hydrostatic profiles, as the reference's does.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -15,26 +15,32 @@
 
 #include "../../include/astraea/presets.hpp"
 #include "../../include/astraea/simulation.hpp"
+#include "../../include/astraea_hip.h"
 
 using namespace astraea;
 
 namespace {
 
-// Radius of a body of uniform material densities (the builder's bodies), metres.
+// Radius of the body's hydrostatic profile, metres (headless_main.cpp:169-177 solves the same).
 double bodyRadius(const BodyInitSpec& b) {
-    struct L { double frac, rho; };
-    std::vector<L> layers;
-    switch (b.archetype) {
-        case PlanetArchetype::IronCore: layers = {{0.68, 7800.0}, {0.32, 2680.0}}; break;
-        case PlanetArchetype::OceanIce: layers = {{0.15, 7800.0}, {0.45, 2680.0}, {0.40, 1000.0}}; break;
-        case PlanetArchetype::LavaIo: layers = {{0.25, 7800.0}, {0.75, 2680.0}}; break;
-        case PlanetArchetype::GasGiant: layers = {{0.05, 7800.0}, {0.10, 2680.0}, {0.85, 0.178}}; break;
-        case PlanetArchetype::ProtoTheia: layers = {{0.22, 7800.0}, {0.58, 2680.0}, {0.20, 2000.0}}; break;
-        default: layers = {{0.32, 7800.0}, {0.675, 2680.0}, {0.005, 1000.0}}; break;
+    const ArchetypeComposition& comp = getArchetypeComposition(b.archetype);
+    astraea_planet_spec ps;
+    astraea_default_planet_spec(&ps);
+    ps.mass_kg = b.massKg;
+    ps.num_layers = 0;
+    for (uint32_t k = 0; k < comp.layerCount; ++k) {
+        if (comp.layers[k].massFraction <= 0.0) continue;
+        ps.layer_material[ps.num_layers] = (uint8_t)comp.layers[k].material;
+        ps.layer_mass_frac[ps.num_layers] = comp.layers[k].massFraction;
+        ++ps.num_layers;
     }
-    double v = 0.0;
-    for (const L& l : layers) v += l.frac / l.rho;
-    return std::cbrt(3.0 * std::max(b.massKg, 1.0) * v / (4.0 * 3.14159265358979323846));
+    ps.surface_temp_k = b.surfaceTempK;
+    ps.core_temp_k = b.coreTempK;
+    ps.surface_pressure_pa = b.archetype == PlanetArchetype::GasGiant ? 1.0e5 : 0.0;
+    astraea_planet_info info{};
+    if (astraea_planet_profile(&ps, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, &info) != ASTRAEA_OK)
+        throw std::invalid_argument(std::string("hydrostatic profile failed: ") + astraea_last_error());
+    return info.radius_m;
 }
 
 double norm3(const double v[3]) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }

@@ -171,33 +171,42 @@ Envelope integrateEnvelope(double K, double rCore, double mCore, double rhoTop, 
     return e;
 }
 
-int buildGasGiant(const astraea_body_spec* spec, uint32_t n, astraea_body_out* out, double* outMass, double* outSpacing) {
-    const int nl = (int)spec->num_layers;
-    const int nc = nl - 1;  // core layers
-    const double K = astraea_hip::material(3).kH2;
-    const double R = spec->radius_m;
-    double fsum = 0.0, rho[ASTRAEA_MAX_LAYERS], f[ASTRAEA_MAX_LAYERS];
-    for (int k = 0; k < nl; ++k) fsum += spec->layer_mass_frac[k];
-    if (!(fsum > 0.0)) return ASTRAEA_E_INVALID;
-    for (int k = 0; k < nl; ++k) {
-        f[k] = spec->layer_mass_frac[k] / fsum;
-        rho[k] = spec->layer_density_kgm3[k] > 0.0 ? spec->layer_density_kgm3[k]
-                                                   : astraea_hip::material(spec->layer_material[k]).rho0;
+// Core layers of uniform density under an n = 1 polytrope H/He envelope (P = K rho^2) holding the
+// outermost layer's mass fraction: for a total mass M the core radii follow from the densities,
+// the density at the core's top from the envelope's mass, and the surface from the envelope.
+struct GasGiantModel {
+    int nl = 0, nc = 0;
+    double K = 0.0, fEnv = 0.0, dr = 0.0;
+    double rho[ASTRAEA_MAX_LAYERS] = {}, f[ASTRAEA_MAX_LAYERS] = {};
+
+    bool init(const astraea_body_spec* spec) {
+        nl = (int)spec->num_layers;
+        nc = nl - 1;
+        K = astraea_hip::material(3).kH2;
+        double fsum = 0.0;
+        for (int k = 0; k < nl; ++k) fsum += spec->layer_mass_frac[k];
+        if (!(fsum > 0.0)) return false;
+        for (int k = 0; k < nl; ++k) {
+            f[k] = spec->layer_mass_frac[k] / fsum;
+            rho[k] = spec->layer_density_kgm3[k] > 0.0 ? spec->layer_density_kgm3[k]
+                                                       : astraea_hip::material(spec->layer_material[k]).rho0;
+        }
+        fEnv = f[nl - 1];
+        if (!(fEnv > 0.0 && fEnv < 1.0)) return false;
+        const double rPure = kPi * std::sqrt(K / (2.0 * kPi * kGsi));
+        dr = rPure / 20000.0;
+        return true;
     }
-    const double fEnv = f[nl - 1];
-    if (!(fEnv > 0.0 && fEnv < 1.0)) return ASTRAEA_E_INVALID;
-    const double rPure = kPi * std::sqrt(K / (2.0 * kPi * kGsi));
-    const double dr = rPure / 20000.0;
-    auto coreTop = [&](double M, double* radii) {
+    double coreTop(double M, double* radii) const {
         double v = 0.0;
         for (int k = 0; k < nc; ++k) {
             v += f[k] * M / rho[k];
             if (radii) radii[k] = std::cbrt(3.0 * v / (4.0 * kPi));
         }
         return std::cbrt(3.0 * v / (4.0 * kPi));
-    };
+    }
     // density at the core's top that puts the envelope's mass fraction inside its surface
-    auto solveTop = [&](double M) {
+    double solveTop(double M) const {
         const double rc = coreTop(M, nullptr), mc = (1.0 - fEnv) * M;
         double lo = std::log(1.0e-8), hi = std::log(1.0e8);
         for (int it = 0; it < 90; ++it) {
@@ -205,12 +214,22 @@ int buildGasGiant(const astraea_body_spec* spec, uint32_t n, astraea_body_out* o
             if (integrateEnvelope(K, rc, mc, std::exp(mid), dr, false).mEnvelope < fEnv * M) lo = mid; else hi = mid;
         }
         return std::exp(0.5 * (lo + hi));
-    };
-    // total mass with the surface at R (the surface radius grows with the mass: the core pushes out)
-    auto surfaceOf = [&](double M) {
-        const double top = solveTop(M);
-        return integrateEnvelope(K, coreTop(M, nullptr), (1.0 - fEnv) * M, top, dr, false).rSurface;
-    };
+    }
+    // surface radius of total mass M (it grows with the mass: the core pushes the envelope out)
+    double surfaceOf(double M) const {
+        return integrateEnvelope(K, coreTop(M, nullptr), (1.0 - fEnv) * M, solveTop(M), dr, false).rSurface;
+    }
+};
+
+int buildGasGiant(const astraea_body_spec* spec, uint32_t n, astraea_body_out* out, double* outMass, double* outSpacing) {
+    GasGiantModel G;
+    if (!G.init(spec)) return ASTRAEA_E_INVALID;
+    const int nc = G.nc;
+    const double K = G.K, R = spec->radius_m, fEnv = G.fEnv, dr = G.dr;
+    const double* rho = G.rho;
+    auto coreTop = [&](double M, double* radii) { return G.coreTop(M, radii); };
+    auto solveTop = [&](double M) { return G.solveTop(M); };
+    auto surfaceOf = [&](double M) { return G.surfaceOf(M); };
     double lo = std::log(1.0e18), hi = std::log(1.0e31);
     if (!(surfaceOf(std::exp(lo)) < R && surfaceOf(std::exp(hi)) > R)) return ASTRAEA_E_INVALID;
     for (int it = 0; it < 70; ++it) {
@@ -329,6 +348,31 @@ int buildGasGiant(const astraea_body_spec* spec, uint32_t n, astraea_body_out* o
 extern "C" {
 
 uint64_t astraea_splitmix64(uint64_t x) { return splitmix(x); }
+
+int astraea_body_radius_for_mass(const astraea_body_spec* spec, double massKg, double* radiusM) {
+    if (!spec || !radiusM || !(massKg > 0.0) || spec->num_layers < 1 || spec->num_layers > ASTRAEA_MAX_LAYERS)
+        return ASTRAEA_E_INVALID;
+    const int nl = (int)spec->num_layers;
+    for (int k = 0; k + 1 < nl; ++k)
+        if (spec->layer_material[k] == 3) return ASTRAEA_E_INVALID;
+    if (spec->layer_material[nl - 1] == 3) {
+        if (nl < 2) return ASTRAEA_E_INVALID;
+        GasGiantModel G;
+        if (!G.init(spec)) return ASTRAEA_E_INVALID;
+        *radiusM = G.surfaceOf(massKg);
+        return std::isfinite(*radiusM) && *radiusM > 0.0 ? ASTRAEA_OK : ASTRAEA_E_INVALID;
+    }
+    double fsum = 0.0, vsum = 0.0;
+    for (int k = 0; k < nl; ++k) {
+        const double rho = spec->layer_density_kgm3[k] > 0.0 ? spec->layer_density_kgm3[k]
+                                                             : astraea_hip::material(spec->layer_material[k]).rho0;
+        fsum += spec->layer_mass_frac[k];
+        vsum += spec->layer_mass_frac[k] / rho;
+    }
+    if (!(fsum > 0.0)) return ASTRAEA_E_INVALID;
+    *radiusM = std::cbrt(3.0 * massKg * (vsum / fsum) / (4.0 * kPi));
+    return ASTRAEA_OK;
+}
 
 void astraea_default_body_spec(astraea_body_spec* s) {
     if (!s) return;

@@ -78,4 +78,12 @@ void launchClassBodies(const DeviceCtx& c, hipStream_t s, double* out, unsigned 
 void launchRemnant(const DeviceCtx& c, hipStream_t s, const double* rem, double* acc);
 void launchClassify(const DeviceCtx& c, hipStream_t s, const double* rem, const double* body, double* out);
 
+// relax.hip: corrections of the damped relaxation.  sums and out hold 8 doubles per segment;
+// anchor holds nAnchor temperatures (K) indexed by original id.
+void launchRelaxCorrect(const DeviceCtx& c, hipStream_t s, double* sums, const float* anchor, uint32_t nAnchor,
+                        double damping);
+void launchReanchor(const DeviceCtx& c, hipStream_t s, const float* anchor, uint32_t nAnchor);
+void launchAnchorFromEos(const DeviceCtx& c, hipStream_t s, float* anchor, uint32_t nAnchor);
+void launchRelaxStats(const DeviceCtx& c, hipStream_t s, double* out);  // sum m, sum m v^2, max |x|^2
+
 }  // namespace astraea_hip

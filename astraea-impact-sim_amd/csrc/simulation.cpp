@@ -13,9 +13,9 @@
 // and spin, impactor centre) comes from the device reductions of astraea_classify, and the
 // collision outcome from the same rules as simulation.cpp:1229-1281.
 //
-// Bodies come from the lattice builder (astraea_build_body: uniform density per layer, the
-// archetype's layer mass fractions, hydrostatic u) instead of planet_builder.cpp, relaxation is
-// not run, and render frames and tracers are not produced.
+// Bodies come from the hydrostatic planet builder (astraea_build_planet) and are relaxed on the
+// device (astraea_relax), both bodies at once, anchored to their profile temperatures; render
+// frames and tracers are not produced.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -105,44 +105,150 @@ void csvRow(std::ostringstream& os, const TelemetrySnapshot& s) {
        << '\n';
 }
 
+// Host particle arrays in input order.
+struct Particles {
+    std::vector<float> x, y, z, vx, vy, vz, m, h, h0, rho, u, eps;
+    std::vector<uint8_t> mat, body;
+
+    size_t size() const { return m.size(); }
+    template <class F>
+    void each(F&& f) {
+        for (auto* v : {&x, &y, &z, &vx, &vy, &vz, &m, &h, &h0, &rho, &u, &eps}) f(*v);
+        f(mat);
+        f(body);
+    }
+    void resize(size_t n) {
+        each([n](auto& v) { v.resize(n); });
+    }
+    void append(const Particles& o) {
+        Particles& t = *this;
+        auto cat = [](auto& a, const auto& b) { a.insert(a.end(), b.begin(), b.end()); };
+        cat(t.x, o.x); cat(t.y, o.y); cat(t.z, o.z); cat(t.vx, o.vx); cat(t.vy, o.vy); cat(t.vz, o.vz);
+        cat(t.m, o.m); cat(t.h, o.h); cat(t.h0, o.h0); cat(t.rho, o.rho); cat(t.u, o.u); cat(t.eps, o.eps);
+        cat(t.mat, o.mat); cat(t.body, o.body);
+    }
+    Particles subset(const std::vector<uint32_t>& idx) const {
+        Particles o;
+        o.resize(idx.size());
+        for (size_t k = 0; k < idx.size(); ++k) {
+            const uint32_t i = idx[k];
+            o.x[k] = x[i]; o.y[k] = y[i]; o.z[k] = z[i]; o.vx[k] = vx[i]; o.vy[k] = vy[i]; o.vz[k] = vz[i];
+            o.m[k] = m[i]; o.h[k] = h[i]; o.h0[k] = h0[i]; o.rho[k] = rho[i]; o.u[k] = u[i]; o.eps[k] = eps[i];
+            o.mat[k] = mat[i]; o.body[k] = body[i];
+        }
+        return o;
+    }
+    // mass-weighted centre and mean velocity
+    void moments(double c[3], double v[3]) const {
+        double M = 0.0;
+        for (int a = 0; a < 3; ++a) c[a] = v[a] = 0.0;
+        for (size_t i = 0; i < size(); ++i) {
+            const double mi = m[i];
+            M += mi;
+            c[0] += mi * x[i]; c[1] += mi * y[i]; c[2] += mi * z[i];
+            v[0] += mi * vx[i]; v[1] += mi * vy[i]; v[2] += mi * vz[i];
+        }
+        if (M > 0.0)
+            for (int a = 0; a < 3; ++a) { c[a] /= M; v[a] /= M; }
+    }
+    void shift(const double c[3], const double v[3]) {
+        for (size_t i = 0; i < size(); ++i) {
+            x[i] = (float)((double)x[i] - c[0]); y[i] = (float)((double)y[i] - c[1]); z[i] = (float)((double)z[i] - c[2]);
+            vx[i] = (float)((double)vx[i] - v[0]); vy[i] = (float)((double)vy[i] - v[1]); vz[i] = (float)((double)vz[i] - v[2]);
+        }
+    }
+    // centerAndZeroBulkVelocity (simulation.cpp:298-330)
+    void centre() {
+        double c[3], v[3];
+        moments(c, v);
+        shift(c, v);
+    }
+};
+
 struct Body {
-    std::vector<float> x, y, z, m, h, rho, u, eps;
-    std::vector<uint8_t> mat;
+    Particles p;
+    std::vector<float> temp;  // profile temperature of each particle (K), the relaxation anchor
     double radiusPsu = 0.0, massKg = 0.0;
 };
 
+void uploadParticles(astraea_ctx* ctx, const Particles& s, const std::vector<uint32_t>& counts) {
+    check(astraea_set_layout(ctx, (uint32_t)counts.size(), counts.data()), "astraea_set_layout");
+    const std::pair<int, const void*> f[] = {
+        {ASTRAEA_F_POS_X, s.x.data()}, {ASTRAEA_F_POS_Y, s.y.data()}, {ASTRAEA_F_POS_Z, s.z.data()},
+        {ASTRAEA_F_VEL_X, s.vx.data()}, {ASTRAEA_F_VEL_Y, s.vy.data()}, {ASTRAEA_F_VEL_Z, s.vz.data()},
+        {ASTRAEA_F_MASS, s.m.data()}, {ASTRAEA_F_H, s.h.data()}, {ASTRAEA_F_H0, s.h0.data()},
+        {ASTRAEA_F_RHO, s.rho.data()}, {ASTRAEA_F_U, s.u.data()}, {ASTRAEA_F_EPS_FAIL, s.eps.data()},
+        {ASTRAEA_F_MATERIAL, s.mat.data()}, {ASTRAEA_F_BODY, s.body.data()}};
+    for (const auto& q : f) check(astraea_upload_field(ctx, q.first, q.second), "astraea_upload_field");
+}
+
+// The loaded particles in input (original id) order.
+Particles downloadParticles(astraea_ctx* ctx, size_t n) {
+    Particles raw;
+    raw.resize(n);
+    std::vector<uint32_t> oid(n);
+    const std::pair<int, void*> f[] = {
+        {ASTRAEA_F_POS_X, raw.x.data()}, {ASTRAEA_F_POS_Y, raw.y.data()}, {ASTRAEA_F_POS_Z, raw.z.data()},
+        {ASTRAEA_F_VEL_X, raw.vx.data()}, {ASTRAEA_F_VEL_Y, raw.vy.data()}, {ASTRAEA_F_VEL_Z, raw.vz.data()},
+        {ASTRAEA_F_MASS, raw.m.data()}, {ASTRAEA_F_H, raw.h.data()}, {ASTRAEA_F_H0, raw.h0.data()},
+        {ASTRAEA_F_RHO, raw.rho.data()}, {ASTRAEA_F_U, raw.u.data()}, {ASTRAEA_F_EPS_FAIL, raw.eps.data()},
+        {ASTRAEA_F_MATERIAL, raw.mat.data()}, {ASTRAEA_F_BODY, raw.body.data()}, {ASTRAEA_F_ORIGINAL_ID, oid.data()}};
+    for (const auto& q : f) check(astraea_download_field(ctx, q.first, q.second), "astraea_download_field");
+    std::vector<uint32_t> slot(n);
+    for (size_t i = 0; i < n; ++i) {
+        if (oid[i] >= n) throw std::runtime_error("downloadParticles: original id out of range");
+        slot[oid[i]] = (uint32_t)i;
+    }
+    return raw.subset(slot);
+}
+
+// Rigid rotation about +Y at omega rad/s (applyRigidSpinAroundY, simulation.cpp:331-341).
+void spinAroundY(Particles& p, double omegaRadS) {
+    const double om = omegaRadS * units::T_UNIT_S;
+    for (size_t i = 0; i < p.size(); ++i) {
+        p.vx[i] = (float)(om * (double)p.z[i]);
+        p.vy[i] = 0.0f;
+        p.vz[i] = (float)(-om * (double)p.x[i]);
+    }
+}
+
+// PlanetBuilder::buildPlanet (planet_builder.cpp:474-735) on the device library's hydrostatic
+// builder: the archetype's layers at their mass fractions, the spec's surface and core
+// temperatures, 1 bar at a gas giant's surface (planet_builder.cpp:407).  Bodies start at rest
+// about their centre; the spin is applied after any relaxation.
 Body buildBody(const BodyInitSpec& spec, uint32_t n, uint64_t seed) {
+    if (!(spec.massKg > 0.0)) throw std::invalid_argument("BodyInitSpec::massKg must be positive");
     const ArchetypeComposition& comp = getArchetypeComposition(spec.archetype);
-    astraea_body_spec bs;
-    astraea_default_body_spec(&bs);
-    double vPerKg = 0.0;
-    bs.num_layers = 0;
+    astraea_planet_spec ps;
+    astraea_default_planet_spec(&ps);
+    ps.mass_kg = spec.massKg;
+    ps.num_layers = 0;
     for (uint32_t k = 0; k < comp.layerCount; ++k) {
         const PlanetLayerDef& l = comp.layers[k];
         if (l.massFraction <= 0.0) continue;
-        if (l.material == MaterialId::HydrogenHelium)
-            throw std::invalid_argument("loadScenario: a body of a given mass with an H/He envelope is not built here "
-                                        "(the polytrope builder fixes the radius, not the mass)");
-        bs.layer_material[bs.num_layers] = (uint8_t)l.material;
-        bs.layer_mass_frac[bs.num_layers] = l.massFraction;
-        bs.layer_density_kgm3[bs.num_layers] = 0.0;
-        ++bs.num_layers;
-        vPerKg += l.massFraction / rho0Of(l.material);
+        ps.layer_material[ps.num_layers] = (uint8_t)l.material;
+        ps.layer_mass_frac[ps.num_layers] = l.massFraction;
+        ++ps.num_layers;
     }
-    const double volume = std::max(spec.massKg, 1.0) * vPerKg;
-    bs.radius_m = std::cbrt(3.0 * volume / (4.0 * kPi));
-    bs.surface_temp_k = spec.surfaceTempK;
-    bs.strength_flaws = 1;
-    bs.seed = seed;
+    ps.surface_temp_k = spec.surfaceTempK;
+    ps.core_temp_k = spec.coreTempK;
+    ps.surface_pressure_pa = spec.archetype == PlanetArchetype::GasGiant ? 1.0e5 : 0.0;
+    ps.seed = seed;
     Body b;
-    for (auto* v : {&b.x, &b.y, &b.z, &b.m, &b.h, &b.rho, &b.u, &b.eps}) v->resize(n);
-    b.mat.resize(n);
-    astraea_body_out out{b.x.data(), b.y.data(), b.z.data(), b.m.data(), b.h.data(), b.rho.data(), b.u.data(),
-                         b.eps.data(), b.mat.data()};
-    double mass = 0.0, spacing = 0.0;
-    check(astraea_build_body(&bs, n, &out, &mass, &spacing), "astraea_build_body");
-    b.radiusPsu = bs.radius_m / units::L_UNIT_M;
-    b.massKg = mass;
+    Particles& q = b.p;
+    q.resize(n);
+    b.temp.resize(n);
+    astraea_body_out out{q.x.data(), q.y.data(), q.z.data(), q.m.data(), q.h.data(), q.rho.data(), q.u.data(),
+                         q.eps.data(), q.mat.data()};
+    astraea_planet_info info{};
+    check(astraea_build_planet(&ps, n, &out, b.temp.data(), &info), "astraea_build_planet");
+    q.h0 = q.h;
+    std::fill(q.vx.begin(), q.vx.end(), 0.0f);
+    std::fill(q.vy.begin(), q.vy.end(), 0.0f);
+    std::fill(q.vz.begin(), q.vz.end(), 0.0f);
+    std::fill(q.body.begin(), q.body.end(), (uint8_t)0);
+    b.radiusPsu = info.radius_m / units::L_UNIT_M;
+    b.massKg = info.mass_kg;
     return b;
 }
 
@@ -191,7 +297,7 @@ struct Simulation::Impl {
     std::vector<TelemetrySnapshot> history;
 
     explicit Impl(uint32_t maxP) : maxParticles(std::max(maxP, 1u)) {
-        check(astraea_create(&ctx, 0, maxParticles, 1), "astraea_create");
+        check(astraea_create(&ctx, 0, maxParticles, 2), "astraea_create");
         if (hipEventCreate(&evStart) != hipSuccess || hipEventCreate(&evStop) != hipSuccess)
             throw std::runtime_error("hipEventCreate failed");
     }
@@ -206,7 +312,7 @@ struct Simulation::Impl {
         astraea_destroy(ctx);
         ctx = nullptr;
         maxParticles = n;
-        check(astraea_create(&ctx, 0, maxParticles, 1), "astraea_create");
+        check(astraea_create(&ctx, 0, maxParticles, 2), "astraea_create");
     }
 
     TelemetrySnapshot telemetry() {
@@ -303,6 +409,78 @@ struct Simulation::Impl {
         else
             s.outcome = CollisionOutcome::Merging;
     }
+
+    void setDefaultParams() {
+        astraea_params p;
+        astraea_default_params(&p);
+        check(astraea_set_params(ctx, &p), "astraea_set_params");
+    }
+
+    // Damped relaxation of bodies at rest (loadScenario, simulation.cpp:1329-1351 and 1366-1385):
+    // the bodies with the same step count relax together on the device, one segment each,
+    // anchored to the builder's temperatures.  Copy-back as
+    // copyRelaxedBodyToBuiltHost (simulation.cpp:783-846): positions, density and u of the
+    // relaxed body, velocities zero, h = h0, the radius the largest |x|, then centred.
+    void relaxBodies(const std::vector<std::pair<Body*, uint32_t>>& list) {
+        std::vector<bool> done(list.size(), false);
+        for (size_t a = 0; a < list.size(); ++a) {
+            if (done[a] || list[a].second == 0) continue;
+            std::vector<Body*> group;
+            for (size_t k = a; k < list.size(); ++k)
+                if (!done[k] && list[k].second == list[a].second) {
+                    group.push_back(list[k].first);
+                    done[k] = true;
+                }
+            Particles all;
+            std::vector<uint32_t> counts;
+            std::vector<double> radii;
+            std::vector<float> anchor;
+            for (Body* b : group) {
+                anchor.insert(anchor.end(), b->temp.begin(), b->temp.end());
+                std::fill(b->p.vx.begin(), b->p.vx.end(), 0.0f);
+                std::fill(b->p.vy.begin(), b->p.vy.end(), 0.0f);
+                std::fill(b->p.vz.begin(), b->p.vz.end(), 0.0f);
+                b->p.centre();
+                all.append(b->p);
+                counts.push_back((uint32_t)b->p.size());
+                radii.push_back(b->radiusPsu);
+            }
+            setDefaultParams();
+            uploadParticles(ctx, all, counts);
+            std::vector<double> vr(group.size());
+            check(astraea_relax(ctx, list[a].second, anchor.data(), radii.data(), vr.data()), "astraea_relax");
+            const Particles r = downloadParticles(ctx, all.size());
+            size_t off = 0;
+            for (Body* b : group) {
+                Particles& q = b->p;
+                double rMax = 0.0;
+                for (size_t i = 0; i < q.size(); ++i) {
+                    q.x[i] = r.x[off + i]; q.y[i] = r.y[off + i]; q.z[i] = r.z[off + i];
+                    q.rho[i] = r.rho[off + i]; q.u[i] = r.u[off + i];
+                    q.vx[i] = q.vy[i] = q.vz[i] = 0.0f;
+                    q.h[i] = q.h0[i];
+                    rMax = std::max(rMax, std::sqrt((double)q.x[i] * q.x[i] + (double)q.y[i] * q.y[i] +
+                                                    (double)q.z[i] * q.z[i]));
+                }
+                if (rMax > 0.0) b->radiusPsu = rMax;
+                q.centre();
+                off += q.size();
+            }
+        }
+    }
+
+    // Forces, counters and history of a freshly uploaded state (simulation.cpp:1449-1453).
+    void startFromUpload(uint32_t n) {
+        count = n;
+        frameSeq = 0;
+        simTimeSec = 0.0;
+        lastDtSec = 0.0f;
+        hasImpacted = false;
+        history.clear();
+        check(astraea_evaluate_initial_forces(ctx), "astraea_evaluate_initial_forces");
+        check(astraea_synchronize(ctx), "astraea_synchronize");
+        telemetry();
+    }
 };
 
 Simulation::Simulation(uint32_t maxParticles) : impl_(std::make_unique<Impl>(maxParticles)) {}
@@ -336,74 +514,73 @@ void Simulation::loadScenario(const CollisionScenarioConfig& config) {
         nT = config.totalParticles - k;
     }
     Body t = buildBody(config.target, nT, config.randomSeed);
+    Body b;
+    if (hasImpactor) b = buildBody(config.impactor, nI, config.randomSeed + 1ULL);
+    std::vector<std::pair<Body*, uint32_t>> relax;
+    if (config.target.relaxSteps > 0u) relax.push_back({&t, config.target.relaxSteps});
+    if (hasImpactor && config.impactor.relaxSteps > 0u) relax.push_back({&b, config.impactor.relaxSteps});
+    I.relaxBodies(relax);
     I.targetRadiusPsu = t.radiusPsu;
     I.livingBodyPresent = config.target.archetype == PlanetArchetype::LivingTerra;
-    const uint32_t n = nT + nI;
-    std::vector<float> x(t.x), y(t.y), z(t.z), m(t.m), h(t.h), rho(t.rho), u(t.u), eps(t.eps);
-    std::vector<uint8_t> mat(t.mat), body(nT, 0);
-    std::vector<float> vx(n, 0.0f), vy(n, 0.0f), vz(n, 0.0f);
-    // rigid spin about +Y (simulation.cpp:331-341)
-    const double omT = config.target.spinOmegaRadS * units::T_UNIT_S;
-    for (uint32_t i = 0; i < nT; ++i) {
-        vx[i] = (float)(omT * (double)z[i]);
-        vz[i] = (float)(-omT * (double)x[i]);
-    }
+    spinAroundY(t.p, config.target.spinOmegaRadS);
+    Particles all = t.p;
     I.qRatioQrd = 0.0;
     if (hasImpactor) {
-        Body b = buildBody(config.impactor, nI, config.randomSeed + 1ULL);
         I.impactorRadiusPsu = b.radiusPsu;
-        const double omI = config.impactor.spinOmegaRadS * units::T_UNIT_S;
-        for (uint32_t i = 0; i < nI; ++i) {
-            x.push_back(b.x[i]); y.push_back(b.y[i]); z.push_back(b.z[i]); m.push_back(b.m[i]);
-            h.push_back(b.h[i]); rho.push_back(b.rho[i]); u.push_back(b.u[i]); eps.push_back(b.eps[i]);
-            mat.push_back(b.mat[i]); body.push_back(1);
-            vx[nT + i] = (float)(omI * (double)b.z[i]);
-            vz[nT + i] = (float)(-omI * (double)b.x[i]);
-        }
+        spinAroundY(b.p, config.impactor.spinOmegaRadS);
+        std::fill(b.p.body.begin(), b.p.body.end(), (uint8_t)1);
+        all.append(b.p);
         double vEscPsu = 0.0;
-        check(astraea_place_two_bodies(nT, nI, x.data(), y.data(), z.data(), vx.data(), vy.data(), vz.data(), m.data(),
-                                       t.radiusPsu, b.radiusPsu, config.impactVelocityRatioVesc, config.impactParamB,
-                                       config.initialSepRadii, &vEscPsu),
+        check(astraea_place_two_bodies(nT, nI, all.x.data(), all.y.data(), all.z.data(), all.vx.data(), all.vy.data(),
+                                       all.vz.data(), all.m.data(), t.radiusPsu, b.radiusPsu,
+                                       config.impactVelocityRatioVesc, config.impactParamB, config.initialSepRadii,
+                                       &vEscPsu),
               "astraea_place_two_bodies");
         I.qRatioQrd = qRatio(t.massKg, b.massKg, t.radiusPsu * units::L_UNIT_M, b.radiusPsu * units::L_UNIT_M,
                              std::max(0.0, config.impactVelocityRatioVesc) * vEscPsu * units::V_UNIT_MS,
                              config.impactParamB);
     }
-    astraea_params p;
-    astraea_default_params(&p);
-    check(astraea_set_params(I.ctx, &p), "astraea_set_params");
-    check(astraea_set_layout(I.ctx, 1, &n), "astraea_set_layout");
-    check(astraea_upload_field(I.ctx, ASTRAEA_F_POS_X, x.data()), "upload");
-    check(astraea_upload_field(I.ctx, ASTRAEA_F_POS_Y, y.data()), "upload");
-    check(astraea_upload_field(I.ctx, ASTRAEA_F_POS_Z, z.data()), "upload");
-    check(astraea_upload_field(I.ctx, ASTRAEA_F_VEL_X, vx.data()), "upload");
-    check(astraea_upload_field(I.ctx, ASTRAEA_F_VEL_Y, vy.data()), "upload");
-    check(astraea_upload_field(I.ctx, ASTRAEA_F_VEL_Z, vz.data()), "upload");
-    check(astraea_upload_field(I.ctx, ASTRAEA_F_MASS, m.data()), "upload");
-    check(astraea_upload_field(I.ctx, ASTRAEA_F_H, h.data()), "upload");
-    check(astraea_upload_field(I.ctx, ASTRAEA_F_H0, h.data()), "upload");
-    check(astraea_upload_field(I.ctx, ASTRAEA_F_RHO, rho.data()), "upload");
-    check(astraea_upload_field(I.ctx, ASTRAEA_F_U, u.data()), "upload");
-    check(astraea_upload_field(I.ctx, ASTRAEA_F_EPS_FAIL, eps.data()), "upload");
-    check(astraea_upload_field(I.ctx, ASTRAEA_F_MATERIAL, mat.data()), "upload");
-    check(astraea_upload_field(I.ctx, ASTRAEA_F_BODY, body.data()), "upload");
-    I.count = n;
-    check(astraea_evaluate_initial_forces(I.ctx), "astraea_evaluate_initial_forces");
-    check(astraea_synchronize(I.ctx), "astraea_synchronize");
-    I.telemetry();
+    I.setDefaultParams();
+    uploadParticles(I.ctx, all, {(uint32_t)all.size()});
+    I.startFromUpload((uint32_t)all.size());
 }
 
 void Simulation::relaxBody(uint8_t bodyId, uint32_t steps, double& outFinalVrmsOverVesc) {
-    (void)bodyId;
-    (void)steps;
-    if (impl_->count == 0u) throw std::runtime_error("Simulation::relaxBody called before loadScenario");
-    // Relaxation is not part of this build: the state is left as loaded and the history reset
-    // as the reference does (simulation.cpp:1464-1477).
-    outFinalVrmsOverVesc = 0.0;
-    impl_->frameSeq = 0;
-    impl_->simTimeSec = 0.0;
-    impl_->history.clear();
-    impl_->telemetry();
+    Impl& I = *impl_;
+    if (I.count == 0u) throw std::runtime_error("Simulation::relaxBody called before loadScenario");
+    // simulation.cpp:1464-1709: the body alone, centred, at rest when steps > 0; relaxed; put back
+    // at its centre with its bulk velocity when other bodies exist; stress and damage cleared.
+    const Particles all = downloadParticles(I.ctx, I.count);
+    std::vector<uint32_t> sel, rest;
+    for (uint32_t i = 0; i < I.count; ++i) (all.body[i] == bodyId ? sel : rest).push_back(i);
+    if (sel.empty()) throw std::invalid_argument("Simulation::relaxBody: no particles for requested bodyId");
+    Particles p = all.subset(sel);
+    double c[3], v[3];
+    p.moments(c, v);
+    const double zero[3] = {0.0, 0.0, 0.0};
+    p.shift(c, zero);
+    if (steps > 0u) {
+        std::fill(p.vx.begin(), p.vx.end(), 0.0f);
+        std::fill(p.vy.begin(), p.vy.end(), 0.0f);
+        std::fill(p.vz.begin(), p.vz.end(), 0.0f);
+    }
+    p.h = p.h0;
+    double nominal = bodyId == 0u ? I.targetRadiusPsu : I.impactorRadiusPsu;
+    I.setDefaultParams();
+    uploadParticles(I.ctx, p, {(uint32_t)p.size()});
+    check(astraea_relax(I.ctx, steps, nullptr, &nominal, &outFinalVrmsOverVesc), "astraea_relax");
+    Particles r = downloadParticles(I.ctx, p.size());
+    double rMax = 0.0;
+    for (size_t i = 0; i < r.size(); ++i)
+        rMax = std::max(rMax, std::sqrt((double)r.x[i] * r.x[i] + (double)r.y[i] * r.y[i] + (double)r.z[i] * r.z[i]));
+    if (!rest.empty()) {
+        const double back[3] = {-c[0], -c[1], -c[2]}, backV[3] = {-v[0], -v[1], -v[2]};
+        r.shift(back, backV);
+    }
+    if (rMax > 0.0) (bodyId == 0u ? I.targetRadiusPsu : I.impactorRadiusPsu) = rMax;
+    if (!rest.empty()) r.append(all.subset(rest));
+    uploadParticles(I.ctx, r, {(uint32_t)r.size()});
+    I.startFromUpload((uint32_t)r.size());
 }
 
 TelemetrySnapshot Simulation::step(double maxDtSec) {

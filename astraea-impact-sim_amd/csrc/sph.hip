@@ -123,6 +123,11 @@ __global__ __launch_bounds__(kBlock) void kPass1(DeviceCtx c) {
     }
 }
 
+// EOS of the current state, and the pass-2 record of the particle refreshed with it: pass 2
+// reads P_eff / rho^2, max(c_s, 0.05), the Balsara factor (from pass 1's div v and curl v and the
+// new c_s) and S / rho^2 from the record, so an EOS evaluation followed by pass 2 without a pass 1
+// in between (the relaxation's forces after its corrections) sees the new state.
+template <bool STRENGTH>
 __global__ __launch_bounds__(kBlock) void kEos(DeviceCtx c) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t sg = c.seg.slotSeg[i];
@@ -131,8 +136,23 @@ __global__ __launch_bounds__(kBlock) void kEos(DeviceCtx c) {
     const float4 hu = c.cur.hu[i];
     const int mat = (int)(c.cur.meta[i] & 0xffu);
     const EosResult e = evaluateEos(mat, (double)vi.w, (double)hu.z, (double)hu.w, (double)c.prm.gammaHHe);
-    c.d.eos[i] = make_float4((float)e.p, (float)e.cs, (float)e.T, 0.0f);
+    const float P = (float)e.p, cs = (float)e.cs;
+    c.d.eos[i] = make_float4(P, cs, (float)e.T, 0.0f);
     c.d.phase[i] = e.phase;
+
+    const float4 a = c.d.p1a[i];
+    const float4 b = c.d.p1b[i];
+    const float csF = fmaxf(cs, 0.05f);
+    const float fBal = balsaraSwitch(a.z, b.x, b.y, b.z, csF, fmaxf(hu.x, 1.0e-6f));
+    const float rhoI = fmaxf(vi.w, 1.0e-9f), invR2 = 1.0f / (rhoI * rhoI);
+    c.d.nb[i] = make_float4(hu.x, effectivePressure(P, e.phase) * invR2, csF, fBal);
+    c.d.p1b[i] = make_float4(b.x, b.y, b.z, fBal);
+    if (STRENGTH) {
+        const float4 s0 = c.cur.str0[i];
+        const float4 s1 = c.cur.str1[i];
+        c.d.sr0[i] = make_float4(s0.x * invR2, s0.y * invR2, s0.z * invR2, s0.w * invR2);
+        c.d.sr1[i] = make_float2(s1.x * invR2, s1.y * invR2);
+    }
 }
 
 template <bool STRENGTH>
@@ -321,7 +341,10 @@ void launchPass1(const DeviceCtx& c, hipStream_t s) {
     else kPass1<false><<<c.blocks, kBlock, 0, s>>>(c);
 }
 
-void launchEos(const DeviceCtx& c, hipStream_t s) { kEos<<<c.blocks, kBlock, 0, s>>>(c); }
+void launchEos(const DeviceCtx& c, hipStream_t s) {
+    if (c.strength) kEos<true><<<c.blocks, kBlock, 0, s>>>(c);
+    else kEos<false><<<c.blocks, kBlock, 0, s>>>(c);
+}
 
 void launchPass2(const DeviceCtx& c, hipStream_t s) {
     if (c.strength) kPass2<true><<<c.blocks, kBlock, 0, s>>>(c);

@@ -122,10 +122,13 @@ int astraea_get_params(astraea_ctx* ctx, astraea_params* out);
 int astraea_set_layout(astraea_ctx* ctx, uint32_t num_segments, const uint32_t* counts);
 int astraea_num_particles(astraea_ctx* ctx, uint32_t* out_total);
 
-/* Copy one field between host (concatenated segments) and device.  Upload is
- * valid for persistent fields; download for every field.  Fields that are not
- * uploaded start as in uploadHostStateToDevice (simulation.cpp:469-520):
- * zeros, except EPS_FAIL = 1, ORIGINAL_ID = index in the concatenation. */
+/* Copy one field between host (concatenated segments) and device, in the context's
+ * current particle order: the input order until the first sort, the sorted order after
+ * it (ORIGINAL_ID holds each particle's input index).  Upload is valid for persistent
+ * fields; download for every field.  Fields that are not uploaded start as in
+ * uploadHostStateToDevice (simulation.cpp:469-520): zeros, except EPS_FAIL = 1,
+ * ORIGINAL_ID = index in the concatenation.  Uploading positions, masses or smoothing
+ * lengths invalidates the tree; other fields leave it valid. */
 int astraea_upload_field(astraea_ctx* ctx, int field, const void* host_src);
 int astraea_download_field(astraea_ctx* ctx, int field, void* host_dst);
 
@@ -228,6 +231,21 @@ int astraea_energy(astraea_ctx* ctx, double* out);
 #define ASTRAEA_CLASS_STRIDE 36
 int astraea_classify(astraea_ctx* ctx, const double* target_radius_psu, double* out);
 
+/* Damped relaxation of the loaded bodies, one body per segment, all segments at once
+ * (replaces Simulation::relaxLoadedSingleBody, simulation.cpp:585-781, which relaxes one body
+ * with a host round trip per step).  Evaluates the initial forces, then `steps` times: a KDK
+ * step with dt_max = 0.02 PSU and the thermal clamp in both kicks; the segment's centre of mass
+ * and mean velocity removed, velocities damped by 0.85, u set to u_cold(rho) + c_v T_anchor,
+ * stress, damage and their rates cleared; forces of the corrected state.  With steps > 0, then
+ * four undamped steps.  `out[s]` = v_rms / v_esc of segment s with v_esc = sqrt(2 G M / r),
+ * r = max(nominal_radius_psu[s], largest |x|) (positions about the origin).  With steps > 0,
+ * u is re-anchored and stress and damage cleared once more at the end.
+ * `anchor_temp_k` holds one temperature (K) per particle, indexed by ORIGINAL_ID (an id past
+ * the end reads 288 K); NULL takes each particle's EOS temperature after the initial forces.
+ * The forces are left stale: evaluate the initial forces again before stepping. */
+int astraea_relax(astraea_ctx* ctx, uint32_t steps, const float* anchor_temp_k,
+                  const double* nominal_radius_psu, double* out_vrms_over_vesc);
+
 /* Counters: `out[0]` = sum over active particles of the particle-node and
  * particle-particle interactions of the gravity evaluations since the last call,
  * `out[1]` = traversal-stack overflows of those (forced acceptances,
@@ -308,6 +326,52 @@ void astraea_default_body_spec(astraea_body_spec* out);
 /* Returns the body's total mass (kg) in *out_mass_kg and lattice spacing (m). */
 int astraea_build_body(const astraea_body_spec* spec, uint32_t n, astraea_body_out* out,
                        double* out_mass_kg, double* out_spacing_m);
+
+/* Layered planet in hydrostatic equilibrium with this library's EOS (replaces
+ * PlanetBuilder::solveHydrostatic1D and buildPlanet, planet_builder.cpp:20-265, 398-735):
+ * RK4 outward from a central pressure, density from the EOS at the pressure and at a
+ * temperature linear in pressure between core and surface, material switched at each
+ * layer's cumulative mass fraction, surface where P reaches surface_pressure_pa; the
+ * central pressure is shot on until the mass is reached.  Particles on Fibonacci-sphere
+ * shells about one spacing (m/rho)^(1/3) thick, each shell turned by a random rotation,
+ * h0 = h_factor (m/rho)^(1/3), u = u_cold(rho) + c_v T, failure strains
+ * eps_min * clamp((-ln U)^(1/9), 1, 3).  Positions about the centre, PSU units. */
+typedef struct astraea_planet_spec {
+    double mass_kg;
+    uint32_t num_layers;                    /* 1..ASTRAEA_MAX_LAYERS, innermost first; H/He only outermost */
+    uint8_t layer_material[ASTRAEA_MAX_LAYERS];
+    double layer_mass_frac[ASTRAEA_MAX_LAYERS];
+    double surface_temp_k;
+    double core_temp_k;
+    double surface_pressure_pa;             /* 1e5 for gas giants, 0 otherwise (planet_builder.cpp:407) */
+    double gamma_hhe;                       /* H/He ideal-gas gamma of the profile (1.4) */
+    double h_factor;                        /* 1.2 (planet_builder.cpp:681) */
+    uint64_t seed;
+} astraea_planet_spec;
+
+typedef struct astraea_planet_info {
+    double mass_kg;                         /* mass of the solved profile */
+    double radius_m;
+    double central_pressure_pa;
+    uint32_t profile_samples;
+} astraea_planet_info;
+
+void astraea_default_planet_spec(astraea_planet_spec* out);
+/* `n` particles into `out` (any pointer may be NULL); `temp_k` (NULL or n floats) receives each
+ * particle's profile temperature, the relaxation anchor of the reference. */
+int astraea_build_planet(const astraea_planet_spec* spec, uint32_t n, astraea_body_out* out, float* temp_k,
+                         astraea_planet_info* info);
+/* The solved radial profile: up to `cap` samples of radius (m), density (kg/m^3), pressure (Pa),
+ * enclosed mass (kg), temperature (K) and material, centre first (any pointer may be NULL). */
+int astraea_planet_profile(const astraea_planet_spec* spec, uint32_t cap, double* r, double* rho, double* p,
+                           double* m, double* t, uint8_t* mat, astraea_planet_info* info);
+
+/* Outer radius (m) of the body `spec` describes (its radius_m is ignored) when its total mass
+ * is `mass_kg`: uniform layers at their densities; for an H/He outermost layer, the surface of
+ * the n = 1 polytrope envelope over the core layers that astraea_build_body builds (no
+ * counterpart in the reference, whose planet_builder.cpp:398-471 integrates the profile to the
+ * surface pressure). */
+int astraea_body_radius_for_mass(const astraea_body_spec* spec, double mass_kg, double* radius_m);
 
 /* Two-body placement on a Keplerian approach orbit in the XZ plane, the
  * formulas of Simulation::loadScenario (simulation.cpp:1389-1436), followed by

@@ -72,6 +72,7 @@ _sig = {
     "or_initial_forces": (C.c_int, [_P]),
     "or_step": (C.c_int, [_P, C.c_double, C.c_uint32]),
     "or_energy": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "or_relax": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, _P, _P]),
     "or_classify_arrays": (None, [C.c_uint32] + [_P] * 10 + [C.c_double, _P]),
     "or_counters": (C.c_int, [_P, C.POINTER(C.c_uint64), C.c_int]),
     "or_neighbour_stats": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
@@ -221,6 +222,17 @@ class Oracle:
         out = np.zeros(len(self.counts) * ENERGY_STRIDE, dtype=np.float64)
         _check(_lib.or_energy(self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
         return out.reshape(len(self.counts), ENERGY_STRIDE)
+
+    def relax(self, steps: int, nominal_radius, anchor_temp=None) -> np.ndarray:
+        """Damped relaxation, one body per segment; returns v_rms / v_esc per segment."""
+        ns = len(self.counts)
+        r = np.ascontiguousarray(np.broadcast_to(np.asarray(nominal_radius, np.float64), (ns,)))
+        out = np.zeros(ns, np.float64)
+        total = int(sum(self.counts))
+        a = None if anchor_temp is None else np.ascontiguousarray(anchor_temp, np.float64)
+        _check(_lib.or_relax(self._h, steps, None if a is None else a.ctypes.data, total, r.ctypes.data,
+                             out.ctypes.data))
+        return out
 
     def counters(self) -> dict:
         out = (C.c_uint64 * 6)()

@@ -20,6 +20,11 @@
 //                                form (tests/cpp/test_gravity_sph.cpp:646-670); Wendland normalisation.
 //   kick / dt                    closed form per particle.
 //   step                         conservation gates of engine/headless_main.cpp:263-281.
+//   relax                        closed forms: steps = 0 gives v_rms / v_esc of the loaded state;
+//                                force-free particles keep 0.85^steps of their velocity about the
+//                                centre of mass through the undamped steps; u after the last
+//                                re-anchor equals u_cold(rho) + c_v T_anchor; stress and damage zero;
+//                                centre of mass and momentum of a real body stay at zero.
 //   classify_arrays              hand-solvable configurations: a test particle on circular,
 //                                parabolic-plus and eccentric orbits about a point remnant (disk /
 //                                ejecta / planet by the vis-viva and periapsis closed forms), rigid
@@ -1126,7 +1131,10 @@ int or_upload_field(or_ctx* c, int f, const void* src) {
         } else {
             return ASTRAEA_E_INVALID;
         }
-        s.tree = false;
+        // positions, masses and smoothing lengths shape the tree; other fields leave it valid
+        if ((f >= ASTRAEA_F_POS_X && f <= ASTRAEA_F_POS_Z) || f == ASTRAEA_F_MASS || f == ASTRAEA_F_H ||
+            f == ASTRAEA_F_H0 || (f >= ASTRAEA_F_POS64_X && f <= ASTRAEA_F_POS64_Z))
+            s.tree = false;
         off += s.n;
     }
     return ASTRAEA_OK;
@@ -1238,6 +1246,97 @@ int or_step(or_ctx* c, double maxDt, uint32_t steps) {
             s.erad += 0.5 * (s.prevRadW + cur) * dtSec / 1.0e30;
             s.prevRadW = cur;
             s.simTime += s.dt;
+        }
+    }
+    return ASTRAEA_OK;
+}
+
+// One relaxation KDK step: engine/simulation.cpp:603-615 (dt_max 0.02, thermal clamp in both kicks).
+static void relaxKdk(or_ctx* c, Seg& s) {
+    s.dt = reduceDt(c, s, 0.02);
+    s.prevDt = s.dt;
+    kick(c, s, 0.5 * s.dt, true);
+    drift(c, s, s.dt);
+    sortAndBuild(c, s);
+    pass1(c, s);
+    eosAll(c, s);
+    gravity(c, s);
+    pass2(c, s);
+    kick(c, s, 0.5 * s.dt, true);
+}
+
+// u from the anchor temperature, stress and damage cleared: engine/simulation.cpp:669-700.
+static void reanchor(Seg& s, const std::vector<double>& anchor) {
+    for (uint32_t i = 0; i < s.n; ++i) {
+        const int mi = matIdx(s.mat[i]);
+        const double t = s.oid[i] < anchor.size() ? anchor[s.oid[i]] : 288.0;
+        s.u[i] = coldEnergy(mi, s.rho[i]) + kMat[mi].cv / kUUnit * t;
+        for (int q = 0; q < 6; ++q) { s.s[q][i] = 0.0; s.dS[q][i] = 0.0; }
+        s.D[i] = 0.0; s.Dc[i] = 0.0; s.dD[i] = 0.0;
+    }
+}
+
+// Simulation::relaxLoadedSingleBody, engine/simulation.cpp:585-781, one body per segment.
+int or_relax(or_ctx* c, uint32_t steps, const double* anchorTemp, uint32_t nAnchor, const double* nominalRadius,
+             double* out) {
+    if (!c || !nominalRadius || !out) return ASTRAEA_E_INVALID;
+    or_initial_forces(c);
+    std::vector<double> anchor;
+    if (anchorTemp) {
+        anchor.assign(anchorTemp, anchorTemp + nAnchor);
+    } else {
+        anchor.assign(nAnchor, 288.0);
+        for (auto& s : c->seg)
+            for (uint32_t i = 0; i < s.n; ++i)
+                if (s.oid[i] < nAnchor) anchor[s.oid[i]] = s.T[i];
+    }
+    for (size_t k = 0; k < c->seg.size(); ++k) {
+        Seg& s = c->seg[k];
+        s.prevDt = 0.0;
+        for (uint32_t st = 0; st < steps && s.n > 0; ++st) {
+            relaxKdk(c, s);
+            double M = 0.0, cx[3] = {}, cv[3] = {};
+            for (uint32_t i = 0; i < s.n; ++i) {
+                M += s.m[i];
+                cx[0] += s.m[i] * s.x[i]; cx[1] += s.m[i] * s.y[i]; cx[2] += s.m[i] * s.z[i];
+                cv[0] += s.m[i] * s.vx[i]; cv[1] += s.m[i] * s.vy[i]; cv[2] += s.m[i] * s.vz[i];
+            }
+            if (M > 0.0)
+                for (int a = 0; a < 3; ++a) { cx[a] /= M; cv[a] /= M; }
+            for (uint32_t i = 0; i < s.n; ++i) {
+                s.x[i] -= cx[0]; s.y[i] -= cx[1]; s.z[i] -= cx[2];
+                s.vx[i] = 0.85 * (s.vx[i] - cv[0]);
+                s.vy[i] = 0.85 * (s.vy[i] - cv[1]);
+                s.vz[i] = 0.85 * (s.vz[i] - cv[2]);
+            }
+            reanchor(s, anchor);
+            if (st + 1 < steps) {
+                eosAll(c, s);
+                gravity(c, s);
+                pass2(c, s);
+            }
+        }
+        if (steps > 0 && s.n > 0) {
+            sortAndBuild(c, s);
+            pass1(c, s);
+            eosAll(c, s);
+            gravity(c, s);
+            pass2(c, s);
+            for (int d = 0; d < 4; ++d) relaxKdk(c, s);
+            eosAll(c, s);
+        }
+        double M = 0.0, mv2 = 0.0, rMax = nominalRadius[k];
+        for (uint32_t i = 0; i < s.n; ++i) {
+            M += s.m[i];
+            mv2 += s.m[i] * (s.vx[i] * s.vx[i] + s.vy[i] * s.vy[i] + s.vz[i] * s.vz[i]);
+            rMax = std::max(rMax, std::sqrt(s.x[i] * s.x[i] + s.y[i] * s.y[i] + s.z[i] * s.z[i]));
+        }
+        const double vRms = M > 0.0 ? std::sqrt(mv2 / M) : 0.0;
+        const double vEsc = std::sqrt(2.0 * kG * M / std::max(rMax, 1.0e-3));
+        out[k] = vEsc > 0.0 ? vRms / vEsc : 0.0;
+        if (steps > 0 && s.n > 0) {
+            reanchor(s, anchor);
+            eosAll(c, s);
         }
     }
     return ASTRAEA_OK;

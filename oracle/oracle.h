@@ -79,6 +79,12 @@ double or_material_param(int mat, int k);
 /* Morton key of one point for given FP32 bounds {minx,miny,minz,maxx,maxy,maxz} */
 uint64_t or_morton_key(float x, float y, float z, const float* bounds6);
 
+/* Damped relaxation of the loaded bodies, one per segment (engine/simulation.cpp:585-781);
+ * anchor_temp (n_anchor doubles, K, by original id) or NULL for the EOS temperatures after the
+ * initial forces.  out[s] = v_rms / v_esc of segment s. */
+int or_relax(or_ctx* c, uint32_t steps, const double* anchor_temp, uint32_t n_anchor, const double* nominal_radius,
+             double* out);
+
 /* Largest-remnant classification of one body pair's arrays (engine/simulation.cpp:1000-1227),
  * in input order; `out` receives ASTRAEA_CLASS_STRIDE doubles (ASTRAEA_CL_* order). */
 void or_classify_arrays(uint32_t n, const double* x, const double* y, const double* z, const double* vx,

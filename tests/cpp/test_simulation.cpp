@@ -1,8 +1,13 @@
-// test_outcomes.cpp — collision-outcome classification end to end through astraea::Simulation:
-// the four cases of the reference's tests/cpp/test_conservation.cpp:479-541 (AC21), same
-// configurations, durations and expectations.  Prints one line per case; the exit status is the
-// number of failed expectations (2 x 100 + k on an exception in case k).
+// test_simulation.cpp — acceptance cases of the reference's C++ tests run end to end through
+// astraea::Simulation on the GPU, same configurations and expectations:
+//   outcomes  tests/cpp/test_conservation.cpp:479-541 (AC21): collision-outcome classification;
+//   relax     tests/cpp/test_planet_builder.cpp:329-360: relaxBody(0, 80) leaves every archetype
+//             with v_rms / v_esc < 0.01.
+// Usage: test_simulation {outcomes|relax}.  Prints one line per case; the exit status is the
+// number of failed expectations (200 + k on an exception in case k).
+#include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <exception>
 
 #include "../../include/astraea/presets.hpp"
@@ -44,7 +49,7 @@ void expect(bool ok, const char* what) {
 
 }  // namespace
 
-int main() {
+int outcomes() {
     int stage = 0;
     try {
         Simulation sim(8000);
@@ -103,4 +108,47 @@ int main() {
         return 200 + stage;
     }
     return fails;
+}
+
+int relax() {
+    const PlanetArchetype archetypes[6] = {PlanetArchetype::LivingTerra, PlanetArchetype::IronCore,
+                                           PlanetArchetype::OceanIce, PlanetArchetype::LavaIo,
+                                           PlanetArchetype::GasGiant, PlanetArchetype::ProtoTheia};
+    int k = 0;
+    try {
+        Simulation sim(12000);
+        for (; k < 6; ++k) {
+            CollisionScenarioConfig cfg;
+            cfg.presetName = "RelaxCheck";
+            cfg.target = makeDefaultBodySpec(archetypes[k]);
+            cfg.target.relaxSteps = 0;
+            cfg.impactor = makeDefaultBodySpec(PlanetArchetype::ProtoTheia);
+            cfg.impactor.massKg = 0.25 * cfg.target.massKg;
+            cfg.impactor.relaxSteps = 0;
+            cfg.impactVelocityRatioVesc = 1.0;
+            cfg.impactParamB = 0.0;
+            cfg.initialSepRadii = 10.0;
+            cfg.totalParticles = 8000;
+            cfg.tracerGridRes = 8;
+            cfg.randomSeed = 42;
+            sim.loadScenario(cfg);
+            double vr = 1.0;
+            sim.relaxBody(0, 80, vr);
+            const bool ok = std::isfinite(vr) && vr < 0.01;
+            fails += ok ? 0 : 1;
+            std::printf("relax archetype=%d vrms_over_vesc=%.3e %s\n", k, vr, ok ? "ok" : "FAIL");
+            std::fflush(stdout);
+        }
+    } catch (const std::exception& e) {
+        std::printf("exception in archetype %d: %s\n", k, e.what());
+        return 200 + k;
+    }
+    return fails;
+}
+
+int main(int argc, char** argv) {
+    if (argc == 2 && std::strcmp(argv[1], "outcomes") == 0) return outcomes();
+    if (argc == 2 && std::strcmp(argv[1], "relax") == 0) return relax();
+    std::fprintf(stderr, "usage: %s {outcomes|relax}\n", argv[0]);
+    return 2;
 }

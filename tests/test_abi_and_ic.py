@@ -221,3 +221,21 @@ def test_gas_giant_envelope_hydrostatic(lib, orc):
     # core: uniform density at each material's reference density, inside the core radius
     core = ~env
     assert r[core].max() <= rc * 1.01 and r[env].min() >= rc * 0.99
+
+
+@pytest.mark.parametrize("which", ["rocky", "gas_giant"])
+def test_body_radius_for_mass_inverts_the_builder(lib, orc, which):
+    """astraea_body_radius_for_mass is the inverse of astraea_build_body's radius -> mass: a body
+    built at radius R has mass M, and the radius for M is R again; for uniform layers it is the
+    closed form R^3 = 3 M sum(f / rho0) / (4 pi)."""
+    from astraea_amd import scenario as S
+
+    layers = S.FE_GRANITE if which == "rocky" else S.GAS_GIANT
+    r = 2.0e6 if which == "rocky" else S.R_GAS_GIANT_M
+    m = lib.build_body(512, r, layers)["mass_kg"]
+    assert lib.body_radius_for_mass(layers, m) == pytest.approx(r, rel=1e-6)
+    if which == "rocky":
+        v = sum(f / orc.material(mat)["rho0"] for mat, f, _ in layers)
+        assert lib.body_radius_for_mass(layers, 1.0e24) ** 3 == pytest.approx(3e24 * v / (4 * np.pi), rel=1e-12)
+    with pytest.raises(lib.AstraeaError):
+        lib.body_radius_for_mass(layers, -1.0)

@@ -199,6 +199,32 @@ def test_force_parity_strength(lib, orc):
     assert rel_rms(g, r) < 1e-5
 
 
+def test_eos_then_pass2_without_pass1(lib, orc):
+    """EOS, gravity and pass 2 with no pass 1 between them (the relaxation's forces after its
+    corrections): pass 2 must see the new pressure, sound speed, Balsara factor and stress.
+    After the initial forces u is raised by 40 % and the stress halved on both sides."""
+    from astraea_amd import scenario as S
+
+    st = perturbed_small_state(strength=True)
+    p = S.plain_params(strength_mask=0x17)
+    ctx, o = run_pair(lib, orc, st, [len(st["mass"])], p, [("initial_forces",)])
+    for side in (ctx, o):
+        oid = side.download("original_id")
+        side.upload("u", (st["u"][oid] * 1.4).astype(np.float32))
+        for k in ("sxx", "syy", "szz", "sxy", "sxz", "syz"):
+            side.upload(k, (st[k][oid] * 0.5).astype(np.float32))
+        side.eos()
+        side.gravity()
+        side.pass2()
+    ctx.synchronize()
+    ag, ar = vec_by_id(ctx, o, ["acc_x", "acc_y", "acc_z"])
+    gg, gr = vec_by_id(ctx, o, ["acc_grav_x", "acc_grav_y", "acc_grav_z"])
+    assert rel_rms(ag - gg, ar - gr) < 2e-5
+    for name, tol in (("du_dt", 1e-5), ("drho_dt", 1e-5), ("ds_xx", 1e-4), ("pressure", 1e-6)):
+        g, r = by_id(ctx, o, name, f64=True)
+        assert rel_rms(g, r) < tol, (name, rel_rms(g, r))
+
+
 @pytest.mark.parametrize("theta", [0.45, 0.7])
 def test_gravity_parity_other_theta(lib, orc, theta):
     """theta^2 not a power of two (the acceptance threshold is found by ulp steps), and

@@ -157,7 +157,7 @@ def test_collision_outcomes_ac21():
     import subprocess
     from pathlib import Path
 
-    exe = Path(__file__).resolve().parent / "cpp" / "test_outcomes"
-    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=110)
+    exe = Path(__file__).resolve().parent / "cpp" / "test_simulation"
+    r = subprocess.run([str(exe), "outcomes"], capture_output=True, text=True, timeout=110)
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stdout + r.stderr
