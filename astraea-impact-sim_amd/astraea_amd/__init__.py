@@ -159,6 +159,12 @@ _sig = {
     "astraea_get_dt": (C.c_int, [_P, C.POINTER(C.c_float)]),
     "astraea_energy": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "astraea_classify": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "astraea_pack_render": (C.c_int, [_P, _P, _P, _P]),
+    "astraea_set_tracers": (C.c_int, [_P, C.c_uint32, _P, _P, _P]),
+    "astraea_extract_tracers": (C.c_int, [_P, _P, _P, _P]),
+    "astraea_select_tracers": (C.c_int, [C.c_uint32] + [_P] * 5 + [C.c_uint32] + [_P] * 5),
+    "astraea_cube_face_uv_to_normal": (None, [C.c_uint32, C.c_float, C.c_float, _P, _P, _P]),
+    "astraea_normal_to_cube_face_uv": (None, [C.c_float, C.c_float, C.c_float, _P, _P, _P]),
     "astraea_relax": (C.c_int, [_P, C.c_uint32, _P, _P, _P]),
     "astraea_counters": (C.c_int, [_P, C.POINTER(C.c_uint64), C.c_int]),
     "astraea_set_neighbour_stride": (C.c_int, [_P, C.c_uint32]),
@@ -344,6 +350,33 @@ class Context:
                                   out.ctypes.data))
         return out
 
+    def pack_render(self, com_psu, speed: bool = False):
+        """Render records (n x 20 bytes) about com_psu (3 per segment) and, if asked, speed bytes."""
+        ns, n = len(self.counts), self.n
+        com = np.ascontiguousarray(np.broadcast_to(np.asarray(com_psu, np.float64).reshape(-1, 3), (ns, 3)))
+        rec = np.zeros((n, 20), np.uint8)
+        spd = np.zeros(n, np.uint8) if speed else None
+        _check(_lib.astraea_pack_render(self._h, com.ctypes.data, rec.ctypes.data, None if spd is None else spd.ctypes.data))
+        return (rec, spd) if speed else rec
+
+    def set_tracers(self, particle_id, rest, ocean):
+        """Tracer t on the particle of ORIGINAL_ID particle_id[t]; rest: (count, 4) radius + normal."""
+        pid = np.ascontiguousarray(particle_id, np.uint32)
+        r = np.ascontiguousarray(rest, np.float32).reshape(-1, 4)
+        oc = np.ascontiguousarray(ocean, np.uint8)
+        assert len(pid) == len(r) == len(oc)
+        _check(_lib.astraea_set_tracers(self._h, len(pid), pid.ctypes.data, r.ctypes.data, oc.ctypes.data))
+        self._tracers = len(pid)
+
+    def extract_tracers(self, living_com_psu, sys_com_psu):
+        """40-byte tracer records (count x 40) about the given centres (3 per segment)."""
+        ns = len(self.counts)
+        lc = np.ascontiguousarray(np.broadcast_to(np.asarray(living_com_psu, np.float64).reshape(-1, 3), (ns, 3)))
+        sc = np.ascontiguousarray(np.broadcast_to(np.asarray(sys_com_psu, np.float64).reshape(-1, 3), (ns, 3)))
+        out = np.zeros((getattr(self, "_tracers", 0), 40), np.uint8)
+        _check(_lib.astraea_extract_tracers(self._h, lc.ctypes.data, sc.ctypes.data, out.ctypes.data))
+        return out
+
     def counters(self) -> dict:
         out = (C.c_uint64 * 7)()
         _check(_lib.astraea_counters(self._h, out, 7))
@@ -452,6 +485,33 @@ def build_planet(spec: PlanetSpec, n: int) -> dict:
     _check(_lib.astraea_build_planet(C.byref(spec), n, C.byref(out), arrs["temp_k"].ctypes.data, C.byref(info)))
     arrs.update(mass_kg=info.mass_kg, radius_m=info.radius_m, central_pressure_pa=info.central_pressure_pa)
     return arrs
+
+
+def select_tracers(x, y, z, h0, material, grid_res: int) -> dict:
+    """LivingTerra crust tracers of a body about its centre (astraea_select_tracers)."""
+    n = len(x)
+    a = [np.ascontiguousarray(v, np.float32) for v in (x, y, z, h0)]
+    mat = np.ascontiguousarray(material, np.uint8)
+    out = dict(index=np.zeros(n, np.int32), rest_radius=np.zeros(n, np.float32), rest_normal=np.zeros((n, 3), np.float32),
+               ocean=np.zeros(n, np.uint8))
+    cnt = C.c_uint32()
+    _check(_lib.astraea_select_tracers(n, *[v.ctypes.data for v in a], mat.ctypes.data, grid_res,
+                                       out["index"].ctypes.data, out["rest_radius"].ctypes.data,
+                                       out["rest_normal"].ctypes.data, out["ocean"].ctypes.data, C.byref(cnt)))
+    out["count"] = cnt.value
+    return out
+
+
+def cube_face_uv_to_normal(face: int, u: float, v: float):
+    n = [C.c_float() for _ in range(3)]
+    _lib.astraea_cube_face_uv_to_normal(face, u, v, *[C.byref(q) for q in n])
+    return tuple(q.value for q in n)
+
+
+def normal_to_cube_face_uv(nx: float, ny: float, nz: float):
+    f, u, v = C.c_uint32(), C.c_float(), C.c_float()
+    _lib.astraea_normal_to_cube_face_uv(nx, ny, nz, C.byref(f), C.byref(u), C.byref(v))
+    return f.value, u.value, v.value
 
 
 def body_radius_for_mass(layers, mass_kg: float) -> float:

@@ -79,6 +79,17 @@ struct astraea_ctx {
     double* dRem = nullptr;                // 3 x 8 per segment: remnant guess | sums | body 0
     unsigned long long* dSeed = nullptr;   // remnant seed key per segment
     float* dAnchor = nullptr;              // relaxation anchor temperatures by original id (capSlots)
+    // render frames
+    uint32_t* dRender = nullptr;           // 5 dwords per particle (capSlots)
+    uint8_t* dSpeed = nullptr;             // capSlots
+    float* dCom = nullptr;                 // 2 x 3 floats per segment
+    uint32_t* dOutBase = nullptr;          // per segment
+    // crust tracers: tables by tracer and by original id, never permuted
+    int32_t* dTracerOf = nullptr;          // tracer of each original id, -1 for none (capSlots)
+    float4* dTracerRest = nullptr;         // (rest radius, rest normal) per tracer
+    uint8_t* dTracerOcean = nullptr;
+    uint32_t* dTracerOut = nullptr;        // 10 dwords per tracer
+    uint32_t tracerCount = 0, tracerCap = 0;
     bool loaded = false, treeValid = false, listValid = false, dtRawValid = false, posNewPending = false;
     int parity = 0;  // number of cur/alt swaps mod 2
     // graph
@@ -628,6 +639,7 @@ int astraea_set_layout(astraea_ctx* c, uint32_t ns, const uint32_t* counts) {
     freeGraph(c);
     c->numSeg = ns;
     c->counts.assign(counts, counts + ns);
+    c->tracerCount = 0;  // the layout clears the tracer flags with the rest of the state
     c->offs = offs;
     c->slots = (uint32_t)std::max<uint64_t>(slots, ASTRAEA_SEGMENT_ALIGN);
     DeviceCtx& d = c->dc;
@@ -1091,6 +1103,97 @@ int astraea_relax(astraea_ctx* c, uint32_t steps, const float* anchorTempK, cons
     c->listValid = false;
     c->dtRawValid = false;
     HIP_TRY(hipStreamSynchronize(s));
+    return ASTRAEA_OK;
+}
+
+static int ensureRenderBuffers(astraea_ctx* c) {
+    if (c->dRender) return ASTRAEA_OK;
+    TRY(devAlloc(c, c->dRender, 5 * (size_t)c->capSlots));
+    TRY(devAlloc(c, c->dSpeed, c->capSlots));
+    TRY(devAlloc(c, c->dCom, 6 * (size_t)c->maxSeg));
+    TRY(devAlloc(c, c->dOutBase, c->maxSeg));
+    TRY(devAlloc(c, c->dTracerOf, c->capSlots));
+    return ASTRAEA_OK;
+}
+
+int astraea_pack_render(astraea_ctx* c, const double* comPsu, uint8_t* records, uint8_t* speed) {
+    TRY(requireLoaded(c));
+    if (!comPsu || !records) return fail(ASTRAEA_E_INVALID, "null argument");
+    HIP_TRY(hipSetDevice(c->device));
+    TRY(ensureRenderBuffers(c));
+    const uint32_t S = c->numSeg;
+    std::vector<float> com(3 * (size_t)S);
+    std::vector<uint32_t> base(S);
+    uint32_t total = 0;
+    for (uint32_t k = 0; k < S; ++k) {
+        for (int a = 0; a < 3; ++a) com[3 * k + a] = (float)comPsu[3 * k + a];
+        base[k] = total;
+        total += c->counts[k];
+    }
+    HIP_TRY(hipMemcpyAsync(c->dCom, com.data(), sizeof(float) * 3 * S, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->dOutBase, base.data(), sizeof(uint32_t) * S, hipMemcpyHostToDevice, c->stream));
+    launchPackRender(c->dc, c->stream, c->dCom, c->dOutBase, c->dRender, speed ? c->dSpeed : nullptr);
+    TRY(checkLaunch(c));
+    HIP_TRY(hipMemcpyAsync(records, c->dRender, 20 * (size_t)total, hipMemcpyDeviceToHost, c->stream));
+    if (speed) HIP_TRY(hipMemcpyAsync(speed, c->dSpeed, total, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return ASTRAEA_OK;
+}
+
+int astraea_set_tracers(astraea_ctx* c, uint32_t count, const uint32_t* particleId, const float* rest,
+                        const uint8_t* ocean) {
+    TRY(requireLoaded(c));
+    if (count > 0 && (!particleId || !rest || !ocean)) return fail(ASTRAEA_E_INVALID, "null argument");
+    if (count > 65535u) return fail(ASTRAEA_E_INVALID, "%u tracers: the record's anchor index is 16 bits", count);
+    HIP_TRY(hipSetDevice(c->device));
+    TRY(ensureRenderBuffers(c));
+    uint32_t total = 0;
+    for (uint32_t k = 0; k < c->numSeg; ++k) total += c->counts[k];
+    std::vector<int32_t> of(total, -1);
+    for (uint32_t t = 0; t < count; ++t) {
+        if (particleId[t] >= total) return fail(ASTRAEA_E_INVALID, "tracer %u: particle id %u out of range", t, particleId[t]);
+        if (of[particleId[t]] >= 0) return fail(ASTRAEA_E_INVALID, "particle %u carries two tracers", particleId[t]);
+        of[particleId[t]] = (int32_t)t;
+    }
+    if (count > c->tracerCap) {
+        // (re)allocated outside any step; the old tables are released with the context
+        TRY(devAlloc(c, c->dTracerRest, count));
+        TRY(devAlloc(c, c->dTracerOcean, count));
+        TRY(devAlloc(c, c->dTracerOut, 10 * (size_t)count));
+        c->tracerCap = count;
+    }
+    HIP_TRY(hipMemcpyAsync(c->dTracerOf, of.data(), sizeof(int32_t) * total, hipMemcpyHostToDevice, c->stream));
+    if (count > 0) {
+        HIP_TRY(hipMemcpyAsync(c->dTracerRest, rest, sizeof(float4) * count, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->dTracerOcean, ocean, count, hipMemcpyHostToDevice, c->stream));
+    }
+    c->tracerCount = count;
+    launchMarkTracers(c->dc, c->stream, c->dTracerOf, total);
+    TRY(checkLaunch(c));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return ASTRAEA_OK;
+}
+
+int astraea_extract_tracers(astraea_ctx* c, const double* livingComPsu, const double* sysComPsu, uint8_t* records) {
+    TRY(requireLoaded(c));
+    if (!livingComPsu || !sysComPsu || (c->tracerCount > 0 && !records)) return fail(ASTRAEA_E_INVALID, "null argument");
+    if (c->tracerCount == 0) return ASTRAEA_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    const uint32_t S = c->numSeg;
+    uint32_t total = 0;
+    for (uint32_t k = 0; k < S; ++k) total += c->counts[k];
+    std::vector<float> com(6 * (size_t)S);
+    for (uint32_t k = 0; k < 3 * S; ++k) {
+        com[k] = (float)livingComPsu[k];
+        com[3 * S + k] = (float)sysComPsu[k];
+    }
+    HIP_TRY(hipMemcpyAsync(c->dCom, com.data(), sizeof(float) * 6 * S, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemsetAsync(c->dTracerOut, 0, 40 * (size_t)c->tracerCount, c->stream));
+    launchExtractTracers(c->dc, c->stream, c->dTracerOf, total, c->dTracerRest, c->dTracerOcean, c->tracerCount,
+                         c->dCom, c->dCom + 3 * S, c->dTracerOut);
+    TRY(checkLaunch(c));
+    HIP_TRY(hipMemcpyAsync(records, c->dTracerOut, 40 * (size_t)c->tracerCount, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return ASTRAEA_OK;
 }
 

@@ -86,4 +86,14 @@ void launchReanchor(const DeviceCtx& c, hipStream_t s, const float* anchor, uint
 void launchAnchorFromEos(const DeviceCtx& c, hipStream_t s, float* anchor, uint32_t nAnchor);
 void launchRelaxStats(const DeviceCtx& c, hipStream_t s, double* out);  // sum m, sum m v^2, max |x|^2
 
+// render.hip: 20-byte particle records (5 dwords each) at outBase[segment] + local index, speed
+// bytes; 40-byte tracer records (10 dwords each) at the tracer index.  com, livingCom and sysCom
+// hold 3 floats per segment.
+void launchPackRender(const DeviceCtx& c, hipStream_t s, const float* com, const uint32_t* outBase, uint32_t* records,
+                      uint8_t* speed);
+void launchExtractTracers(const DeviceCtx& c, hipStream_t s, const int32_t* tracerOfOid, uint32_t nOid,
+                          const float4* rest, const uint8_t* ocean, uint32_t nTracers, const float* livingCom,
+                          const float* sysCom, uint32_t* out);
+void launchMarkTracers(const DeviceCtx& c, hipStream_t s, const int32_t* tracerOfOid, uint32_t nOid);
+
 }  // namespace astraea_hip

@@ -454,4 +454,159 @@ int astraea_build_planet(const astraea_planet_spec* spec, uint32_t n, astraea_bo
     return ASTRAEA_OK;
 }
 
+void astraea_cube_face_uv_to_normal(uint32_t face, float u, float v, float* nx, float* ny, float* nz) {
+    const float sx = 2.0f * u - 1.0f, sy = 2.0f * v - 1.0f;
+    // face axes (planet_builder.cpp:329-349): +X, -X, +Y, -Y, +Z, -Z
+    float vx, vy, vz;
+    switch (face % 6u) {
+        case 0: vx = 1.0f; vy = sy; vz = -sx; break;
+        case 1: vx = -1.0f; vy = sy; vz = sx; break;
+        case 2: vx = sx; vy = 1.0f; vz = -sy; break;
+        case 3: vx = sx; vy = -1.0f; vz = sy; break;
+        case 4: vx = sx; vy = sy; vz = 1.0f; break;
+        default: vx = -sx; vy = sy; vz = -1.0f; break;
+    }
+    const float inv = 1.0f / std::sqrt(vx * vx + vy * vy + vz * vz);
+    *nx = vx * inv;
+    *ny = vy * inv;
+    *nz = vz * inv;
+}
+
+void astraea_normal_to_cube_face_uv(float nx, float ny, float nz, uint32_t* face, float* u, float* v) {
+    // the dominant axis picks the face; the other two, divided by it, are the face coordinates
+    // (planet_builder.cpp:351-395)
+    const float ax = std::fabs(nx), ay = std::fabs(ny), az = std::fabs(nz);
+    float sx, sy;
+    if (ax >= ay && ax >= az) {
+        const float d = std::max(ax, 1.0e-12f);
+        *face = nx >= 0.0f ? 0u : 1u;
+        sx = (nx >= 0.0f ? -nz : nz) / d;
+        sy = ny / d;
+    } else if (ay >= ax && ay >= az) {
+        const float d = std::max(ay, 1.0e-12f);
+        *face = ny >= 0.0f ? 2u : 3u;
+        sx = nx / d;
+        sy = (ny >= 0.0f ? -nz : nz) / d;
+    } else {
+        const float d = std::max(az, 1.0e-12f);
+        *face = nz >= 0.0f ? 4u : 5u;
+        sx = (nz >= 0.0f ? nx : -nx) / d;
+        sy = ny / d;
+    }
+    *u = std::clamp(0.5f * (sx + 1.0f), 0.0f, 1.0f);
+    *v = std::clamp(0.5f * (sy + 1.0f), 0.0f, 1.0f);
+}
+
+int astraea_select_tracers(uint32_t n, const float* x, const float* y, const float* z, const float* h0,
+                           const uint8_t* mat, uint32_t gridRes, int32_t* tracerIndex, float* restRadius,
+                           float* restNormal, uint8_t* ocean, uint32_t* outCount) {
+    if (!x || !y || !z || !h0 || !mat || !tracerIndex || !restRadius || !restNormal || !ocean || !outCount)
+        return astraea_hip::reportError(ASTRAEA_E_INVALID, "null argument");
+    constexpr uint8_t kWaterIce = 2;
+    for (uint32_t i = 0; i < n; ++i) {
+        tracerIndex[i] = -1;
+        restRadius[i] = 0.0f;
+        restNormal[3 * i] = restNormal[3 * i + 1] = restNormal[3 * i + 2] = 0.0f;
+        ocean[i] = 0;
+    }
+    *outCount = 0;
+    const uint64_t target64 = 6ull * gridRes * gridRes;
+    if (gridRes == 0 || target64 > 65535ull || n < target64) return ASTRAEA_OK;
+    const uint32_t target = (uint32_t)target64;
+    // candidates from the outside in (planet_builder.cpp:748-771)
+    std::vector<float> rad(n);
+    std::vector<uint32_t> cand;
+    for (uint32_t i = 0; i < n; ++i) {
+        rad[i] = std::sqrt(x[i] * x[i] + y[i] * y[i] + z[i] * z[i]);
+        if (rad[i] > 1.0e-6f) cand.push_back(i);
+    }
+    std::stable_sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) { return rad[a] > rad[b]; });
+    for (uint32_t i = 0; i < n && cand.size() < target; ++i)
+        if (rad[i] <= 1.0e-6f) cand.push_back(i);
+    if (cand.empty()) return ASTRAEA_OK;
+    // the pool: the outermost shell (1.8 smoothing lengths) and any ocean, at least 3 x target
+    const float rMax = rad[cand.front()], drSurf = h0[cand.front()];
+    const size_t minPool = std::min(cand.size(), (size_t)target * 3u);
+    size_t pool = 0;
+    for (; pool < cand.size(); ++pool) {
+        const uint32_t q = cand[pool];
+        const bool shell = rad[q] >= rMax - 1.8f * drSurf || mat[q] == kWaterIce;
+        if (!shell && pool >= minPool) break;
+    }
+    if (pool < target) pool = std::min(cand.size(), (size_t)target);
+    cand.resize(pool);
+    std::vector<float> cn(3 * pool);
+    std::vector<uint8_t> used(pool, 0);
+    std::vector<std::vector<uint32_t>> bins(target);
+    const uint32_t G = gridRes;
+    for (size_t k = 0; k < pool; ++k) {
+        const uint32_t q = cand[k];
+        if (rad[q] > 1.0e-6f) {
+            const float inv = 1.0f / rad[q];
+            cn[3 * k] = x[q] * inv; cn[3 * k + 1] = y[q] * inv; cn[3 * k + 2] = z[q] * inv;
+        } else {
+            cn[3 * k] = 0.0f; cn[3 * k + 1] = 0.0f; cn[3 * k + 2] = 1.0f;
+        }
+        uint32_t face;
+        float u, v;
+        astraea_normal_to_cube_face_uv(cn[3 * k], cn[3 * k + 1], cn[3 * k + 2], &face, &u, &v);
+        const uint32_t gx = std::min(G - 1u, (uint32_t)(u * (float)G));
+        const uint32_t gy = std::min(G - 1u, (uint32_t)(v * (float)G));
+        bins[face * G * G + gy * G + gx].push_back((uint32_t)k);
+    }
+    std::vector<float> tn(3 * (size_t)target);
+    std::vector<uint8_t> filled(target, 0);
+    uint32_t assigned = 0;
+    auto assign = [&](uint32_t b, size_t k) {
+        const uint32_t q = cand[k];
+        used[k] = 1;
+        filled[b] = 1;
+        ++assigned;
+        tracerIndex[q] = (int32_t)b;
+        restRadius[q] = rad[q];
+        for (int a = 0; a < 3; ++a) restNormal[3 * q + a] = cn[3 * k + a];
+        ocean[q] = mat[q] == kWaterIce ? 1u : 0u;
+    };
+    auto best = [&](uint32_t b, const std::vector<uint32_t>* within) {
+        float bestDot = -2.0f;
+        size_t bestK = SIZE_MAX;
+        auto consider = [&](size_t k) {
+            if (used[k]) return;
+            const float d = cn[3 * k] * tn[3 * b] + cn[3 * k + 1] * tn[3 * b + 1] + cn[3 * k + 2] * tn[3 * b + 2];
+            if (d > bestDot) {
+                bestDot = d;
+                bestK = k;
+            }
+        };
+        if (within)
+            for (uint32_t k : *within) consider(k);
+        else
+            for (size_t k = 0; k < pool; ++k) consider(k);
+        return bestK;
+    };
+    // pass A: in each cube-sphere cell, the candidate nearest the cell centre (planet_builder.cpp:834-859)
+    for (uint32_t face = 0; face < 6; ++face)
+        for (uint32_t gy = 0; gy < G; ++gy)
+            for (uint32_t gx = 0; gx < G; ++gx) {
+                const uint32_t b = face * G * G + gy * G + gx;
+                astraea_cube_face_uv_to_normal(face, (gx + 0.5f) / (float)G, (gy + 0.5f) / (float)G, &tn[3 * b],
+                                               &tn[3 * b + 1], &tn[3 * b + 2]);
+                const size_t k = best(b, &bins[b]);
+                if (k != SIZE_MAX) assign(b, k);
+            }
+    // pass B: empty cells take the nearest unused candidate of the pool (861-881)
+    for (uint32_t b = 0; b < target; ++b) {
+        if (filled[b]) continue;
+        const size_t k = best(b, nullptr);
+        if (k != SIZE_MAX) assign(b, k);
+    }
+    if (assigned < target) {  // compact the indices (883-891)
+        int32_t next = 0;
+        for (uint32_t i = 0; i < n; ++i)
+            if (tracerIndex[i] >= 0) tracerIndex[i] = next++;
+    }
+    *outCount = assigned;
+    return ASTRAEA_OK;
+}
+
 }  // extern "C"

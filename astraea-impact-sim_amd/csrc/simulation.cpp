@@ -168,8 +168,26 @@ struct Particles {
 struct Body {
     Particles p;
     std::vector<float> temp;  // profile temperature of each particle (K), the relaxation anchor
+    // crust tracers, per particle (empty when the body has none)
+    std::vector<int32_t> tracer;
+    std::vector<float> restR, restN;
+    std::vector<uint8_t> ocean;
+    uint32_t tracerCount = 0;
     double radiusPsu = 0.0, massKg = 0.0;
 };
+
+// LivingTerra crust tracers of a body about its centre (planet_builder.cpp:745-906).
+void selectTracers(Body& b, uint32_t gridRes) {
+    const size_t n = b.p.size();
+    b.tracer.assign(n, -1);
+    b.restR.assign(n, 0.0f);
+    b.restN.assign(3 * n, 0.0f);
+    b.ocean.assign(n, 0);
+    check(astraea_select_tracers((uint32_t)n, b.p.x.data(), b.p.y.data(), b.p.z.data(), b.p.h0.data(), b.p.mat.data(),
+                                 gridRes, b.tracer.data(), b.restR.data(), b.restN.data(), b.ocean.data(),
+                                 &b.tracerCount),
+          "astraea_select_tracers");
+}
 
 void uploadParticles(astraea_ctx* ctx, const Particles& s, const std::vector<uint32_t>& counts) {
     check(astraea_set_layout(ctx, (uint32_t)counts.size(), counts.data()), "astraea_set_layout");
@@ -291,6 +309,13 @@ struct Simulation::Impl {
     double targetRadiusPsu = 1.0;
     double impactorRadiusPsu = 0.0;
     bool hasImpacted = false;
+    // centres of the last telemetry (PSU), for render frames
+    double sysCom[3] = {0.0, 0.0, 0.0}, livingCom[3] = {0.0, 0.0, 0.0};
+    // crust tracers of the loaded state, by tracer index
+    uint32_t tracerCount = 0, tracerGridRes = 0;
+    std::vector<uint32_t> trParticle;
+    std::vector<float> trRest;  // 4 per tracer
+    std::vector<uint8_t> trOcean;
     uint32_t count = 0;
     bool livingBodyPresent = false;
     TelemetrySnapshot lastSnapshot{};
@@ -325,6 +350,8 @@ struct Simulation::Impl {
         s.qRatioQrd = qRatioQrd;
         s.livingBodyPresent = livingBodyPresent;
         s.initialTotalEnergyJ = initialTotalEnergyJ;
+        s.tracerCount = tracerCount;
+        s.tracerGridRes = tracerCount > 0 ? tracerGridRes : 0;
         if (count == 0) return s;
         double e[ASTRAEA_ENERGY_STRIDE];
         check(astraea_energy(ctx, e), "astraea_energy");
@@ -365,6 +392,10 @@ struct Simulation::Impl {
         const double* sys = e + ASTRAEA_EN_COM_X;
         const double* com0 = c + ASTRAEA_CL_BODY0_COM;
         const double* com1 = c + ASTRAEA_CL_BODY1_COM;
+        for (int a = 0; a < 3; ++a) {  // simulation.cpp:1033-1053
+            sysCom[a] = total > 0.0 ? sys[a] : 0.0;
+            livingCom[a] = m0 > 0.0 ? com0[a] : sysCom[a];
+        }
         if (livingBodyPresent) {
             for (int a = 0; a < 3; ++a) s.livingBodyComKm[a] = ((m0 > 0.0 ? com0[a] : sys[a]) - sys[a]) * 1000.0;
             const double iyy = c[ASTRAEA_CL_SPIN_I];
@@ -447,6 +478,27 @@ struct Simulation::Impl {
             }
             setDefaultParams();
             uploadParticles(ctx, all, counts);
+            {  // the bodies' tracers stay flagged while they relax (the living-body cooling test)
+                std::vector<uint32_t> id;
+                std::vector<float> rest;
+                std::vector<uint8_t> oc;
+                uint32_t base = 0;
+                for (Body* b : group) {
+                    std::vector<std::pair<int32_t, uint32_t>> byT;
+                    for (size_t i = 0; i < b->tracer.size(); ++i)
+                        if (b->tracer[i] >= 0) byT.push_back({b->tracer[i], (uint32_t)i});
+                    std::sort(byT.begin(), byT.end());
+                    for (const auto& q : byT) {
+                        id.push_back(base + q.second);
+                        rest.push_back(b->restR[q.second]);
+                        for (int a = 0; a < 3; ++a) rest.push_back(b->restN[3 * q.second + a]);
+                        oc.push_back(b->ocean[q.second]);
+                    }
+                    base += (uint32_t)b->p.size();
+                }
+                check(astraea_set_tracers(ctx, (uint32_t)id.size(), id.data(), rest.data(), oc.data()),
+                      "astraea_set_tracers");
+            }
             std::vector<double> vr(group.size());
             check(astraea_relax(ctx, list[a].second, anchor.data(), radii.data(), vr.data()), "astraea_relax");
             const Particles r = downloadParticles(ctx, all.size());
@@ -459,8 +511,16 @@ struct Simulation::Impl {
                     q.rho[i] = r.rho[off + i]; q.u[i] = r.u[off + i];
                     q.vx[i] = q.vy[i] = q.vz[i] = 0.0f;
                     q.h[i] = q.h0[i];
-                    rMax = std::max(rMax, std::sqrt((double)q.x[i] * q.x[i] + (double)q.y[i] * q.y[i] +
-                                                    (double)q.z[i] * q.z[i]));
+                    const double ri = std::sqrt((double)q.x[i] * q.x[i] + (double)q.y[i] * q.y[i] + (double)q.z[i] * q.z[i]);
+                    rMax = std::max(rMax, ri);
+                    if (!b->tracer.empty() && b->tracer[i] >= 0) {  // tracers at rest where the body settled
+                        b->restR[i] = (float)ri;
+                        if (ri > 1.0e-9) {
+                            b->restN[3 * i] = (float)(q.x[i] / ri);
+                            b->restN[3 * i + 1] = (float)(q.y[i] / ri);
+                            b->restN[3 * i + 2] = (float)(q.z[i] / ri);
+                        }
+                    }
                 }
                 if (rMax > 0.0) b->radiusPsu = rMax;
                 q.centre();
@@ -470,7 +530,30 @@ struct Simulation::Impl {
     }
 
     // Forces, counters and history of a freshly uploaded state (simulation.cpp:1449-1453).
+    // The tracer table on the device (cleared by every upload of a layout).
+    void uploadTracers() {
+        check(astraea_set_tracers(ctx, tracerCount, trParticle.data(), trRest.data(), trOcean.data()),
+              "astraea_set_tracers");
+    }
+
+    void tracersFromBody(const Body& b, uint32_t firstParticle, uint32_t gridRes) {
+        tracerCount = b.tracerCount;
+        tracerGridRes = gridRes;
+        trParticle.assign(tracerCount, 0);
+        trRest.assign(4 * (size_t)tracerCount, 0.0f);
+        trOcean.assign(tracerCount, 0);
+        for (size_t i = 0; i < b.tracer.size(); ++i) {
+            const int32_t t = b.tracer[i];
+            if (t < 0 || (uint32_t)t >= tracerCount) continue;
+            trParticle[t] = firstParticle + (uint32_t)i;
+            trRest[4 * t] = b.restR[i];
+            for (int a = 0; a < 3; ++a) trRest[4 * t + 1 + a] = b.restN[3 * i + a];
+            trOcean[t] = b.ocean[i];
+        }
+    }
+
     void startFromUpload(uint32_t n) {
+        uploadTracers();
         count = n;
         frameSeq = 0;
         simTimeSec = 0.0;
@@ -514,6 +597,8 @@ void Simulation::loadScenario(const CollisionScenarioConfig& config) {
         nT = config.totalParticles - k;
     }
     Body t = buildBody(config.target, nT, config.randomSeed);
+    if (config.target.archetype == PlanetArchetype::LivingTerra && config.tracerGridRes > 0)
+        selectTracers(t, config.tracerGridRes);
     Body b;
     if (hasImpactor) b = buildBody(config.impactor, nI, config.randomSeed + 1ULL);
     std::vector<std::pair<Body*, uint32_t>> relax;
@@ -540,6 +625,7 @@ void Simulation::loadScenario(const CollisionScenarioConfig& config) {
                              std::max(0.0, config.impactVelocityRatioVesc) * vEscPsu * units::V_UNIT_MS,
                              config.impactParamB);
     }
+    I.tracersFromBody(t, 0, config.tracerGridRes);  // the target's particles come first
     I.setDefaultParams();
     uploadParticles(I.ctx, all, {(uint32_t)all.size()});
     I.startFromUpload((uint32_t)all.size());
@@ -568,11 +654,42 @@ void Simulation::relaxBody(uint8_t bodyId, uint32_t steps, double& outFinalVrmsO
     double nominal = bodyId == 0u ? I.targetRadiusPsu : I.impactorRadiusPsu;
     I.setDefaultParams();
     uploadParticles(I.ctx, p, {(uint32_t)p.size()});
+    std::vector<uint32_t> newIndex(I.count, 0);
+    for (size_t k = 0; k < sel.size(); ++k) newIndex[sel[k]] = (uint32_t)k;
+    for (size_t k = 0; k < rest.size(); ++k) newIndex[rest[k]] = (uint32_t)(sel.size() + k);
+    {  // the body's tracers stay flagged while it relaxes
+        std::vector<uint32_t> id;
+        std::vector<float> tr;
+        std::vector<uint8_t> oc;
+        for (uint32_t t = 0; t < I.tracerCount; ++t) {
+            const uint32_t q = newIndex[I.trParticle[t]];
+            if (q >= p.size()) continue;
+            id.push_back(q);
+            tr.insert(tr.end(), I.trRest.begin() + 4 * t, I.trRest.begin() + 4 * t + 4);
+            oc.push_back(I.trOcean[t]);
+        }
+        check(astraea_set_tracers(I.ctx, (uint32_t)id.size(), id.data(), tr.data(), oc.data()), "astraea_set_tracers");
+    }
     check(astraea_relax(I.ctx, steps, nullptr, &nominal, &outFinalVrmsOverVesc), "astraea_relax");
     Particles r = downloadParticles(I.ctx, p.size());
     double rMax = 0.0;
     for (size_t i = 0; i < r.size(); ++i)
         rMax = std::max(rMax, std::sqrt((double)r.x[i] * r.x[i] + (double)r.y[i] * r.y[i] + (double)r.z[i] * r.z[i]));
+    // tracers follow their particles into the merged order (the relaxed body first); those of
+    // the relaxed body rest where it settled (simulation.cpp:1624-1648)
+    for (uint32_t t = 0; t < I.tracerCount; ++t) {
+        const uint32_t q = newIndex[I.trParticle[t]];
+        I.trParticle[t] = q;
+        if (q < p.size()) {
+            const double ri = std::sqrt((double)r.x[q] * r.x[q] + (double)r.y[q] * r.y[q] + (double)r.z[q] * r.z[q]);
+            I.trRest[4 * t] = (float)ri;
+            if (ri > 1.0e-9) {
+                I.trRest[4 * t + 1] = (float)(r.x[q] / ri);
+                I.trRest[4 * t + 2] = (float)(r.y[q] / ri);
+                I.trRest[4 * t + 3] = (float)(r.z[q] / ri);
+            }
+        }
+    }
     if (!rest.empty()) {
         const double back[3] = {-c[0], -c[1], -c[2]}, backV[3] = {-v[0], -v[1], -v[2]};
         r.shift(back, backV);
@@ -646,12 +763,27 @@ TelemetrySnapshot Simulation::computeTelemetry() {
     return impl_->telemetry();
 }
 
+// simulation.cpp:1805-1867: particle records about the system centre, tracer records about the
+// living body's, both of the last telemetry.
 void Simulation::extractRenderFrame(std::vector<uint8_t>& outParticleBytes, std::vector<uint8_t>& outTracerBytes,
                                     std::vector<uint8_t>* outSpeedBytes) {
-    // Render packing (sph_gravity.cu:1554-1743) is outside the physics path built here.
-    outParticleBytes.clear();
-    outTracerBytes.clear();
-    if (outSpeedBytes) outSpeedBytes->clear();
+    Impl& I = *impl_;
+    if (I.count == 0u) {
+        outParticleBytes.clear();
+        outTracerBytes.clear();
+        if (outSpeedBytes) outSpeedBytes->clear();
+        return;
+    }
+    outParticleBytes.resize(20u * (size_t)I.count);
+    if (outSpeedBytes) outSpeedBytes->resize(I.count);
+    check(astraea_pack_render(I.ctx, I.sysCom, outParticleBytes.data(), outSpeedBytes ? outSpeedBytes->data() : nullptr),
+          "astraea_pack_render");
+    if (I.tracerCount > 0u) {
+        outTracerBytes.resize(40u * (size_t)I.tracerCount);
+        check(astraea_extract_tracers(I.ctx, I.livingCom, I.sysCom, outTracerBytes.data()), "astraea_extract_tracers");
+    } else {
+        outTracerBytes.clear();
+    }
 }
 
 std::string Simulation::exportTelemetryCsv() const {

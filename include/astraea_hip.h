@@ -231,6 +231,30 @@ int astraea_energy(astraea_ctx* ctx, double* out);
 #define ASTRAEA_CLASS_STRIDE 36
 int astraea_classify(astraea_ctx* ctx, const double* target_radius_psu, double* out);
 
+/* Render frame particle records (packRenderRecordsKernel, sph_gravity.cu:1671-1743, as
+ * Simulation::extractRenderFrame uses it, simulation.cpp:1805-1867): 20 bytes per particle,
+ * little-endian: x, y, z km about com_psu of the particle's segment (3 doubles per segment,
+ * PSU) as f32; u16 log temperature; u16 log pressure (0 at P <= 1e-3 GPa); u8 material;
+ * u8 body << 4 | phase; u8 damage * 255; u8 log radius of the volume-equivalent sphere.
+ * `speed` (NULL or one byte per particle): u8 log speed.  Records are in the context's
+ * current particle order (ORIGINAL_ID gives the input index), segments concatenated. */
+int astraea_pack_render(astraea_ctx* ctx, const double* com_psu, uint8_t* records, uint8_t* speed);
+
+/* Crust tracers (the tracer fields of the reference's particle buffers, types.hpp:150-156,
+ * filled by planet_builder.cpp:737-906).  Tracer t sits on the particle whose ORIGINAL_ID is
+ * particle_id[t]; rest = 4 floats per tracer (rest radius PSU, rest unit normal x, y, z about
+ * the living body's centre); ocean = 1 for an ocean tracer.  Sets the TRACER flag of those
+ * particles (which drives the living-body test of the cooling) and clears it elsewhere.  At most
+ * 65535 tracers.  set_layout clears the tracers. */
+int astraea_set_tracers(astraea_ctx* ctx, uint32_t count, const uint32_t* particle_id, const float* rest,
+                        const uint8_t* ocean);
+/* Crust tracer records (extractTracersKernel, sph_gravity.cu:1586-1668): 40 bytes per tracer,
+ * at its index: u16 index, u16 flags (1 ocean, 2 disrupted), f32 x, y, z km about sys_com_psu,
+ * radial displacement km and normal change (rad) about living_com_psu, T (K), damage,
+ * overpressure (kPa), tsunami height (m).  Centres: 3 doubles per segment. */
+int astraea_extract_tracers(astraea_ctx* ctx, const double* living_com_psu, const double* sys_com_psu,
+                            uint8_t* records);
+
 /* Damped relaxation of the loaded bodies, one body per segment, all segments at once
  * (replaces Simulation::relaxLoadedSingleBody, simulation.cpp:585-781, which relaxes one body
  * with a host round trip per step).  Evaluates the initial forces, then `steps` times: a KDK
@@ -365,6 +389,21 @@ int astraea_build_planet(const astraea_planet_spec* spec, uint32_t n, astraea_bo
  * enclosed mass (kg), temperature (K) and material, centre first (any pointer may be NULL). */
 int astraea_planet_profile(const astraea_planet_spec* spec, uint32_t cap, double* r, double* rho, double* p,
                            double* m, double* t, uint8_t* mat, astraea_planet_info* info);
+
+/* Crust tracers of a LivingTerra body (planet_builder.cpp:737-906): of the n particles
+ * (positions about the body's centre, PSU), the outermost shell (within 1.8 h0 of the largest
+ * radius, plus any water ice, at least 3 x 6 G^2 candidates) is binned on the cube-sphere grid
+ * of 6 G^2 cells; each cell takes the candidate nearest its centre, then empty cells the nearest
+ * unused candidate of the pool; indices are compacted when fewer than 6 G^2 were found.  Per
+ * particle: tracer_index (-1 for none), rest_radius, rest_normal (3 floats), ocean (water ice).
+ * `out_count` receives the number of tracers (0 when n < 6 G^2 or 6 G^2 > 65535). */
+int astraea_select_tracers(uint32_t n, const float* x, const float* y, const float* z, const float* h0,
+                           const uint8_t* material, uint32_t grid_res, int32_t* tracer_index,
+                           float* rest_radius, float* rest_normal, uint8_t* ocean, uint32_t* out_count);
+/* Cube-sphere mapping of the tracer grid (planet_builder.cpp:329-395): face 0..5 = +X, -X, +Y,
+ * -Y, +Z, -Z; (u, v) in [0, 1]^2. */
+void astraea_cube_face_uv_to_normal(uint32_t face, float u, float v, float* nx, float* ny, float* nz);
+void astraea_normal_to_cube_face_uv(float nx, float ny, float nz, uint32_t* face, float* u, float* v);
 
 /* Outer radius (m) of the body `spec` describes (its radius_m is ignored) when its total mass
  * is `mass_kg`: uniform layers at their densities; for an H/He outermost layer, the surface of

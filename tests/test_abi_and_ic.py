@@ -239,3 +239,15 @@ def test_body_radius_for_mass_inverts_the_builder(lib, orc, which):
         assert lib.body_radius_for_mass(layers, 1.0e24) ** 3 == pytest.approx(3e24 * v / (4 * np.pi), rel=1e-12)
     with pytest.raises(lib.AstraeaError):
         lib.body_radius_for_mass(layers, -1.0)
+
+
+def test_pcls_wire_format():
+    """The PCLS frame of server/binary_protocol.cpp (include/astraea/binary_protocol.hpp): header
+    bytes at their offsets (magic "SLCP", version, flags, f64 time, counts, raw payload size),
+    payload order particles | tracers | speed, a speed array of the wrong size dropped, wrong
+    magic / version / short headers rejected, zlib round trips (no GPU is used)."""
+    import subprocess
+
+    exe = Path(__file__).resolve().parent / "cpp" / "test_simulation"
+    r = subprocess.run([str(exe), "pcls"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
