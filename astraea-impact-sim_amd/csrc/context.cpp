@@ -150,17 +150,36 @@ void freeGraph(astraea_ctx* c) {
 
 constexpr uint32_t kMaxNeighbourStride = 1u << 16;
 
+// Union entries a wavefront may hold for a table of `stride` entries per slot: as many bytes as
+// the entries themselves, and at most 2^16 (the entries are 16-bit positions).
+uint32_t unionCapOf(uint32_t stride) { return (uint32_t)std::min<uint64_t>(32ull * stride, 1ull << 16); }
+
+template <class T>
+void releaseArray(astraea_ctx* c, T*& p) {
+    if (!p) return;
+    (void)hipFree(p);
+    auto it = std::find(c->allocs.begin(), c->allocs.end(), (void*)p);
+    if (it != c->allocs.end()) c->allocs.erase(it);
+    p = nullptr;
+}
+
 int allocNeighbourList(astraea_ctx* c, uint32_t stride) {
-    if (c->dc.nb.list) {
-        (void)hipFree(c->dc.nb.list);
-        auto it = std::find(c->allocs.begin(), c->allocs.end(), (void*)c->dc.nb.list);
-        if (it != c->allocs.end()) c->allocs.erase(it);
-        c->dc.nb.list = nullptr;
-    }
+    releaseArray(c, c->dc.nb.list);
+    releaseArray(c, c->dc.nb.unionIds);
+    const uint32_t uc = unionCapOf(stride);
     TRY(devAlloc(c, c->dc.nb.list, (size_t)stride * c->capSlots));
+    TRY(devAlloc(c, c->dc.nb.unionIds, (size_t)uc * (c->capSlots / kWave)));
     c->dc.nb.stride = stride;
+    c->dc.nb.unionCap = uc;
     freeGraph(c);
     return ASTRAEA_OK;
+}
+
+// Table width for the sizes a walk wanted (counter 7: the width the fullest bank of a slot needs,
+// counter 5: the largest union of a wavefront), with a quarter to spare.
+uint64_t wantedStride(const unsigned long long* v) {
+    const uint64_t need = std::max<uint64_t>(v[7], (v[5] + 31ull) / 32ull);
+    return std::min<uint64_t>(need + need / 4, kMaxNeighbourStride);
 }
 
 Params toDeviceParams(const astraea_params& p) {
@@ -534,6 +553,7 @@ int astraea_create(astraea_ctx** out, int device, uint32_t capacity, uint32_t ma
     A(devAlloc(c, d.nb.count, C));
     A(devAlloc(c, d.nb.waveSum, C / kWave));
     A(devAlloc(c, d.nb.waveMax, C / kWave));
+    A(devAlloc(c, d.nb.unionLen, C / kWave));
     A(devAlloc(c, d.seg.off, maxSegments + 1));
     A(devAlloc(c, d.seg.n, maxSegments));
     A(devAlloc(c, d.seg.slotSeg, C));
@@ -678,9 +698,10 @@ int astraea_set_layout(astraea_ctx* c, uint32_t ns, const uint32_t* counts) {
     HIP_TRY(hipMemset(d.d.dS1, 0, sizeof(float2) * c->slots));
     HIP_TRY(hipMemset(d.d.sr1, 0, sizeof(float2) * c->slots));
     HIP_TRY(hipMemset(d.d.phase, 0, c->slots));
-    HIP_TRY(hipMemset(d.nb.count, 0, sizeof(uint32_t) * c->slots));
+    HIP_TRY(hipMemset(d.nb.count, 0, sizeof(uint2) * c->slots));
     HIP_TRY(hipMemset(d.nb.waveSum, 0, sizeof(uint32_t) * (c->slots / kWave)));
     HIP_TRY(hipMemset(d.nb.waveMax, 0, sizeof(uint32_t) * (c->slots / kWave)));
+    HIP_TRY(hipMemset(d.nb.unionLen, 0, sizeof(uint32_t) * (c->slots / kWave)));
     HIP_TRY(hipMemset(d.ss.dtPrev, 0, sizeof(float) * ns));
     HIP_TRY(hipMemset(d.ss.dt, 0, sizeof(float) * ns));
     HIP_TRY(hipMemset(d.ss.erad, 0, sizeof(double) * ns));
@@ -732,14 +753,15 @@ static int fitNeighbourTable(astraea_ctx* c) {
     unsigned long long v[8];
     HIP_TRY(hipMemcpy(v, c->dc.cnt.v, sizeof(v), hipMemcpyDeviceToHost));
     if (v[3] == 0ull) return ASTRAEA_OK;
-    const uint64_t need = v[7];
-    if (need > kMaxNeighbourStride)
-        return fail(ASTRAEA_E_OVERFLOW, "a particle has %llu neighbour candidates, more than the table can hold (%u)",
-                    (unsigned long long)need, kMaxNeighbourStride);
+    if (v[7] > kMaxNeighbourStride || v[5] > (1ull << 16))
+        return fail(ASTRAEA_E_OVERFLOW,
+                    "a particle needs a table of %llu entries and a wavefront a union of %llu, more than the table "
+                    "can hold (%u and %u)",
+                    (unsigned long long)v[7], (unsigned long long)v[5], kMaxNeighbourStride, 1u << 16);
     HIP_TRY(hipMemset(c->dc.cnt.v + 3, 0, sizeof(unsigned long long)));
+    HIP_TRY(hipMemset(c->dc.cnt.v + 5, 0, sizeof(unsigned long long)));
     HIP_TRY(hipMemset(c->dc.cnt.v + 7, 0, sizeof(unsigned long long)));
-    const uint64_t want = std::min<uint64_t>(need + need / 4, kMaxNeighbourStride);
-    TRY(allocNeighbourList(c, std::max<uint32_t>(roundUp((uint32_t)want, 64), c->dc.nb.stride)));
+    TRY(allocNeighbourList(c, std::max<uint32_t>(roundUp((uint32_t)wantedStride(v), 64), c->dc.nb.stride)));
     launchNeighbours(c->dc, c->stream);
     return ASTRAEA_OK;
 }
@@ -889,17 +911,16 @@ int astraea_synchronize(astraea_ctx* c) {
     if (v[3] != 0ull || v[6] != 0ull) {
         // clear the sticky flags; a table too narrow is widened for the following steps
         HIP_TRY(hipMemset(c->dc.cnt.v + 3, 0, sizeof(unsigned long long)));
+        HIP_TRY(hipMemset(c->dc.cnt.v + 5, 0, sizeof(unsigned long long)));
         HIP_TRY(hipMemset(c->dc.cnt.v + 6, 0, 2 * sizeof(unsigned long long)));
         if (v[3] != 0ull) {
             const uint32_t was = c->dc.nb.stride;
-            const uint64_t need = v[7];
-            const uint64_t want = std::min<uint64_t>(need + need / 4, kMaxNeighbourStride);
-            TRY(allocNeighbourList(c, std::max<uint32_t>(roundUp((uint32_t)want, 64), was)));
+            TRY(allocNeighbourList(c, std::max<uint32_t>(roundUp((uint32_t)wantedStride(v), 64), was)));
             c->listValid = false;
             return fail(ASTRAEA_E_OVERFLOW,
-                        "neighbour table overflow: a particle has %llu neighbour candidates, the table holds %u per "
-                        "particle (widened to %u; the affected step is invalid)",
-                        (unsigned long long)need, was, c->dc.nb.stride);
+                        "neighbour table overflow: a particle needs %llu entries and a wavefront a union of %llu, the "
+                        "table holds %u per particle (widened to %u; the affected step is invalid)",
+                        (unsigned long long)v[7], (unsigned long long)v[5], was, c->dc.nb.stride);
         }
         return fail(ASTRAEA_E_OVERFLOW, "neighbour traversal stack overflow");
     }
@@ -1201,7 +1222,7 @@ int astraea_set_neighbour_stride(astraea_ctx* c, uint32_t stride) {
     if (!c) return fail(ASTRAEA_E_INVALID, "null context");
     if (stride == 0 || stride > kMaxNeighbourStride)
         return fail(ASTRAEA_E_INVALID, "neighbour stride %u outside [1, %u]", stride, kMaxNeighbourStride);
-    stride = roundUp(stride, 4);  // the table is kept in chunks of four entries
+    stride = roundUp(stride, 8 * kBanks);  // kBanks lists per slot, in chunks of eight entries
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));
     TRY(allocNeighbourList(c, stride));

@@ -72,42 +72,118 @@ struct Tree {
     float4* bmax;         // (max x, y, z, max support bound of the subtree)
 };
 
+// Banks of the neighbour table (see tableChunks).
+constexpr uint32_t kBanks = 4;
+
 struct Neighbours {
-    uint32_t* count;      // entries of each slot (at most stride)
+    uint2* count;         // entries of each slot per bank (four u16, bank b in half b & 1 of word b >> 1)
     uint32_t* waveSum;    // entries wanted by each wavefront's particles (uncapped)
     uint32_t* waveMax;    // largest entry count one particle of the wavefront wanted
-    uint32_t* list;       // see tableChunks
-    uint32_t stride;      // entries per slot, a multiple of 4
+    uint16_t* list;       // see tableChunks
+    uint32_t* unionIds;   // per wavefront: the union of its particles' neighbours (slots), unionCap each
+    uint32_t* unionLen;   // per wavefront: length of that union (uncapped)
+    uint32_t stride;      // entries per slot, a multiple of 8 kBanks (stride / kBanks per bank)
+    uint32_t unionCap;    // union entries per wavefront, a multiple of kBanks (<= 65536)
 };
 
-// Table layout: entry k of slot i = 64 w + l is word (k & 3) of chunk 64 (k >> 2) + l of the
-// wavefront's block, which starts at word 64 w stride.  A lane's entries come in 16-byte chunks,
-// so the build writes one chunk per four entries and a pass reads four with one load; chunk r of
-// the 64 lanes is one contiguous kilobyte.
+// Table layout.  Each wavefront w owns a union list: the slots that are a neighbour of at least
+// one of its particles (or one of them).  The q-th slot the walk found goes to bank q % kBanks at
+// position q / kBanks; bank b of w's union starts at unionIds + w unionCap + b unionCap / kBanks.
+// Dealing the union out to the banks like cards gives every particle about the same number of
+// entries in each bank, so the rounds of a pass (one bank each, staged in LDS) keep the lanes
+// equally busy; in walk order a particle's neighbours would crowd into a few rounds.
+// An entry is a 16-bit position in its bank, and a particle's entries of one bank ascend.  Each
+// slot's `stride` entries are kBanks lists of stride / kBanks; entry k of bank b of slot
+// i = 64 w + l is half-word (k & 7) of chunk 64 (b stride / (8 kBanks) + (k >> 3)) + l of the
+// wavefront's block, which starts at half-word 64 w stride: a lane's entries come in 16-byte
+// chunks of eight, chunk r of the 64 lanes is one contiguous kilobyte.
 __device__ __forceinline__ uint4* tableChunks(const Neighbours& nb, uint32_t i) {
     return reinterpret_cast<uint4*>(nb.list + (uint64_t)(i >> 6) * (64ull * nb.stride)) + (i & 63u);
 }
+__device__ __forceinline__ uint32_t bankChunks(const Neighbours& nb) { return nb.stride / (8u * kBanks); }
+__device__ __forceinline__ uint32_t bankCount(uint2 c, uint32_t b) {
+    return ((b < 2u ? c.x : c.y) >> (16u * (b & 1u))) & 0xffffu;
+}
 
-// Walks the table row of slot i four entries at a time, in table order.  The next chunk of
-// indices and the neighbour records of the current one are requested together, so a lane keeps
-// about eight loads in flight instead of one dependent chain per entry.  load(j, rec) gathers
-// the record of neighbour j, pair(rec) consumes it.
-template <class Rec, class Load, class Pair>
-__device__ __forceinline__ void forEachNeighbour(const Neighbours& nb, uint32_t i, Load&& load, Pair&& pair) {
-    const uint32_t cnt = nb.count[i];
-    const uint4* ch = tableChunks(nb, i);
-    uint4 jn = cnt > 0u ? ch[0] : make_uint4(0u, 0u, 0u, 0u);
-    for (uint32_t k = 0; k < cnt; k += 4u) {
-        const uint32_t js[4] = {jn.x, jn.y, jn.z, jn.w};
-        jn = k + 4u < cnt ? ch[64ull * ((k >> 2) + 1u)] : make_uint4(0u, 0u, 0u, 0u);
-        Rec rec[4];
+// Orders the LDS accesses of a wavefront's lanes around this point: the lanes of one wavefront
+// run in lockstep and LDS serves a wavefront's accesses in order, so only the compiler must be
+// kept from moving accesses across.
+__device__ __forceinline__ void waveSync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Walks the neighbours of slot i.  Every lane of the wavefront must call it together (inactive
+// slots have no entries).  Bank by bank, the wave's union is staged through `buf` (S records per
+// round): the round's slot ids are loaded together, then stage(slot, rec) fills the records with
+// coalesced loads; each lane consumes its entries that fall in the round, pair(rec) in table
+// order, reading the next entry's record from LDS before it runs pair on the current one.
+// Past the end of a bank a round stages copies of its last slot (never read).
+template <int S, class Rec, class Stage, class Pair>
+__device__ __forceinline__ void forEachNeighbour(const Neighbours& nb, uint32_t i, Rec* buf, Stage&& stage,
+                                                 Pair&& pair) {
+    static_assert(S % 64 == 0, "a round is whole wavefront loads");
+    constexpr int R = S / 64;
+    constexpr uint32_t kPast = 0xffffffffu;  // an entry position beyond every round
+    const uint32_t w = i >> 6, lane = i & 63u;
+    const uint32_t total = __builtin_amdgcn_readfirstlane(min(nb.unionLen[w], nb.unionCap));
+    const uint32_t bankCap = nb.unionCap / kBanks;
+    const uint2 counts = nb.count[i];
+    const uint32_t perBank = bankChunks(nb);
+    const uint32_t lastChunk = perBank - 1u;
+#pragma unroll 1
+    for (uint32_t bank = 0; bank < kBanks; ++bank) {
+        const uint32_t len = (total + (kBanks - 1u) - bank) / kBanks;
+        if (len == 0u) break;  // banks fill in turn: the later ones are empty too
+        const uint32_t* U = nb.unionIds + (uint64_t)w * nb.unionCap + bank * bankCap;
+        const uint32_t cnt = bankCount(counts, bank);
+        // chunks are loaded one ahead, unconditionally (clamped to the bank's last chunk: entries
+        // past cnt are never used), so that no load result has to be merged with an older value
+        const uint4* ch = tableChunks(nb, i) + 64ull * (bank * perBank);
+        uint4 cur = ch[0];
+        uint4 nxt = ch[64ull * min(1u, lastChunk)];
+        uint32_t k = 0;
+        uint32_t e = cnt > 0u ? (cur.x & 0xffffu) : kPast;  // position of entry k; entries ascend, so e >= s0
+        // moves to entry k + 1 and returns its position (kPast after the last)
+        const auto advance = [&]() -> uint32_t {
+            ++k;
+            if ((k & 7u) == 0u) {
+                cur = nxt;
+                nxt = ch[64ull * min((k >> 3) + 1u, lastChunk)];
+            } else if ((k & 1u) == 0u) {
+                cur.x = cur.y;
+                cur.y = cur.z;
+                cur.z = cur.w;
+            }
+            return k < cnt ? ((k & 1u) ? (cur.x >> 16) : (cur.x & 0xffffu)) : kPast;
+        };
+        for (uint32_t s0 = 0; s0 < len; s0 += S) {
+            const uint32_t s1 = min(s0 + (uint32_t)S, len);
+            uint32_t id[R];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (k + u < cnt) load(js[u], rec[u]);
+            for (int r = 0; r < R; ++r) id[r] = U[min(s0 + (uint32_t)(64 * r) + lane, len - 1u)];
+            waveSync();
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (k + u < cnt) pair(rec[u]);
+            for (int r = 0; r < R; ++r) stage(id[r], buf[64 * r + lane]);
+            waveSync();
+            // two records alternate (a holds entry k when the loop body starts), so that a record
+            // read ahead from LDS never has to be copied into the other's registers
+            Rec a = buf[min(e - s0, (uint32_t)S - 1u)];
+            while (e < s1) {
+                uint32_t en = advance();
+                const Rec b = buf[min(en - s0, (uint32_t)S - 1u)];
+                pair(a);
+                e = en;
+                if (e >= s1) break;
+                en = advance();
+                a = buf[min(en - s0, (uint32_t)S - 1u)];
+                pair(b);
+                e = en;
+            }
+        }
     }
+    waveSync();
 }
 
 struct Segments {
@@ -132,7 +208,8 @@ struct SegScalars {
 
 struct Counters {
     unsigned long long* v;  // [8]: 0 interactions, 1 stack overflows (gravity), 3 nbr overflow flag,
-                            // 6 nbr stack overflow, 7 largest per-slot entry count wanted (sticky max)
+                            // 5 largest union a wavefront wanted (sticky max), 6 nbr stack overflow,
+                            // 7 table width wanted: kBanks x the fullest bank of a slot (sticky max)
 };
 
 struct Params {

@@ -167,46 +167,49 @@ __global__ __launch_bounds__(kBlock) void kDtCandidates(DeviceCtx c) {
 
 template <bool FP64>
 __global__ __launch_bounds__(kBlock) void kDrift(DeviceCtx c) {
+    struct Rec {
+        float4 p, v;
+        float h;
+    };
+    constexpr int kStage = 256;
     __shared__ float sb[kWavesPerBlock][6];
+    __shared__ Rec stageBuf[kWavesPerBlock][kStage];
     const uint32_t b = xcdBlock(blockIdx.x, gridDim.x);
     const uint32_t i = b * kBlock + threadIdx.x;
     const int lane = laneId(), wave = threadIdx.x >> 6;
     const uint32_t sg = c.seg.slotSeg[i];
     const bool active = i - c.seg.off[sg] < c.seg.n[sg];
     float4 pn = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    // every lane walks (inactive slots have no entries), so that the wave stages its union together
+    const float4 pi = c.cur.pos[i];
+    const float4 vi = c.cur.vel[i];
+    const float hi = fmaxf(c.cur.hu[i].x, 1.0e-6f), Hi = 2.0f * hi;
+    const float rhoI = fmaxf(vi.w, 1.0e-9f);
+    float xs = 0.0f, ys = 0.0f, zs = 0.0f;
+    const Support sI = supportOf(Hi);
+    forEachNeighbour<kStage>(
+        c.nb, i, stageBuf[wave],
+        [&](uint32_t j, Rec& r) {
+            r.p = c.cur.pos[j];
+            r.v = c.cur.vel[j];
+            r.h = c.d.nb[j].x;
+        },
+        [&](const Rec& n) {
+            const float4 pj = n.p, vj = n.v;
+            const float Hj = 2.0f * fmaxf(n.h, 1.0e-6f);
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float r2 = dist2f(dx, dy, dz);
+            const float Hm = fmaxf(Hi, Hj);
+            if (!(r2 < Hm * Hm)) return;
+            const float r = fastSqrt(r2);
+            const float wb = 0.5f * (wendlandS(r, sI) + wendlandS(r, supportOf(Hj)));
+            const float rb = 0.5f * (rhoI + fmaxf(vj.w, 1.0e-9f));
+            const float co = (pj.w * fastRcp(rb)) * wb;
+            xs += co * (vj.x - vi.x);
+            ys += co * (vj.y - vi.y);
+            zs += co * (vj.z - vi.z);
+        });
     if (active) {
-        const float4 pi = c.cur.pos[i];
-        const float4 vi = c.cur.vel[i];
-        const float hi = fmaxf(c.cur.hu[i].x, 1.0e-6f), Hi = 2.0f * hi;
-        const float rhoI = fmaxf(vi.w, 1.0e-9f);
-        float xs = 0.0f, ys = 0.0f, zs = 0.0f;
-        const Support sI = supportOf(Hi);
-        struct Rec {
-            float4 p, v;
-            float h;
-        };
-        forEachNeighbour<Rec>(
-            c.nb, i,
-            [&](uint32_t j, Rec& r) {
-                r.p = c.cur.pos[j];
-                r.v = c.cur.vel[j];
-                r.h = c.d.nb[j].x;
-            },
-            [&](const Rec& n) {
-                const float4 pj = n.p, vj = n.v;
-                const float Hj = 2.0f * fmaxf(n.h, 1.0e-6f);
-                const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                const float r2 = dist2f(dx, dy, dz);
-                const float Hm = fmaxf(Hi, Hj);
-                if (!(r2 < Hm * Hm)) return;
-                const float r = fastSqrt(r2);
-                const float wb = 0.5f * (wendlandS(r, sI) + wendlandS(r, supportOf(Hj)));
-                const float rb = 0.5f * (rhoI + fmaxf(vj.w, 1.0e-9f));
-                const float co = (pj.w * fastRcp(rb)) * wb;
-                xs += co * (vj.x - vi.x);
-                ys += co * (vj.y - vi.y);
-                zs += co * (vj.z - vi.z);
-            });
         const float dt = c.ss.dt[sg];
         if (FP64) {
             const size_t cap = (size_t)c.stride64;

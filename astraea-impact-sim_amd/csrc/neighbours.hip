@@ -13,9 +13,12 @@
 // sets are exactly the reference's.  Each lane then loops over its own list: the pair
 // physics runs only for real pairs.
 //
-// Layout (tableChunks): slot i owns `stride` entries, in 16-byte chunks of four.  The table is
-// written in a single walk, a chunk at a time; a slot that wants more than `stride` entries keeps
-// the first `stride`, and the overflow flag and the largest count wanted are left for the host.
+// Layout (tableChunks): each wavefront writes the union of its particles' neighbour sets (the hit
+// candidates, in the order the walk found them, dealt out to kBanks banks) and, for every
+// particle and bank, the ascending 16-bit positions of its neighbours in that bank, one 16-bit
+// store each (the table's lines fill in L2).  A bank of a slot that wants more than stride / kBanks entries keeps the
+// first ones; a union longer than `unionCap` is cut; the overflow flag and the sizes wanted are
+// left for the host, which widens the table.
 #include "launch.hpp"
 
 namespace astraea_hip {
@@ -29,7 +32,6 @@ struct WaveLds {
     int stack[kSCap];
     float4 cand[kCCap];  // candidate particle (x, y, z, squared support bound)
     int candId[kCCap];
-    uint4 chunk[kWave];  // each lane's table chunk being filled
 };
 
 // Wide walk: the wave expands up to 64 stack nodes per round, one per lane.  A node must be opened
@@ -45,7 +47,8 @@ struct WaveLds {
 // that loaded it.  Leaf children are loaded by their lanes in parallel, kept when within reach of
 // the group's box, and staged in LDS, where every lane tests its own particle against them:
 // 32 candidates into a hit mask, then the lane appends its hits in candidate order, so the
-// bookkeeping of an entry runs once per hit rather than once per candidate.
+// bookkeeping of an entry runs once per hit rather than once per candidate.  The candidates hit by
+// any lane form the wave's union list; a lane's entry is the hit's position in it.
 __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
     __shared__ WaveLds lds[kWavesPerBlock];
     const uint32_t b = xcdBlock(blockIdx.x, gridDim.x);
@@ -55,13 +58,17 @@ __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
     const uint32_t sg = uniformU32(c.seg.slotSeg[b * kBlock]);
     const uint32_t off = uniformU32(c.seg.off[sg]), n = uniformU32(c.seg.n[sg]);
     const bool active = i - off < n;
-    const uint32_t K = c.nb.stride;
+    const uint32_t K = c.nb.stride, Kb = K / kBanks, perBank = bankChunks(c.nb);
     uint4* out = tableChunks(c.nb, i);
-    uint32_t cnt = 0;
+    // entries per bank: bank b in bits 16 b .. 16 b + 15
+    uint64_t cnt4 = 0;
+    uint32_t ulen = 0;
     bool stackFull = false;
     WaveLds& L = lds[wave];
-    uint32_t* chunk = reinterpret_cast<uint32_t*>(&L.chunk[lane]);
-    if (ballot(active) != 0ull && n >= 2u) {
+    const uint32_t UC = c.nb.unionCap, UCb = UC / kBanks;
+    uint32_t* U = c.nb.unionIds + (uint64_t)gw * UC;
+    const uint64_t actMask = ballot(active);
+    if (actMask != 0ull && n >= 2u) {
         float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float Hb = 0.0f;
         if (active) {
@@ -174,45 +181,72 @@ __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
                 __builtin_amdgcn_wave_barrier();
             }
             // candidates: every lane tests its own particle (r^2 < max(Hb_i, Hb_j)^2), 32 at a time;
-            // slots past nl hold stale records and are masked off
+            // slots past nl hold stale records and are masked off; the candidates hit by any
+            // lane (a wave-uniform mask from the ballots) join the union, in candidate order
             for (int c0 = 0; c0 < nl; c0 += 32) {
-                uint32_t m = 0u;
+                uint32_t m = 0u, any = 0u;
 #pragma unroll
                 for (int u = 0; u < 32; ++u) {
                     const float4 q = L.cand[c0 + u];
                     const float r2 = dist2f(pi.x - q.x, pi.y - q.y, pi.z - q.z);
-                    m |= (r2 < Hb2 || r2 < q.w) ? (1u << u) : 0u;
+                    const bool hit = r2 < Hb2 || r2 < q.w;
+                    m |= hit ? (1u << u) : 0u;
+                    any |= (ballot(hit) & actMask) != 0ull ? (1u << u) : 0u;
                 }
                 const int left = nl - c0;
-                if (left < 32) m &= (1u << left) - 1u;
+                if (left < 32) {
+                    m &= (1u << left) - 1u;
+                    any &= (1u << left) - 1u;
+                }
                 if (!active) m = 0u;
+                if (lane < 32 && ((any >> lane) & 1u)) {
+                    const uint32_t q = ulen + (uint32_t)__popc(any & ((1u << lane) - 1u));
+                    if (q < UC) U[(q % kBanks) * UCb + q / kBanks] = (uint32_t)L.candId[c0 + lane];
+                }
                 while (m != 0u) {
                     const int u = __builtin_ctz(m);
                     m &= m - 1u;
                     const uint32_t jj = (uint32_t)L.candId[c0 + u];
                     if (jj == i) continue;
-                    chunk[cnt & 3u] = jj;
-                    if ((cnt & 3u) == 3u && cnt < K) out[64ull * (cnt >> 2)] = L.chunk[lane];
-                    ++cnt;
+                    const uint32_t q = ulen + (uint32_t)__popc(any & ((1u << u) - 1u));
+                    if (q >= UC) continue;  // the union overflowed: the host widens the table
+                    const uint32_t bk = q % kBanks, sh = bk << 4;
+                    const uint32_t cb = (uint32_t)(cnt4 >> sh) & 0xffffu;
+                    if (cb < Kb)
+                        reinterpret_cast<uint16_t*>(out + 64ull * (bk * perBank + (cb >> 3)))[cb & 7u] =
+                            (uint16_t)(q / kBanks);
+                    cnt4 += 1ull << sh;
                 }
+                ulen += (uint32_t)__popc(any);
             }
             nl = 0;
             __builtin_amdgcn_wave_barrier();
             if (sp == 0) break;
         }
     }
-    if ((cnt & 3u) != 0u && cnt < K) out[64ull * (cnt >> 2)] = L.chunk[lane];
-    c.nb.count[i] = cnt < K ? cnt : K;
+    // the stored counts are capped at the bank's capacity
+    uint32_t cnt = 0, maxBank = 0, stored[kBanks];
+#pragma unroll
+    for (uint32_t bk = 0; bk < kBanks; ++bk) {
+        const uint32_t cb = (uint32_t)(cnt4 >> (16u * bk)) & 0xffffu;
+        cnt += cb;
+        maxBank = max(maxBank, cb);
+        stored[bk] = min(cb, Kb);
+    }
+    c.nb.count[i] = make_uint2(stored[0] | (stored[1] << 16), stored[2] | (stored[3] << 16));
+    if (lane == 0) c.nb.unionLen[gw] = ulen;
     uint32_t sum = cnt;
     for (int o = 32; o > 0; o >>= 1) sum += (uint32_t)__shfl_xor((int)sum, o, 64);
     const uint32_t mxc = waveMaxU(cnt);
+    const uint32_t want = waveMaxU(maxBank) * kBanks;  // the stride the fullest bank needs
     const bool full = ballot(stackFull) != 0ull;
     if (lane == 0) {
         c.nb.waveSum[gw] = sum;
         c.nb.waveMax[gw] = mxc;
-        if (mxc > K) {
+        if (want > K || ulen > UC) {
             atomicOr(&c.cnt.v[3], 1ull);
-            atomicMax(&c.cnt.v[7], (unsigned long long)mxc);
+            atomicMax(&c.cnt.v[7], (unsigned long long)want);
+            atomicMax(&c.cnt.v[5], (unsigned long long)ulen);
         }
         if (full) atomicOr(&c.cnt.v[6], 1ull);
     }

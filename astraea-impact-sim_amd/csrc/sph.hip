@@ -41,12 +41,20 @@ __device__ __forceinline__ float maxPrincipalStress(float p, float sxx, float sy
     return -p + 2.0f * q * cosf(acosf(r) * (1.0f / 3.0f));
 }
 
+// Union records staged per round by the neighbour passes (one LDS buffer per wavefront).
+constexpr int kStage1 = 256, kStage2 = 128;
+
 template <bool STRENGTH>
 __global__ __launch_bounds__(kBlock) void kPass1(DeviceCtx c) {
+    struct Rec {
+        float4 p, v;
+    };
+    __shared__ Rec stageBuf[kWavesPerBlock][kStage1];
     const uint32_t b = xcdBlock(blockIdx.x, gridDim.x);
     const uint32_t i = b * kBlock + threadIdx.x;
     const uint32_t sg = c.seg.slotSeg[i];
-    if (i - c.seg.off[sg] >= c.seg.n[sg]) return;
+    const bool active = i - c.seg.off[sg] < c.seg.n[sg];
+    if (ballot(active) == 0ull) return;  // a wave of padding slots
 
     const float4 pi = c.cur.pos[i];
     const float4 vi = c.cur.vel[i];
@@ -62,11 +70,8 @@ __global__ __launch_bounds__(kBlock) void kPass1(DeviceCtx c) {
     float gxx = 0.0f, gxy = 0.0f, gxz = 0.0f, gyx = 0.0f, gyy = 0.0f, gyz = 0.0f, gzx = 0.0f, gzy = 0.0f, gzz = 0.0f;
 
     const Support sI = supportOf(Hi);
-    struct Rec {
-        float4 p, v;
-    };
-    forEachNeighbour<Rec>(
-        c.nb, i,
+    forEachNeighbour<kStage1>(
+        c.nb, i, stageBuf[threadIdx.x >> 6],
         [&](uint32_t j, Rec& r) {
             r.p = c.cur.pos[j];
             r.v = c.cur.vel[j];
@@ -92,6 +97,7 @@ __global__ __launch_bounds__(kBlock) void kPass1(DeviceCtx c) {
             gyx += dvy * gx; gyy += dvy * gy; gyz += dvy * gz;
             gzx += dvz * gx; gzy += dvz * gy; gzz += dvz * gz;
         });
+    if (!active) return;
 
     const float normNs = sqrtf(nsx * nsx + nsy * nsy + nsz * nsz);
     float fs = fminf(1.0f, fmaxf(0.0f, 1.6f * fmaxf(1.0f - shep, normNs / hi) - 0.15f));
@@ -157,10 +163,17 @@ __global__ __launch_bounds__(kBlock) void kEos(DeviceCtx c) {
 
 template <bool STRENGTH>
 __global__ __launch_bounds__(kBlock) void kPass2(DeviceCtx c) {
+    struct Rec {
+        float4 p, n, v, s0;
+        float2 s1;
+        uint32_t mat;
+    };
+    __shared__ Rec stageBuf[kWavesPerBlock][kStage2];
     const uint32_t b = xcdBlock(blockIdx.x, gridDim.x);
     const uint32_t i = b * kBlock + threadIdx.x;
     const uint32_t sg = c.seg.slotSeg[i];
-    if (i - c.seg.off[sg] >= c.seg.n[sg]) return;
+    const bool active = i - c.seg.off[sg] < c.seg.n[sg];
+    if (ballot(active) == 0ull) return;  // a wave of padding slots
 
     const float4 pi = c.cur.pos[i];
     const float4 vi = c.cur.vel[i];
@@ -193,13 +206,8 @@ __global__ __launch_bounds__(kBlock) void kPass2(DeviceCtx c) {
 
     const Support sI = supportOf(Hi);
     const float dampI = 0.01f * Hi * Hi;
-    struct Rec {
-        float4 p, n, v, s0;
-        float2 s1;
-        uint32_t mat;
-    };
-    forEachNeighbour<Rec>(
-        c.nb, i,
+    forEachNeighbour<kStage2>(
+        c.nb, i, stageBuf[threadIdx.x >> 6],
         [&](uint32_t j, Rec& r) {
             r.p = c.cur.pos[j];
             r.n = c.d.nb[j];
@@ -274,6 +282,7 @@ __global__ __launch_bounds__(kBlock) void kPass2(DeviceCtx c) {
                 gzx += volJ * wz * gx; gzy += volJ * wz * gy; gzz += volJ * wz * gz;
             }
         });
+    if (!active) return;
 
     const float4 g = c.d.grav[i];
     const float4 p1 = c.d.p1a[i];

@@ -280,9 +280,13 @@ int astraea_relax(astraea_ctx* ctx, uint32_t steps, const float* anchor_temp_k,
 int astraea_counters(astraea_ctx* ctx, uint64_t* out, int n);
 
 /* Width of the neighbour table, in entries per particle, rounded up to a multiple
- * of 4 (no counterpart in the reference, which walks the tree in every pass).  The
- * table takes 4 * stride * capacity bytes.  A particle with more candidates than this makes
- * astraea_synchronize report ASTRAEA_E_OVERFLOW for the step and widen the table. */
+ * of 32 (no counterpart in the reference, which walks the tree in every pass).  Each
+ * wavefront of 64 particles keeps the union of its particles' candidates (up to
+ * min(32 * stride, 65536) ids, 4 bytes each, in four banks) and each particle 16-bit
+ * positions into them, stride / 4 per bank, so the table takes 4 * stride * capacity
+ * bytes.  A particle with more candidates in a bank than stride / 4, or a wavefront
+ * with more than its union holds, makes astraea_synchronize report ASTRAEA_E_OVERFLOW
+ * for the step and widen the table. */
 #define ASTRAEA_DEFAULT_NEIGHBOUR_STRIDE 320
 int astraea_set_neighbour_stride(astraea_ctx* ctx, uint32_t stride);
 

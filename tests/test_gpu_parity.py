@@ -513,17 +513,18 @@ def test_neighbour_table_overflow_in_steps_is_reported(lib):
     ctx = lib.Context(n)
     ctx.set_params(S.plain_params())
     ctx.set_layout([n])
-    ctx.set_neighbour_stride(64)
+    ctx.set_neighbour_stride(32)
     ctx.load_state(st)
-    ctx.initial_forces()
+    ctx.initial_forces()  # widens the table to what the loaded state needs, with a quarter to spare
     ctx.synchronize()
-    assert ctx.counters()["nbr_stride"] == 64
+    width = ctx.counters()["nbr_stride"]
+    assert ctx.counters()["nbr_max"] <= width
     with pytest.raises(lib.AstraeaError) as e:
         for _ in range(30):
             ctx.step(MAX_DT, 1, graph=False)
             ctx.synchronize()
     assert e.value.code == -5 and "neighbour table overflow" in str(e.value)
-    assert ctx.counters()["nbr_stride"] > 64
+    assert ctx.counters()["nbr_stride"] > width
     with pytest.raises(lib.AstraeaError):
         ctx.step(MAX_DT, 1)   # the table was rebuilt empty: forces must be evaluated again
     ctx.initial_forces()
