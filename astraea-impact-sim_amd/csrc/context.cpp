@@ -505,7 +505,8 @@ void astraea_default_params(astraea_params* p) {
 int astraea_create(astraea_ctx** out, int device, uint32_t capacity, uint32_t maxSegments) {
     if (!out) return fail(ASTRAEA_E_INVALID, "null output pointer");
     *out = nullptr;
-    if (capacity == 0) return fail(ASTRAEA_E_INVALID, "capacity must be > 0");
+    if (capacity == 0 || capacity > ASTRAEA_MAX_CAPACITY)
+        return fail(ASTRAEA_E_INVALID, "capacity must be in [1, %u]", (unsigned)ASTRAEA_MAX_CAPACITY);
     if (maxSegments == 0 || maxSegments > 256) return fail(ASTRAEA_E_INVALID, "max_segments must be in [1, 256]");
     int ndev = 0;
     HIP_TRY(hipGetDeviceCount(&ndev));

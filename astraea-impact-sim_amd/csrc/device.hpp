@@ -89,9 +89,10 @@ struct Neighbours {
 // Table layout.  Each wavefront w owns a union list: the slots that are a neighbour of at least
 // one of its particles (or one of them).  The q-th slot the walk found goes to bank q % kBanks at
 // position q / kBanks; bank b of w's union starts at unionIds + w unionCap + b unionCap / kBanks.
-// Dealing the union out to the banks like cards gives every particle about the same number of
-// entries in each bank, so the rounds of a pass (one bank each, staged in LDS) keep the lanes
-// equally busy; in walk order a particle's neighbours would crowd into a few rounds.
+// A particle's neighbours come in runs of slots the walk found one after another, so dealing the
+// union out to the banks like cards gives every particle about the same number of entries in
+// each bank, and the rounds of a pass (a bank at a time, staged in LDS) keep the lanes equally
+// busy; in walk order a particle's neighbours would crowd into a few rounds.
 // An entry is a 16-bit position in its bank, and a particle's entries of one bank ascend.  Each
 // slot's `stride` entries are kBanks lists of stride / kBanks; entry k of bank b of slot
 // i = 64 w + l is half-word (k & 7) of chunk 64 (b stride / (8 kBanks) + (k >> 3)) + l of the

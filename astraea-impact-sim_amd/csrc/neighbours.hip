@@ -27,11 +27,17 @@ namespace {
 constexpr int kSCap = 1024;                  // per-wave node stack
 constexpr int kCCap = 256;                   // per-wave candidate records
 constexpr int kFlushAt = kCCap - 2 * kWave;  // one round lists at most 128 leaves
+constexpr int kBucket = 8;                   // an open node with this many leaves or fewer is listed whole
+constexpr int kBCap = 256;                   // per-wave listed buckets
+constexpr int kBucketFlushAt = kBCap - 2 * kWave;
+constexpr uint32_t kBucketSlotMask = (1u << 29) - 1u;  // a bucket: first leaf slot | (leaves - 1) << 29
+static_assert(kBucket <= 8 && kWave % kBucket == 0, "a bucket's leaf count fits three bits");
 
 struct WaveLds {
     int stack[kSCap];
     float4 cand[kCCap];  // candidate particle (x, y, z, squared support bound)
     int candId[kCCap];
+    uint32_t bucket[kBCap];
 };
 
 // Wide walk: the wave expands up to 64 stack nodes per round, one per lane.  A node must be opened
@@ -105,84 +111,35 @@ __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
             }
         }
         const uint64_t lt = (1ull << lane) - 1ull;
-        int sp = 1, nl = 0;
+        int sp = 1, nl = 0, nr = 0;
         L.stack[0] = (int)off;
         __builtin_amdgcn_wave_barrier();
-        for (;;) {
-            while (sp > 0 && nl <= kFlushAt) {
-                const int k = sp < kWave ? sp : kWave;
-                sp -= k;
-                const bool has = lane < k;
-                int2 ch = make_int2(0, 0);
-                if (has) ch = c.t.child[L.stack[sp + lane]];
-                __builtin_amdgcn_wave_barrier();
-#pragma unroll
-                for (int side = 0; side < 2; ++side) {
-                    const int cc = side == 0 ? ch.x : ch.y;
-                    const bool isInt = has && cc >= 0;
-                    bool isLeaf = has && cc < 0;
-                    float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f), mn = q, mx = q;
-                    bool open = false, undecided = false;
-                    if (isInt) {
-                        mn = c.t.bmin[cc];
-                        mx = c.t.bmax[cc];
-#pragma unroll
-                        for (int g = 0; g < 4; ++g) {
-                            const float reach = fmaxf(sH[g], mx.w);
-                            const float gx = fmaxf(0.0f, fmaxf(mn.x - sh[g][0], sl[g][0] - mx.x));
-                            const float gy = fmaxf(0.0f, fmaxf(mn.y - sh[g][1], sl[g][1] - mx.y));
-                            const float gz = fmaxf(0.0f, fmaxf(mn.z - sh[g][2], sl[g][2] - mx.z));
-                            const bool hit = dist2f(gx, gy, gz) <= reach * reach;
-                            if (compact[g]) open = open || hit;
-                            else undecided = undecided || hit;
-                        }
-                        undecided = undecided && !open;
-                    } else if (isLeaf) {
-                        const uint32_t j = (uint32_t)~cc;
-                        const float4 pj = c.cur.pos[j];
-                        const float4 hj = c.cur.hu[j];
-                        const float hbj = supportBound(hj.x, hj.y);
-                        q = make_float4(pj.x, pj.y, pj.z, hbj * hbj);  // max(a, b)^2 = max(a^2, b^2) in FP32 too
-                        const float reach = fmaxf(HbMax, hbj);
-                        const float gx = fmaxf(0.0f, fmaxf(lx - q.x, q.x - hx));
-                        const float gy = fmaxf(0.0f, fmaxf(ly - q.y, q.y - hy));
-                        const float gz = fmaxf(0.0f, fmaxf(lz - q.z, q.z - hz));
-                        isLeaf = dist2f(gx, gy, gz) < reach * reach;
-                    }
-                    // exact test of the nodes only straddling sub-groups reach: lane = particle
-                    uint64_t um = ballot(undecided);
-                    while (um != 0ull) {
-                        const int src = (int)__builtin_ctzll(um);
-                        um &= um - 1ull;
-                        const float4 bmn = make_float4(readLaneF(mn.x, src), readLaneF(mn.y, src), readLaneF(mn.z, src), 0.0f);
-                        const float4 bmx = make_float4(readLaneF(mx.x, src), readLaneF(mx.y, src), readLaneF(mx.z, src), 0.0f);
-                        const float rr = fmaxf(Hb, readLaneF(mx.w, src));
-                        const bool hit = active && boxDist2f(pi.x, pi.y, pi.z, bmn, bmx) <= rr * rr;
-                        const bool any = ballot(hit) != 0ull;
-                        if (lane == src) open = any;
-                    }
-                    const uint64_t mPush = ballot(open);
-                    const int pushAt = sp + __popcll(mPush & lt);
-                    if (open && pushAt >= kSCap) {
-                        open = false;
-                        stackFull = true;
-                    }
-                    if (open) L.stack[pushAt] = cc;
-                    const int nPush = __popcll(mPush);
-                    sp = sp + nPush < kSCap ? sp + nPush : kSCap;
-                    const uint64_t mList = ballot(isLeaf);
-                    if (isLeaf) {
-                        const int at = nl + __popcll(mList & lt);
-                        L.cand[at] = q;
-                        L.candId[at] = ~cc;
-                    }
-                    nl += __popcll(mList);
-                }
-                __builtin_amdgcn_wave_barrier();
+        // a leaf j as a candidate record, kept when within reach of the group's box
+        const auto leafCandidate = [&](uint32_t j, float4& q) -> bool {
+            const float4 pj = c.cur.pos[j];
+            const float4 hj = c.cur.hu[j];
+            const float hbj = supportBound(hj.x, hj.y);
+            q = make_float4(pj.x, pj.y, pj.z, hbj * hbj);  // max(a, b)^2 = max(a^2, b^2) in FP32 too
+            const float reach = fmaxf(HbMax, hbj);
+            const float gx = fmaxf(0.0f, fmaxf(lx - q.x, q.x - hx));
+            const float gy = fmaxf(0.0f, fmaxf(ly - q.y, q.y - hy));
+            const float gz = fmaxf(0.0f, fmaxf(lz - q.z, q.z - hz));
+            return dist2f(gx, gy, gz) < reach * reach;
+        };
+        const auto listLeaves = [&](bool keep, const float4& q, uint32_t j) {
+            const uint64_t mList = ballot(keep);
+            if (keep) {
+                const int at = nl + __popcll(mList & lt);
+                L.cand[at] = q;
+                L.candId[at] = (int)j;
             }
-            // candidates: every lane tests its own particle (r^2 < max(Hb_i, Hb_j)^2), 32 at a time;
-            // slots past nl hold stale records and are masked off; the candidates hit by any
-            // lane (a wave-uniform mask from the ballots) join the union, in candidate order
+            nl += __popcll(mList);
+        };
+        // candidates: every lane tests its own particle (r^2 < max(Hb_i, Hb_j)^2), 32 at a time;
+        // slots past nl hold stale records and are masked off; the candidates hit by any
+        // lane (a wave-uniform mask from the ballots) join the union, in candidate order
+        const auto testCandidates = [&]() {
+            __builtin_amdgcn_wave_barrier();
             for (int c0 = 0; c0 < nl; c0 += 32) {
                 uint32_t m = 0u, any = 0u;
 #pragma unroll
@@ -221,6 +178,86 @@ __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
             }
             nl = 0;
             __builtin_amdgcn_wave_barrier();
+        };
+        for (;;) {
+            while (sp > 0 && nl <= kFlushAt && nr <= kBucketFlushAt) {
+                const int k = sp < kWave ? sp : kWave;
+                sp -= k;
+                const bool has = lane < k;
+                int2 ch = make_int2(0, 0);
+                if (has) ch = c.t.child[L.stack[sp + lane]];
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int side = 0; side < 2; ++side) {
+                    const int cc = side == 0 ? ch.x : ch.y;
+                    const bool isInt = has && cc >= 0;
+                    bool isLeaf = has && cc < 0;
+                    float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f), mn = q, mx = q;
+                    bool open = false, undecided = false;
+                    int far = 0;
+                    if (isInt) {
+                        mn = c.t.bmin[cc];
+                        mx = c.t.bmax[cc];
+                        far = c.t.other[cc];
+#pragma unroll
+                        for (int g = 0; g < 4; ++g) {
+                            const float reach = fmaxf(sH[g], mx.w);
+                            const float gx = fmaxf(0.0f, fmaxf(mn.x - sh[g][0], sl[g][0] - mx.x));
+                            const float gy = fmaxf(0.0f, fmaxf(mn.y - sh[g][1], sl[g][1] - mx.y));
+                            const float gz = fmaxf(0.0f, fmaxf(mn.z - sh[g][2], sl[g][2] - mx.z));
+                            const bool hit = dist2f(gx, gy, gz) <= reach * reach;
+                            if (compact[g]) open = open || hit;
+                            else undecided = undecided || hit;
+                        }
+                        undecided = undecided && !open;
+                    } else if (isLeaf) {
+                        isLeaf = leafCandidate((uint32_t)~cc, q);
+                    }
+                    // exact test of the nodes only straddling sub-groups reach: lane = particle
+                    uint64_t um = ballot(undecided);
+                    while (um != 0ull) {
+                        const int src = (int)__builtin_ctzll(um);
+                        um &= um - 1ull;
+                        const float4 bmn = make_float4(readLaneF(mn.x, src), readLaneF(mn.y, src), readLaneF(mn.z, src), 0.0f);
+                        const float4 bmx = make_float4(readLaneF(mx.x, src), readLaneF(mx.y, src), readLaneF(mx.z, src), 0.0f);
+                        const float rr = fmaxf(Hb, readLaneF(mx.w, src));
+                        const bool hit = active && boxDist2f(pi.x, pi.y, pi.z, bmn, bmx) <= rr * rr;
+                        const bool any = ballot(hit) != 0ull;
+                        if (lane == src) open = any;
+                    }
+                    // an open node of at most kBucket leaves is listed whole instead of walked
+                    const int span = far > cc ? far - cc : cc - far;  // leaves - 1
+                    const bool bucket = open && span < kBucket;
+                    open = open && !bucket;
+                    const uint64_t mBucket = ballot(bucket);
+                    if (bucket) L.bucket[nr + __popcll(mBucket & lt)] = (uint32_t)min(cc, far) | ((uint32_t)span << 29);
+                    nr += __popcll(mBucket);
+                    const uint64_t mPush = ballot(open);
+                    const int pushAt = sp + __popcll(mPush & lt);
+                    if (open && pushAt >= kSCap) {
+                        open = false;
+                        stackFull = true;
+                    }
+                    if (open) L.stack[pushAt] = cc;
+                    const int nPush = __popcll(mPush);
+                    sp = sp + nPush < kSCap ? sp + nPush : kSCap;
+                    listLeaves(isLeaf, q, (uint32_t)~cc);
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+            // the listed buckets, kWave / kBucket at a time: lane 8 r + e takes leaf e of bucket r
+            for (int r0 = 0; r0 < nr; r0 += kWave / kBucket) {
+                if (nl > kCCap - kWave) testCandidates();
+                const int r = r0 + lane / kBucket, e = lane % kBucket;
+                const uint32_t rec = r < nr ? L.bucket[r] : 0u;
+                const uint32_t j = (rec & kBucketSlotMask) + (uint32_t)e;
+                float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                bool keep = r < nr && (uint32_t)e <= (rec >> 29);
+                if (keep) keep = leafCandidate(j, q);
+                listLeaves(keep, q, j);
+            }
+            nr = 0;
+            if (nl > kFlushAt || sp == 0) testCandidates();
             if (sp == 0) break;
         }
     }

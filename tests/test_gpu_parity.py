@@ -508,7 +508,7 @@ def test_neighbour_table_overflow_in_steps_is_reported(lib):
     from astraea_amd import scenario as S
 
     st = S.config_small()
-    st["h"] = (st["h"] * 0.6).astype(np.float32)
+    st["h"] = (st["h"] * 0.35).astype(np.float32)
     n = len(st["mass"])
     ctx = lib.Context(n)
     ctx.set_params(S.plain_params())
