@@ -27,11 +27,12 @@ namespace {
 constexpr int kSCap = 1024;                  // per-wave node stack
 constexpr int kCCap = 256;                   // per-wave candidate records
 constexpr int kFlushAt = kCCap - 2 * kWave;  // one round lists at most 128 leaves
-constexpr int kBucket = 8;                   // an open node with this many leaves or fewer is listed whole
+constexpr int kBucket = 16;                  // an open node with this many leaves or fewer is listed whole
 constexpr int kBCap = 256;                   // per-wave listed buckets
 constexpr int kBucketFlushAt = kBCap - 2 * kWave;
-constexpr uint32_t kBucketSlotMask = (1u << 29) - 1u;  // a bucket: first leaf slot | (leaves - 1) << 29
-static_assert(kBucket <= 8 && kWave % kBucket == 0, "a bucket's leaf count fits three bits");
+constexpr int kSlotBits = 28;                // a bucket: first leaf slot | (leaves - 1) << kSlotBits
+constexpr uint32_t kBucketSlotMask = (1u << kSlotBits) - 1u;
+static_assert(kBucket <= (1 << (32 - kSlotBits)) && kWave % kBucket == 0, "a bucket's leaf count fits the top bits");
 
 struct WaveLds {
     int stack[kSCap];
@@ -230,7 +231,7 @@ __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
                     const bool bucket = open && span < kBucket;
                     open = open && !bucket;
                     const uint64_t mBucket = ballot(bucket);
-                    if (bucket) L.bucket[nr + __popcll(mBucket & lt)] = (uint32_t)min(cc, far) | ((uint32_t)span << 29);
+                    if (bucket) L.bucket[nr + __popcll(mBucket & lt)] = (uint32_t)min(cc, far) | ((uint32_t)span << kSlotBits);
                     nr += __popcll(mBucket);
                     const uint64_t mPush = ballot(open);
                     const int pushAt = sp + __popcll(mPush & lt);
@@ -245,14 +246,14 @@ __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
                 }
                 __builtin_amdgcn_wave_barrier();
             }
-            // the listed buckets, kWave / kBucket at a time: lane 8 r + e takes leaf e of bucket r
+            // the listed buckets, kWave / kBucket at a time: lane kBucket r + e takes leaf e of bucket r
             for (int r0 = 0; r0 < nr; r0 += kWave / kBucket) {
                 if (nl > kCCap - kWave) testCandidates();
                 const int r = r0 + lane / kBucket, e = lane % kBucket;
                 const uint32_t rec = r < nr ? L.bucket[r] : 0u;
                 const uint32_t j = (rec & kBucketSlotMask) + (uint32_t)e;
                 float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                bool keep = r < nr && (uint32_t)e <= (rec >> 29);
+                bool keep = r < nr && (uint32_t)e <= (rec >> kSlotBits);
                 if (keep) keep = leafCandidate(j, q);
                 listLeaves(keep, q, j);
             }

@@ -108,9 +108,9 @@ typedef struct astraea_ctx astraea_ctx;
 /* Allocate a context on `device` for up to `capacity` particles in up to
  * `max_segments` segments.  Replaces allocateParticleBuffers/allocateRadixBvh
  * (sph_gravity.cuh:83-86) and Simulation::Impl::ensureCapacity (simulation.cpp:428-454).
- * Slots are addressed with 29 bits in the neighbour walk, so capacity is at most
+ * Slots are addressed with 28 bits in the neighbour walk, so capacity is at most
  * ASTRAEA_MAX_CAPACITY (far beyond what 288 GB holds with the neighbour table). */
-#define ASTRAEA_MAX_CAPACITY (1u << 28)
+#define ASTRAEA_MAX_CAPACITY (1u << 27)
 int astraea_create(astraea_ctx** out, int device, uint32_t capacity, uint32_t max_segments);
 int astraea_destroy(astraea_ctx* ctx);
 /* The context's HIP stream (a hipStream_t), for callers that time or order work on it. */
