@@ -93,10 +93,25 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
     const float e2Node = __builtin_fmaf(epsI, epsI, 1.0e-20f);
     L.self[lane] = 0u;
 
-    // group bounding box over the active particles
-    const float lx = waveMinF(active ? pi.x : 3.0e38f), hx = waveMaxF(active ? pi.x : -3.0e38f);
-    const float ly = waveMinF(active ? pi.y : 3.0e38f), hy = waveMaxF(active ? pi.y : -3.0e38f);
-    const float lz = waveMinF(active ? pi.z : 3.0e38f), hz = waveMaxF(active ? pi.z : -3.0e38f);
+    // bounding boxes of the four sub-groups of 16 lanes over their active particles (a sub-group
+    // without one gets an inverted box: every node is apart from it and accepted by it)
+    float bl[4][3], bh[4][3];
+    {
+        float v[6] = {active ? pi.x : 3.0e38f, active ? pi.y : 3.0e38f, active ? pi.z : 3.0e38f,
+                      active ? pi.x : -3.0e38f, active ? pi.y : -3.0e38f, active ? pi.z : -3.0e38f};
+        for (int o = 1; o < 16; o <<= 1) {
+            for (int a = 0; a < 3; ++a) v[a] = fminf(v[a], __shfl_xor(v[a], o, 64));
+            for (int a = 3; a < 6; ++a) v[a] = fmaxf(v[a], __shfl_xor(v[a], o, 64));
+        }
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                bl[g][a] = readLaneF(v[a], 16 * g);
+                bh[g][a] = readLaneF(v[3 + a], 16 * g);
+            }
+        }
+    }
 
     float ax = 0.0f, ay = 0.0f, az = 0.0f, phi = 0.0f;
     uint32_t listed = 0, forced = 0;
@@ -132,22 +147,31 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
                         const float e2 = mn.w * mn.w;
                         T = acceptThreshold(e2, theta2, invTheta2, pow2);
                     }
-                    // squared distance from the com to the group box (g2) and to its farthest
-                    // corner (f2); every particle's FP32 r^2 lies within 1e-6 (relative) of
-                    // [g2, f2], far inside the 4e-5 margins below
-                    const float ux = lx - cm.x, vx = cm.x - hx;
-                    const float uy = ly - cm.y, vy = cm.y - hy;
-                    const float uz = lz - cm.z, vz = cm.z - hz;
-                    const float gx = fmaxf(0.0f, fmaxf(ux, vx)), fx = fmaxf(fabsf(ux), fabsf(vx));
-                    const float gy = fmaxf(0.0f, fmaxf(uy, vy)), fy = fmaxf(fabsf(uy), fabsf(vy));
-                    const float gz = fmaxf(0.0f, fmaxf(uz, vz)), fz = fmaxf(fabsf(uz), fabsf(vz));
-                    const float g2 = gx * gx + gy * gy + gz * gz;
-                    const float f2 = fx * fx + fy * fy + fz * fz;
-                    bool apart = true;  // no group particle can lie inside the node's box
-                    if (CHECK_INSIDE)
-                        apart = hx < mn.x || lx > mx.x || hy < mn.y || ly > mx.y || hz < mn.z || lz > mx.z;
-                    if (apart && g2 > T * (1.0f + 4.0e-5f)) accept = true;
-                    else if (!(f2 * (1.0f + 4.0e-5f) < T)) undecided = true;
+                    // per sub-group: squared distance from the com to its box (g2) and to the
+                    // box's farthest corner (f2); every particle's FP32 r^2 lies within 1e-6
+                    // (relative) of [g2, f2], far inside the 4e-5 margins below.  The node is
+                    // accepted when every sub-group surely accepts it, opened when one sub-group
+                    // surely rejects it (all its particles fail), and tested exactly otherwise.
+                    bool allPass = true, oneFails = false;
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) {
+                        const float ux = bl[g][0] - cm.x, vx = cm.x - bh[g][0];
+                        const float uy = bl[g][1] - cm.y, vy = cm.y - bh[g][1];
+                        const float uz = bl[g][2] - cm.z, vz = cm.z - bh[g][2];
+                        const float gx = fmaxf(0.0f, fmaxf(ux, vx)), fx = fmaxf(fabsf(ux), fabsf(vx));
+                        const float gy = fmaxf(0.0f, fmaxf(uy, vy)), fy = fmaxf(fabsf(uy), fabsf(vy));
+                        const float gz = fmaxf(0.0f, fmaxf(uz, vz)), fz = fmaxf(fabsf(uz), fabsf(vz));
+                        const float g2 = gx * gx + gy * gy + gz * gz;
+                        const float f2 = fx * fx + fy * fy + fz * fz;
+                        bool apart = true;  // no particle of the sub-group can lie inside the node's box
+                        if (CHECK_INSIDE)
+                            apart = bh[g][0] < mn.x || bl[g][0] > mx.x || bh[g][1] < mn.y || bl[g][1] > mx.y ||
+                                    bh[g][2] < mn.z || bl[g][2] > mx.z;
+                        allPass = allPass && apart && g2 > T * (1.0f + 4.0e-5f);
+                        oneFails = oneFails || f2 * (1.0f + 4.0e-5f) < T;
+                    }
+                    if (allPass) accept = true;
+                    else if (!oneFails) undecided = true;
                     rec = make_float4(cm.x, cm.y, cm.z, kGf * cm.w);
                 } else if (isLeaf) {
                     const uint32_t j = (uint32_t)~cc;
