@@ -171,7 +171,7 @@ __global__ __launch_bounds__(kBlock) void kDrift(DeviceCtx c) {
         float4 p, v;
         float h;
     };
-    constexpr int kStage = 256;
+    constexpr int kStage = 128;  // 24 KB of records per block: five wavefronts per SIMD
     __shared__ float sb[kWavesPerBlock][6];
     __shared__ Rec stageBuf[kWavesPerBlock][kStage];
     const uint32_t b = xcdBlock(blockIdx.x, gridDim.x);

@@ -161,13 +161,23 @@ __global__ __launch_bounds__(kBlock) void kEos(DeviceCtx c) {
     }
 }
 
+// Pass-2 neighbour record staged in LDS: 64 bytes, 80 with the stress deviator S/rho^2 (a
+// smaller record lets five blocks share a CU instead of four).
+template <bool STRENGTH>
+struct Pass2Rec {
+    float4 p, n, v;
+    uint32_t mat;
+};
+template <>
+struct Pass2Rec<true> {
+    float4 p, n, v, s0;
+    float2 s1;
+    uint32_t mat;
+};
+
 template <bool STRENGTH>
 __global__ __launch_bounds__(kBlock) void kPass2(DeviceCtx c) {
-    struct Rec {
-        float4 p, n, v, s0;
-        float2 s1;
-        uint32_t mat;
-    };
+    using Rec = Pass2Rec<STRENGTH>;
     __shared__ Rec stageBuf[kWavesPerBlock][kStage2];
     const uint32_t b = xcdBlock(blockIdx.x, gridDim.x);
     const uint32_t i = b * kBlock + threadIdx.x;
@@ -213,7 +223,7 @@ __global__ __launch_bounds__(kBlock) void kPass2(DeviceCtx c) {
             r.n = c.d.nb[j];
             r.v = c.cur.vel[j];
             r.mat = c.cur.meta[j] & 0xffu;
-            if (STRENGTH) {
+            if constexpr (STRENGTH) {
                 r.s0 = c.d.sr0[j];
                 r.s1 = c.d.sr1[j];
             }
@@ -265,7 +275,7 @@ __global__ __launch_bounds__(kBlock) void kPass2(DeviceCtx c) {
             duP += mj * vdg;
             duAv += 0.5f * mj * piIJ * vdg;
             duT += 0.5f * mj * tens * vdg;
-            if (STRENGTH) {
+            if constexpr (STRENGTH) {
                 const float4 sj0 = n.s0;
                 const float2 sj1 = n.s1;
                 const float sxx = srI0.x + sj0.x, syy = srI0.y + sj0.y, szz = srI0.z + sj0.z;
