@@ -270,6 +270,7 @@ def main():
         "energy_drift_rel": {"max_abs": float(np.max(np.abs(drift))), "median_abs": float(np.median(np.abs(drift))),
                              "steps": K},
         "interactions_per_particle": b["cnt"]["interactions"] / (n * K),
+        "gravity_forced_accepts": int(b["cnt"]["forced_accepts"]),
         "neighbour_candidates_per_particle": b["cnt2"]["nbr_entries"] / n,
         "stage_ms": {k: (v[0] / max(v[1], 1)) for k, v in stages.items()},
         "roofline": {
