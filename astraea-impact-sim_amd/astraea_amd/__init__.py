@@ -123,6 +123,11 @@ class PlanetInfo(C.Structure):
                 ("profile_samples", C.c_uint32)]
 
 
+class Tuning(C.Structure):
+    """astraea_tuning: union records staged per round by pass 1, pass 2 and the drift."""
+    _fields_ = [("pass1_round", C.c_uint32), ("pass2_round", C.c_uint32), ("drift_round", C.c_uint32)]
+
+
 class BodyOut(C.Structure):
     _fields_ = [(n, C.POINTER(C.c_float)) for n in ("x", "y", "z", "m", "h", "rho", "u", "eps_fail")] + [
         ("mat", C.POINTER(C.c_uint8))
@@ -170,6 +175,9 @@ _sig = {
     "astraea_set_neighbour_stride": (C.c_int, [_P, C.c_uint32]),
     "astraea_download_tree": (C.c_int, [_P, C.c_uint32] + [_P] * 9),
     "astraea_enable_stage_timing": (C.c_int, [_P, C.c_int]),
+    "astraea_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+    "astraea_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+    "astraea_autotune": (C.c_int, [_P, C.c_float, C.c_uint32, C.POINTER(Tuning), C.POINTER(C.c_double)]),
     "astraea_stage_times": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "astraea_default_body_spec": (None, [C.POINTER(BodySpec)]),
     "astraea_default_planet_spec": (None, [C.POINTER(PlanetSpec)]),
@@ -407,6 +415,50 @@ class Context:
         n = (C.c_uint64 * len(STAGES))()
         _check(_lib.astraea_stage_times(self._h, ms, n))
         return {s: (ms[k], int(n[k])) for k, s in enumerate(STAGES)}
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        _check(_lib.astraea_get_tuning(self._h, C.byref(t)))
+        return {k: int(getattr(t, k)) for k, _ in Tuning._fields_}
+
+    def set_tuning(self, pass1_round: int, pass2_round: int, drift_round: int):
+        t = Tuning(pass1_round, pass2_round, drift_round)
+        _check(_lib.astraea_set_tuning(self._h, C.byref(t)))
+
+    def autotune(self, max_dt: float, steps_per_variant: int = 1) -> dict:
+        """Runs 4 * steps_per_variant steps timing every launch variant (astraea_autotune) and
+        keeps the fastest; returns the choice and the measured mean milliseconds per step."""
+        t = Tuning()
+        ms = (C.c_double * 8)()
+        _check(_lib.astraea_autotune(self._h, float(max_dt), int(steps_per_variant), C.byref(t), ms))
+        chosen = {k: int(getattr(t, k)) for k, _ in Tuning._fields_}
+        return dict(chosen=chosen, pass1_ms={128: ms[0], 256: ms[1]}, pass2_ms={64: ms[2], 128: ms[3]},
+                    drift_ms={64: ms[4], 128: ms[5], 192: ms[6], 256: ms[7]})
+
+
+def tuning_key(particles: int, segments: int, strength: bool, fp64: bool) -> str:
+    """Key of a workload in the tuning table: the particle count to a power of two, the segments
+    and the kernel variants in use."""
+    return f"gfx950/n{max(int(particles), 1).bit_length()}/s{int(segments)}/{'strength' if strength else 'plain'}" \
+           f"/{'fp64' if fp64 else 'fp32'}"
+
+
+def tune_cached(ctx: "Context", max_dt: float, path, key: str, steps_per_variant: int = 1) -> dict:
+    """Applies the tuning stored under `key` in the JSON table at `path`, or runs the autotuner
+    (4 * steps_per_variant steps of the loaded state) and stores its choice there."""
+    import json
+
+    p = Path(path)
+    table = json.loads(p.read_text()) if p.exists() else {}
+    if key in table:
+        t = table[key]
+        ctx.set_tuning(t["pass1_round"], t["pass2_round"], t["drift_round"])
+        return dict(chosen=dict(t), cached=True)
+    r = ctx.autotune(max_dt, steps_per_variant)
+    table[key] = r["chosen"]
+    p.parent.mkdir(parents=True, exist_ok=True)
+    p.write_text(json.dumps(table, indent=1, sort_keys=True) + "\n")
+    return dict(r, cached=False)
 
 
 def splitmix64(x: int) -> int:

@@ -1299,6 +1299,72 @@ int astraea_download_tree(astraea_ctx* c, uint32_t seg, uint64_t* keys, uint32_t
     return ASTRAEA_OK;
 }
 
+static bool validTuning(const astraea_tuning& t) {
+    return (t.pass1_round == 128u || t.pass1_round == 256u) && (t.pass2_round == 64u || t.pass2_round == 128u) &&
+           (t.drift_round == 64u || t.drift_round == 128u || t.drift_round == 192u || t.drift_round == 256u);
+}
+
+int astraea_get_tuning(astraea_ctx* c, astraea_tuning* out) {
+    if (!c || !out) return fail(ASTRAEA_E_INVALID, "null argument");
+    out->pass1_round = c->dc.tune.pass1Round;
+    out->pass2_round = c->dc.tune.pass2Round;
+    out->drift_round = c->dc.tune.driftRound;
+    return ASTRAEA_OK;
+}
+
+int astraea_set_tuning(astraea_ctx* c, const astraea_tuning* t) {
+    if (!c || !t) return fail(ASTRAEA_E_INVALID, "null argument");
+    if (!validTuning(*t))
+        return fail(ASTRAEA_E_INVALID, "tuning (%u, %u, %u) outside pass 1 {128, 256}, pass 2 {64, 128}, drift {64, 128, 192, 256}",
+                    t->pass1_round, t->pass2_round, t->drift_round);
+    c->dc.tune.pass1Round = t->pass1_round;
+    c->dc.tune.pass2Round = t->pass2_round;
+    c->dc.tune.driftRound = t->drift_round;
+    freeGraph(c);  // a captured step holds the old kernels
+    return ASTRAEA_OK;
+}
+
+int astraea_autotune(astraea_ctx* c, float maxDt, uint32_t stepsPerVariant, astraea_tuning* chosen, double* ms) {
+    if (!c) return fail(ASTRAEA_E_INVALID, "null context");
+    if (stepsPerVariant == 0) return fail(ASTRAEA_E_INVALID, "steps per variant must be > 0");
+    static constexpr uint32_t p1[2] = {128u, 256u}, p2[2] = {64u, 128u}, dr[4] = {64u, 128u, 192u, 256u};
+    const bool wasTiming = c->timing;
+    const astraea_tuning before = {c->dc.tune.pass1Round, c->dc.tune.pass2Round, c->dc.tune.driftRound};
+    double t1[2] = {0.0, 0.0}, t2[2] = {0.0, 0.0}, td[4] = {0.0, 0.0, 0.0, 0.0};
+    int rc = astraea_enable_stage_timing(c, 1);
+    // run r takes pass-1 and pass-2 variant r % 2 and drift variant r: four runs time every variant
+    for (int r = 0; r < 4 && rc == ASTRAEA_OK; ++r) {
+        const astraea_tuning t = {p1[r % 2], p2[r % 2], dr[r]};
+        rc = astraea_set_tuning(c, &t);
+        if (rc == ASTRAEA_OK) rc = astraea_step(c, maxDt, stepsPerVariant, 0);
+        double stage[ASTRAEA_STAGE_COUNT_];
+        uint64_t n[ASTRAEA_STAGE_COUNT_];
+        if (rc == ASTRAEA_OK) rc = astraea_stage_times(c, stage, n);
+        if (rc != ASTRAEA_OK) break;
+        const auto mean = [&](int k) { return n[k] ? stage[k] / (double)n[k] : 0.0; };
+        t1[r % 2] += 0.5 * mean(ASTRAEA_STAGE_PASS1);
+        t2[r % 2] += 0.5 * mean(ASTRAEA_STAGE_PASS2);
+        td[r] = mean(ASTRAEA_STAGE_DRIFT);
+    }
+    const int rt = astraea_enable_stage_timing(c, wasTiming ? 1 : 0);
+    if (rc != ASTRAEA_OK) {
+        (void)astraea_set_tuning(c, &before);
+        return rc;
+    }
+    TRY(rt);
+    int bd = 0;
+    for (int k = 1; k < 4; ++k)
+        if (td[k] < td[bd]) bd = k;
+    const astraea_tuning best = {t1[0] < t1[1] ? p1[0] : p1[1], t2[0] < t2[1] ? p2[0] : p2[1], dr[bd]};
+    TRY(astraea_set_tuning(c, &best));
+    if (chosen) *chosen = best;
+    if (ms) {
+        const double all[8] = {t1[0], t1[1], t2[0], t2[1], td[0], td[1], td[2], td[3]};
+        for (int k = 0; k < 8; ++k) ms[k] = all[k];
+    }
+    return ASTRAEA_OK;
+}
+
 int astraea_enable_stage_timing(astraea_ctx* c, int enable) {
     if (!c) return fail(ASTRAEA_E_INVALID, "null context");
     if (enable && c->evPool.empty()) {

@@ -165,15 +165,16 @@ __global__ __launch_bounds__(kBlock) void kDtCandidates(DeviceCtx c) {
     reduceDtRad(c, sg, dtI, rad);
 }
 
-template <bool FP64>
+// S: union records staged per round (every value gives the same results; the launcher takes
+// the one the context's tuning names).
+template <bool FP64, int S>
 __global__ __launch_bounds__(kBlock) void kDrift(DeviceCtx c) {
     struct Rec {
         float4 p, v;
         float h;
     };
-    constexpr int kStage = 128;  // 24 KB of records per block: five wavefronts per SIMD
     __shared__ float sb[kWavesPerBlock][6];
-    __shared__ Rec stageBuf[kWavesPerBlock][kStage];
+    __shared__ Rec stageBuf[kWavesPerBlock][S];
     const uint32_t b = xcdBlock(blockIdx.x, gridDim.x);
     const uint32_t i = b * kBlock + threadIdx.x;
     const int lane = laneId(), wave = threadIdx.x >> 6;
@@ -187,7 +188,7 @@ __global__ __launch_bounds__(kBlock) void kDrift(DeviceCtx c) {
     const float rhoI = fmaxf(vi.w, 1.0e-9f);
     float xs = 0.0f, ys = 0.0f, zs = 0.0f;
     const Support sI = supportOf(Hi);
-    forEachNeighbour<kStage>(
+    forEachNeighbour<S>(
         c.nb, i, stageBuf[wave],
         [&](uint32_t j, Rec& r) {
             r.p = c.cur.pos[j];
@@ -402,9 +403,21 @@ void launchKick(const DeviceCtx& c, hipStream_t s, bool second, bool clamp) {
 #undef ASTRAEA_KICK
 }
 
+namespace {
+template <int S>
+void driftAs(const DeviceCtx& c, hipStream_t s) {
+    if (c.fp64) kDrift<true, S><<<c.blocks, kBlock, 0, s>>>(c);
+    else kDrift<false, S><<<c.blocks, kBlock, 0, s>>>(c);
+}
+}  // namespace
+
 void launchDrift(const DeviceCtx& c, hipStream_t s) {
-    if (c.fp64) kDrift<true><<<c.blocks, kBlock, 0, s>>>(c);
-    else kDrift<false><<<c.blocks, kBlock, 0, s>>>(c);
+    switch (c.tune.driftRound) {
+        case 64u: driftAs<64>(c, s); break;
+        case 192u: driftAs<192>(c, s); break;
+        case 256u: driftAs<256>(c, s); break;
+        default: driftAs<128>(c, s); break;
+    }
 }
 
 void launchStepBegin(const DeviceCtx& c, hipStream_t s, float maxDt) {

@@ -20,6 +20,14 @@ struct SortBuffers {
 };
 
 // Everything a kernel may need, passed by value.
+// Launch variants: union records each neighbour pass stages in LDS per round.  They change LDS
+// use (and so occupancy), never results.
+struct Tuning {
+    uint32_t pass1Round = 256;  // 128 or 256
+    uint32_t pass2Round = 128;  // 64 or 128
+    uint32_t driftRound = 128;  // 64, 128, 192 or 256
+};
+
 struct DeviceCtx {
     uint32_t slots;      // padded particle slots in use
     uint32_t blocks;     // slots / kBlock
@@ -36,6 +44,7 @@ struct DeviceCtx {
     Params prm;
     bool fp64;           // FP64 position/velocity accumulators
     bool strength;       // any material has strength enabled
+    Tuning tune;
 };
 
 // sort.hip: stable LSD radix sort of (key, value) over `bits` key bits, then (segments > 1)

@@ -41,15 +41,15 @@ __device__ __forceinline__ float maxPrincipalStress(float p, float sxx, float sy
     return -p + 2.0f * q * cosf(acosf(r) * (1.0f / 3.0f));
 }
 
-// Union records staged per round by the neighbour passes (one LDS buffer per wavefront).
-constexpr int kStage1 = 256, kStage2 = 128;
-
-template <bool STRENGTH>
+// The union records a neighbour pass stages per round (S, one LDS buffer per wavefront) is a
+// template parameter: every value gives the same results, and the launcher takes the one the
+// context's tuning names.
+template <bool STRENGTH, int S>
 __global__ __launch_bounds__(kBlock) void kPass1(DeviceCtx c) {
     struct Rec {
         float4 p, v;
     };
-    __shared__ Rec stageBuf[kWavesPerBlock][kStage1];
+    __shared__ Rec stageBuf[kWavesPerBlock][S];
     const uint32_t b = xcdBlock(blockIdx.x, gridDim.x);
     const uint32_t i = b * kBlock + threadIdx.x;
     const uint32_t sg = c.seg.slotSeg[i];
@@ -70,7 +70,7 @@ __global__ __launch_bounds__(kBlock) void kPass1(DeviceCtx c) {
     float gxx = 0.0f, gxy = 0.0f, gxz = 0.0f, gyx = 0.0f, gyy = 0.0f, gyz = 0.0f, gzx = 0.0f, gzy = 0.0f, gzz = 0.0f;
 
     const Support sI = supportOf(Hi);
-    forEachNeighbour<kStage1>(
+    forEachNeighbour<S>(
         c.nb, i, stageBuf[threadIdx.x >> 6],
         [&](uint32_t j, Rec& r) {
             r.p = c.cur.pos[j];
@@ -175,10 +175,10 @@ struct Pass2Rec<true> {
     uint32_t mat;
 };
 
-template <bool STRENGTH>
+template <bool STRENGTH, int S>
 __global__ __launch_bounds__(kBlock) void kPass2(DeviceCtx c) {
     using Rec = Pass2Rec<STRENGTH>;
-    __shared__ Rec stageBuf[kWavesPerBlock][kStage2];
+    __shared__ Rec stageBuf[kWavesPerBlock][S];
     const uint32_t b = xcdBlock(blockIdx.x, gridDim.x);
     const uint32_t i = b * kBlock + threadIdx.x;
     const uint32_t sg = c.seg.slotSeg[i];
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(kBlock) void kPass2(DeviceCtx c) {
 
     const Support sI = supportOf(Hi);
     const float dampI = 0.01f * Hi * Hi;
-    forEachNeighbour<kStage2>(
+    forEachNeighbour<S>(
         c.nb, i, stageBuf[threadIdx.x >> 6],
         [&](uint32_t j, Rec& r) {
             r.p = c.cur.pos[j];
@@ -353,11 +353,22 @@ __global__ __launch_bounds__(kBlock) void kPass2(DeviceCtx c) {
     c.d.rate2[i] = make_float4(du, dDc, (float)pRad, (float)pairs);
 }
 
+template <int S>
+void pass1As(const DeviceCtx& c, hipStream_t s) {
+    if (c.strength) kPass1<true, S><<<c.blocks, kBlock, 0, s>>>(c);
+    else kPass1<false, S><<<c.blocks, kBlock, 0, s>>>(c);
+}
+template <int S>
+void pass2As(const DeviceCtx& c, hipStream_t s) {
+    if (c.strength) kPass2<true, S><<<c.blocks, kBlock, 0, s>>>(c);
+    else kPass2<false, S><<<c.blocks, kBlock, 0, s>>>(c);
+}
+
 }  // namespace
 
 void launchPass1(const DeviceCtx& c, hipStream_t s) {
-    if (c.strength) kPass1<true><<<c.blocks, kBlock, 0, s>>>(c);
-    else kPass1<false><<<c.blocks, kBlock, 0, s>>>(c);
+    if (c.tune.pass1Round == 128u) pass1As<128>(c, s);
+    else pass1As<256>(c, s);
 }
 
 void launchEos(const DeviceCtx& c, hipStream_t s) {
@@ -366,8 +377,8 @@ void launchEos(const DeviceCtx& c, hipStream_t s) {
 }
 
 void launchPass2(const DeviceCtx& c, hipStream_t s) {
-    if (c.strength) kPass2<true><<<c.blocks, kBlock, 0, s>>>(c);
-    else kPass2<false><<<c.blocks, kBlock, 0, s>>>(c);
+    if (c.tune.pass2Round == 64u) pass2As<64>(c, s);
+    else pass2As<128>(c, s);
 }
 
 }  // namespace astraea_hip

@@ -73,7 +73,7 @@ def time_steps(ctx, steps, graph=True):
     return time.perf_counter() - t0
 
 
-def run_batched(A, state, counts, steps, warmup):
+def run_batched(A, state, counts, steps, warmup, autotune=True):
     from astraea_amd import scenario as S
 
     n = int(sum(counts))
@@ -85,6 +85,12 @@ def run_batched(A, state, counts, steps, warmup):
     ctx.initial_forces()
     ctx.synchronize()
     e_init = energy_total(ctx.energy())
+    tune = None
+    if autotune:
+        # untimed like the warm-up: four eager steps, one per launch variant of each pass
+        log("[bench] batched: autotune (4 steps)")
+        tune = ctx.autotune(MAX_DT, steps_per_variant=1)
+        log(f"[bench] batched: tuning {tune['chosen']}")
     ctx.step(MAX_DT, warmup, graph=True)
     ctx.synchronize()
     e_start = energy_total(ctx.energy())
@@ -102,7 +108,7 @@ def run_batched(A, state, counts, steps, warmup):
     stages = ctx.stage_times()
     ctx.enable_stage_timing(False)
     cnt2 = ctx.counters()
-    return dict(wall=wall, drift=drift, cnt=cnt, cnt2=cnt2, stages=stages, n=n)
+    return dict(wall=wall, drift=drift, cnt=cnt, cnt2=cnt2, stages=stages, n=n, tune=tune)
 
 
 def run_sequential_plain(A, states, counts, steps, warmup):
@@ -217,6 +223,8 @@ def main():
     ap.add_argument("--cpu-particles", type=int, default=1_000_000)
     ap.add_argument("--drift-steps", type=int, default=1000,
                     help="steps of the energy-drift comparison with the FP64 oracle (0: skip)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch variants instead of timing them before the warm-up")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the batched scenarios would shard "
@@ -232,7 +240,7 @@ def main():
     states, counts = S.build_sweep(specs, split=True)
     merged = {f: np.ascontiguousarray(np.concatenate([s[f] for s in states])) for f in states[0]}
 
-    b = run_batched(A, merged, counts, K, W)
+    b = run_batched(A, merged, counts, K, W, autotune=not args.no_autotune)
     n = b["n"]
     value = n * K / b["wall"]
     stages = b["stages"]
@@ -273,6 +281,8 @@ def main():
         "gravity_forced_accepts": int(b["cnt"]["forced_accepts"]),
         "neighbour_candidates_per_particle": b["cnt2"]["nbr_entries"] / n,
         "stage_ms": {k: (v[0] / max(v[1], 1)) for k, v in stages.items()},
+        "tuning": b["tune"] and {"chosen": b["tune"]["chosen"], "steps": 4,
+                                 "ms_per_step": {k: b["tune"][k] for k in ("pass1_ms", "pass2_ms", "drift_ms")}},
         "roofline": {
             "bound": "valu",
             "kernel": "kGravity",

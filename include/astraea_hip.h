@@ -325,6 +325,25 @@ int astraea_download_tree(astraea_ctx* ctx, uint32_t seg,
 int astraea_enable_stage_timing(astraea_ctx* ctx, int enable);
 int astraea_stage_times(astraea_ctx* ctx, double* out_ms, uint64_t* out_samples);
 
+/* Launch variants (no counterpart in the reference, whose block sizes and stack depth are
+ * compile-time constants, sph_gravity.cu:19-22): the union records each neighbour pass stages
+ * in LDS per round.  Every variant gives bit-identical results (a particle's pairs run in the
+ * same order); they differ in LDS use, and so in how many wavefronts share a SIMD. */
+typedef struct astraea_tuning {
+    uint32_t pass1_round; /* 128 or 256 (default 256) */
+    uint32_t pass2_round; /* 64 or 128 (default 128) */
+    uint32_t drift_round; /* 64, 128, 192 or 256 (default 128) */
+} astraea_tuning;
+int astraea_get_tuning(astraea_ctx* ctx, astraea_tuning* out);
+int astraea_set_tuning(astraea_ctx* ctx, const astraea_tuning* tuning);
+/* The autotuner: advances the loaded state by 4 * steps_per_variant eager steps, timing every
+ * variant of each pass with the stage events, and keeps the fastest for each pass.  The
+ * simulation does not depend on the variant, so these are ordinary steps.  `chosen` and `ms`
+ * (mean milliseconds per step: pass 1 at 128 and 256, pass 2 at 64 and 128, drift at 64, 128,
+ * 192 and 256; 8 values) may be NULL.  On an error the previous tuning is restored. */
+int astraea_autotune(astraea_ctx* ctx, float max_dt_psu, uint32_t steps_per_variant, astraea_tuning* chosen,
+                     double* ms);
+
 /* ---- host-side initial conditions (no device work) ---- */
 
 /* A differentiated body on a seed-jittered hexagonal-close-packed lattice:

@@ -251,3 +251,33 @@ def test_pcls_wire_format():
     exe = Path(__file__).resolve().parent / "cpp" / "test_simulation"
     r = subprocess.run([str(exe), "pcls"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_tuning_table_round_trip(lib, tmp_path):
+    """tune_cached runs the autotuner once per workload key and applies the stored choice after."""
+
+    class FakeContext:
+        def __init__(self):
+            self.applied, self.tuned = None, 0
+
+        def set_tuning(self, p1, p2, d):
+            self.applied = (p1, p2, d)
+
+        def autotune(self, max_dt, steps_per_variant):
+            self.tuned += 1
+            return dict(chosen={"pass1_round": 128, "pass2_round": 64, "drift_round": 192})
+
+    path = tmp_path / "tuning.json"
+    key = lib.tuning_key(9_602_616, 64, False, True)
+    assert key == "gfx950/n24/s64/plain/fp64"
+    ctx = FakeContext()
+    first = lib.tune_cached(ctx, 5e-3, path, key)
+    assert not first["cached"] and ctx.tuned == 1
+    again = FakeContext()
+    second = lib.tune_cached(again, 5e-3, path, key)
+    assert second["cached"] and again.tuned == 0 and again.applied == (128, 64, 192)
+    other = FakeContext()
+    assert not lib.tune_cached(other, 5e-3, path, lib.tuning_key(1_000_000, 1, False, False))["cached"]
+    import json
+
+    assert set(json.loads(path.read_text())) == {key, "gfx950/n20/s1/plain/fp32"}

@@ -1,0 +1,73 @@
+"""Launch variants and the autotuner (astraea_set_tuning, astraea_autotune).
+
+A variant changes how many union records a neighbour pass stages in LDS per round, never the
+order in which a particle's pairs are summed, so every variant must give bit-identical states;
+the autotuner's steps are ordinary steps and must equal the same number of untuned steps.
+"""
+import numpy as np
+import pytest
+
+from test_gpu_parity import MAX_DT, perturbed_small_state
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("pos_x", "pos_y", "pos_z", "vel_x", "vel_y", "vel_z", "u", "rho", "h", "damage")
+
+
+def _context(lib, st, strength):
+    from astraea_amd import scenario as S
+
+    n = len(st["mass"])
+    ctx = lib.Context(n)
+    if strength:
+        ctx.set_params(S.plain_params(strength_mask=(1 << S.GRANITE) | (1 << S.IRON)))
+    else:
+        ctx.set_params(S.plain_params())
+    ctx.set_layout([n])
+    ctx.load_state(st)
+    ctx.initial_forces()
+    return ctx
+
+
+def _state(ctx):
+    ctx.synchronize()
+    return {k: ctx.download(k) for k in FIELDS}
+
+
+@pytest.mark.parametrize("strength", [False, True])
+def test_variants_are_bit_identical(lib, strength):
+    st = perturbed_small_state(strength=strength)
+    want = None
+    for t in [(256, 128, 128), (128, 64, 64), (256, 64, 192), (128, 128, 256)]:
+        ctx = _context(lib, st, strength)
+        ctx.set_tuning(*t)
+        assert tuple(ctx.tuning().values()) == t
+        ctx.step(MAX_DT, 3, graph=True)
+        got = _state(ctx)
+        if want is None:
+            want = got
+            continue
+        for k in FIELDS:
+            assert np.array_equal(got[k], want[k]), (t, k)
+
+
+def test_autotune_steps_are_ordinary_steps(lib):
+    st = perturbed_small_state()
+    tuned = _context(lib, st, False)
+    r = tuned.autotune(MAX_DT, steps_per_variant=1)
+    assert r["chosen"]["pass1_round"] in (128, 256) and r["chosen"]["pass2_round"] in (64, 128)
+    assert r["chosen"]["drift_round"] in (64, 128, 192, 256)
+    assert tuned.tuning() == r["chosen"]
+    assert all(v > 0.0 for d in ("pass1_ms", "pass2_ms", "drift_ms") for v in r[d].values())
+    plain = _context(lib, st, False)
+    plain.step(MAX_DT, 4, graph=False)
+    a, b = _state(tuned), _state(plain)
+    for k in FIELDS:
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_invalid_tuning_is_rejected(lib):
+    ctx = _context(lib, perturbed_small_state(), False)
+    with pytest.raises(lib.AstraeaError):
+        ctx.set_tuning(512, 128, 128)
+    assert ctx.tuning() == {"pass1_round": 256, "pass2_round": 128, "drift_round": 128}
