@@ -569,7 +569,7 @@ int astraea_create(astraea_ctx** out, int device, uint32_t capacity, uint32_t ma
     A(devAlloc(c, d.ss.simTime, maxSegments));
     A(devAlloc(c, d.ss.living, maxSegments));
     A(devAlloc(c, d.cnt.v, 8));
-    const uint32_t tiles = (C + 4095) / 4096;
+    const uint32_t tiles = (C + kSortTile - 1) / kSortTile;
     A(devAlloc(c, d.sb.keyA, C));
     A(devAlloc(c, d.sb.keyB, C));
     A(devAlloc(c, d.sb.valA, C));

@@ -8,6 +8,9 @@
 
 namespace astraea_hip {
 
+// Elements per tile of the radix sort (one 256-thread block, eight per thread).
+constexpr uint32_t kSortTile = 2048;
+
 struct SortBuffers {
     uint64_t* keyA;
     uint64_t* keyB;
@@ -16,7 +19,7 @@ struct SortBuffers {
     uint32_t* counts;      // [256 * tiles] digit-major tile histograms
     uint32_t* tilePrefix;  // [256 * tiles] exclusive prefix over tiles per digit
     uint32_t* digitTotal;  // [256]
-    uint32_t tilesCap;
+    uint32_t tilesCap;     // tiles of kSortTile the counts hold
 };
 
 // Everything a kernel may need, passed by value.

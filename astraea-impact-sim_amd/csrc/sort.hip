@@ -4,12 +4,12 @@
 // The result is exactly std::stable_sort by key: every pass is stable.
 //
 // One pass (8-bit digit) = three kernels:
-//   hist    per 4096-element tile, digit counts (digit-major: counts[d * tiles + t]);
+//   hist    per 2048-element tile, digit counts (digit-major: counts[d * tiles + t]);
 //   scan    one block per digit: exclusive prefix over tiles + digit totals;
 //   scatter per tile: stable local ranks from wavefront ballots (each lane's digit is
 //           matched against the wave with 8 ballots), per-wave running counts in LDS,
 //           an LDS-staged reorder, then coalesced runs to HBM.
-// Tiles are wave-contiguous: wave w of a tile owns 16 consecutive 64-element rows, so the
+// Tiles are wave-contiguous: wave w of a tile owns kIPT consecutive 64-element rows, so the
 // element order inside a tile is (wave, row, lane) and ranks stay stable.
 // With several segments a final stable pass on the segment id groups the scenarios.
 #include "launch.hpp"
@@ -17,8 +17,9 @@
 namespace astraea_hip {
 namespace {
 
-constexpr int kIPT = 16;
-constexpr int kTile = kBlock * kIPT;  // 4096 elements
+constexpr int kTile = (int)kSortTile;
+constexpr int kIPT = kTile / kBlock;
+static_assert(kTile % kBlock == 0, "a tile is whole rows of the block");
 
 __device__ __forceinline__ uint32_t digitOf(uint64_t key, uint32_t val, int shift, const uint32_t* slotSeg, bool seg) {
     return seg ? slotSeg[val] : (uint32_t)(key >> shift) & 0xffu;
