@@ -41,6 +41,10 @@ METRIC = "particle-steps/s (full KDK step), batched mixed vs plain FP32, ≤1e-4
 MAX_DT = 5.0 / 1.0e3          # Simulation::step default maxDtSec = 5 s (PSU time)
 PEAK_FP32_TFLOPS = 157.3      # MI355X published FP32 peak
 FLOP_PER_INTERACTION = 23     # 20 acceleration (GPU Gems 3 N-body convention) + 3 potential
+PEAK_HBM_GBS = 8000.0         # MI355X published HBM3E bandwidth
+# algorithmic bytes per particle of the stages the HBM roofline is asked for (compulsory traffic,
+# neighbour records cache-served, FP32 state): sort = 8 + 24 per 8-bit digit pass
+ALG_BYTES = {"sort": 200, "pass1": 126, "pass2": 151, "drift": 88}
 TRAFFIC_PROFILE = ROOT / "profiles" / "pmc_gravity_batched_sweep.md"
 
 
@@ -281,6 +285,10 @@ def main():
         "gravity_forced_accepts": int(b["cnt"]["forced_accepts"]),
         "neighbour_candidates_per_particle": b["cnt2"]["nbr_entries"] / n,
         "stage_ms": {k: (v[0] / max(v[1], 1)) for k, v in stages.items()},
+        "hbm": {k: {"GB/s": n * ALG_BYTES[k] / (stages[k][0] / max(stages[k][1], 1) * 1e-3) / 1e9,
+                    "frac": n * ALG_BYTES[k] / (stages[k][0] / max(stages[k][1], 1) * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                    "bytes_per_particle": ALG_BYTES[k]} for k in ALG_BYTES},
+        "gravity_interactions_per_s": inter_per_launch / (grav_avg_ms * 1e-3),
         "tuning": b["tune"] and {"chosen": b["tune"]["chosen"], "steps": 4,
                                  "ms_per_step": {k: b["tune"][k] for k in ("pass1_ms", "pass2_ms", "drift_ms")}},
         "roofline": {
