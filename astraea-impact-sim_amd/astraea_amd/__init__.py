@@ -426,7 +426,7 @@ class Context:
         _check(_lib.astraea_set_tuning(self._h, C.byref(t)))
 
     def autotune(self, max_dt: float, steps_per_variant: int = 1) -> dict:
-        """Runs 4 * steps_per_variant steps timing every launch variant (astraea_autotune) and
+        """Runs 8 * steps_per_variant steps timing every launch variant (astraea_autotune) and
         keeps the fastest; returns the choice and the measured mean milliseconds per step."""
         t = Tuning()
         ms = (C.c_double * 8)()
@@ -445,7 +445,7 @@ def tuning_key(particles: int, segments: int, strength: bool, fp64: bool) -> str
 
 def tune_cached(ctx: "Context", max_dt: float, path, key: str, steps_per_variant: int = 1) -> dict:
     """Applies the tuning stored under `key` in the JSON table at `path`, or runs the autotuner
-    (4 * steps_per_variant steps of the loaded state) and stores its choice there."""
+    (8 * steps_per_variant steps of the loaded state) and stores its choice there."""
     import json
 
     p = Path(path)

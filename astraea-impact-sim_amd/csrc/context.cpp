@@ -1328,13 +1328,15 @@ int astraea_autotune(astraea_ctx* c, float maxDt, uint32_t stepsPerVariant, astr
     if (!c) return fail(ASTRAEA_E_INVALID, "null context");
     if (stepsPerVariant == 0) return fail(ASTRAEA_E_INVALID, "steps per variant must be > 0");
     static constexpr uint32_t p1[2] = {128u, 256u}, p2[2] = {64u, 128u}, dr[4] = {64u, 128u, 192u, 256u};
+    // eight runs in palindromic orders: every variant's runs sit on average at the middle of the
+    // sequence, so a steady drift of the step's cost as the state evolves cancels out
+    static constexpr int order2[8] = {0, 1, 1, 0, 0, 1, 1, 0}, order4[8] = {0, 1, 2, 3, 3, 2, 1, 0};
     const bool wasTiming = c->timing;
     const astraea_tuning before = {c->dc.tune.pass1Round, c->dc.tune.pass2Round, c->dc.tune.driftRound};
     double t1[2] = {0.0, 0.0}, t2[2] = {0.0, 0.0}, td[4] = {0.0, 0.0, 0.0, 0.0};
     int rc = astraea_enable_stage_timing(c, 1);
-    // run r takes pass-1 and pass-2 variant r % 2 and drift variant r: four runs time every variant
-    for (int r = 0; r < 4 && rc == ASTRAEA_OK; ++r) {
-        const astraea_tuning t = {p1[r % 2], p2[r % 2], dr[r]};
+    for (int r = 0; r < 8 && rc == ASTRAEA_OK; ++r) {
+        const astraea_tuning t = {p1[order2[r]], p2[order2[r]], dr[order4[r]]};
         rc = astraea_set_tuning(c, &t);
         if (rc == ASTRAEA_OK) rc = astraea_step(c, maxDt, stepsPerVariant, 0);
         double stage[ASTRAEA_STAGE_COUNT_];
@@ -1342,9 +1344,9 @@ int astraea_autotune(astraea_ctx* c, float maxDt, uint32_t stepsPerVariant, astr
         if (rc == ASTRAEA_OK) rc = astraea_stage_times(c, stage, n);
         if (rc != ASTRAEA_OK) break;
         const auto mean = [&](int k) { return n[k] ? stage[k] / (double)n[k] : 0.0; };
-        t1[r % 2] += 0.5 * mean(ASTRAEA_STAGE_PASS1);
-        t2[r % 2] += 0.5 * mean(ASTRAEA_STAGE_PASS2);
-        td[r] = mean(ASTRAEA_STAGE_DRIFT);
+        t1[order2[r]] += 0.25 * mean(ASTRAEA_STAGE_PASS1);
+        t2[order2[r]] += 0.25 * mean(ASTRAEA_STAGE_PASS2);
+        td[order4[r]] += 0.5 * mean(ASTRAEA_STAGE_DRIFT);
     }
     const int rt = astraea_enable_stage_timing(c, wasTiming ? 1 : 0);
     if (rc != ASTRAEA_OK) {

@@ -336,9 +336,10 @@ typedef struct astraea_tuning {
 } astraea_tuning;
 int astraea_get_tuning(astraea_ctx* ctx, astraea_tuning* out);
 int astraea_set_tuning(astraea_ctx* ctx, const astraea_tuning* tuning);
-/* The autotuner: advances the loaded state by 4 * steps_per_variant eager steps, timing every
- * variant of each pass with the stage events, and keeps the fastest for each pass.  The
- * simulation does not depend on the variant, so these are ordinary steps.  `chosen` and `ms`
+/* The autotuner: advances the loaded state by 8 * steps_per_variant eager steps, timing every
+ * variant of each pass with the stage events (in palindromic orders, so that the state's slow
+ * change of cost cancels), and keeps the fastest for each pass.  The simulation does not depend
+ * on the variant, so these are ordinary steps.  `chosen` and `ms`
  * (mean milliseconds per step: pass 1 at 128 and 256, pass 2 at 64 and 128, drift at 64, 128,
  * 192 and 256; 8 values) may be NULL.  On an error the previous tuning is restored. */
 int astraea_autotune(astraea_ctx* ctx, float max_dt_psu, uint32_t steps_per_variant, astraea_tuning* chosen,

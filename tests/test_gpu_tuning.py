@@ -60,7 +60,7 @@ def test_autotune_steps_are_ordinary_steps(lib):
     assert tuned.tuning() == r["chosen"]
     assert all(v > 0.0 for d in ("pass1_ms", "pass2_ms", "drift_ms") for v in r[d].values())
     plain = _context(lib, st, False)
-    plain.step(MAX_DT, 4, graph=False)
+    plain.step(MAX_DT, 8, graph=False)
     a, b = _state(tuned), _state(plain)
     for k in FIELDS:
         assert np.array_equal(a[k], b[k]), k
