@@ -83,7 +83,8 @@ class Params(C.Structure):
         ("cooling_min_temp_k", C.c_float),
         ("strength_mask", C.c_uint32),
         ("precision", C.c_uint32),
-        ("reserved", C.c_uint32 * 5),
+        ("group_size", C.c_uint32),
+        ("reserved", C.c_uint32 * 4),
     ]
 
 

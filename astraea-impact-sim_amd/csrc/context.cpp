@@ -196,6 +196,7 @@ Params toDeviceParams(const astraea_params& p) {
     d.gammaHHe = p.gamma_hhe;
     d.coolMinT = p.cooling_min_temp_k;
     d.strengthMask = p.strength_mask & 0x1fu;
+    d.group = p.group_size == 32u ? 32u : 64u;
     return d;
 }
 
@@ -612,6 +613,8 @@ int astraea_set_params(astraea_ctx* c, const astraea_params* p) {
         return fail(ASTRAEA_E_INVALID, "theta, soft_factor must be >= 0 and cfl_factor > 0");
     if (p->precision != ASTRAEA_PREC_FP32 && p->precision != ASTRAEA_PREC_FP64_ACCUM)
         return fail(ASTRAEA_E_INVALID, "unknown precision mode %u", p->precision);
+    if (p->group_size != 0u && p->group_size != 32u && p->group_size != 64u)
+        return fail(ASTRAEA_E_INVALID, "group_size %u is not 0, 32 or 64", p->group_size);
     const bool wasFp64 = c->params.precision == ASTRAEA_PREC_FP64_ACCUM;
     c->params = *p;
     syncDerivedFlags(c);

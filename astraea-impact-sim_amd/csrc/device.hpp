@@ -221,6 +221,7 @@ struct Params {
     float gammaHHe;
     float coolMinT;
     uint32_t strengthMask;
+    uint32_t group;       // gravity opening group: 64 (a wavefront) or 32 (lanes 0-31 of one)
 };
 
 // ---------------------------------------------------------------------------

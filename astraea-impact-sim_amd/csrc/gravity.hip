@@ -62,16 +62,21 @@ __device__ __forceinline__ float acceptThreshold(float e2, float theta2, float i
     return fmaxf(x, floor20);
 }
 
-template <bool CHECK_INSIDE>
+// GROUP particles share the opening decisions of one wavefront: 64 (all lanes), or 32, the
+// reference's warp groups, with lanes 32-63 idle.  A block covers kWavesPerBlock * GROUP slots,
+// which never straddle a segment (segments start at multiples of 256).
+template <bool CHECK_INSIDE, int GROUP>
 __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
+    static_assert(GROUP == kWave || GROUP == kWave / 2, "a group is a wavefront or half of one");
+    constexpr uint32_t kSlots = (uint32_t)(kWavesPerBlock * GROUP);
     __shared__ WaveLds lds[kWavesPerBlock];
     const uint32_t b = xcdBlock(blockIdx.x, gridDim.x);
     const int wave = threadIdx.x >> 6, lane = laneId();
-    const uint32_t i = b * kBlock + threadIdx.x;
-    const uint32_t waveBase = b * kBlock + (uint32_t)wave * kWave;
-    const uint32_t sg = uniformU32(c.seg.slotSeg[b * kBlock]);
+    const uint32_t waveBase = b * kSlots + (uint32_t)wave * GROUP;
+    const uint32_t i = waveBase + (uint32_t)lane;
+    const uint32_t sg = uniformU32(c.seg.slotSeg[b * kSlots]);
     const uint32_t off = uniformU32(c.seg.off[sg]), n = uniformU32(c.seg.n[sg]);
-    const bool active = i - off < n;
+    const bool active = lane < GROUP && i - off < n;
     const uint64_t am = ballot(active);
     if (am == 0ull) return;
     if (n == 1u) {
@@ -308,8 +313,16 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
 }  // namespace
 
 void launchGravity(const DeviceCtx& c, hipStream_t s) {
-    if (c.prm.theta2 > 0.333333f) kGravity<true><<<c.blocks, kBlock, 0, s>>>(c);
-    else kGravity<false><<<c.blocks, kBlock, 0, s>>>(c);
+    const bool inside = c.prm.theta2 > 0.333333f;
+    if (c.prm.group == 32u) {
+        // 128 slots per block: twice the blocks over the same slots
+        if (inside) kGravity<true, 32><<<2 * c.blocks, kBlock, 0, s>>>(c);
+        else kGravity<false, 32><<<2 * c.blocks, kBlock, 0, s>>>(c);
+    } else if (inside) {
+        kGravity<true, kWave><<<c.blocks, kBlock, 0, s>>>(c);
+    } else {
+        kGravity<false, kWave><<<c.blocks, kBlock, 0, s>>>(c);
+    }
 }
 
 }  // namespace astraea_hip

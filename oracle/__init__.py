@@ -47,7 +47,7 @@ class Params(C.Structure):
     _fields_ = [
         ("theta", C.c_float), ("soft_factor", C.c_float), ("cfl_factor", C.c_float), ("gamma_hhe", C.c_float),
         ("cooling_min_temp_k", C.c_float), ("strength_mask", C.c_uint32), ("precision", C.c_uint32),
-        ("reserved", C.c_uint32 * 5),
+        ("group_size", C.c_uint32), ("reserved", C.c_uint32 * 4),
     ]
 
 

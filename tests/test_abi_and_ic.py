@@ -33,6 +33,12 @@ def test_library_exports_every_declared_symbol(lib):
     assert sorted(lib.EXPORTED_SYMBOLS) == names
 
 
+def ctypes_size(t):
+    import ctypes
+
+    return ctypes.sizeof(t)
+
+
 def test_field_sizes_and_defaults(lib):
     f = lib.FIELD
     assert lib._lib.astraea_field_size(f["pos_x"]) == 4
@@ -45,6 +51,8 @@ def test_field_sizes_and_defaults(lib):
     # sph_gravity.cu:842 (0.8 h0), 1510 (0.25), eos.cu:93 (1.4), simulation.cpp:1729 (theta 0.5)
     assert (p.theta, p.cfl_factor, p.strength_mask, p.precision) == (0.5, 0.25, 0x17, 0)
     assert abs(p.soft_factor - 0.8) < 1e-7 and abs(p.gamma_hhe - 1.4) < 1e-7
+    assert p.group_size == 0 and lib.GROUP_SIZE == 64  # 0: one wavefront of 64
+    assert ctypes_size(lib.Params) == 48 == ctypes_size(__import__("oracle").Params)
 
 
 def test_context_without_device_reports_error(lib):

@@ -46,7 +46,7 @@ def run_pair(lib, orc, state, counts, params, stages):
     ctx.set_params(params)
     ctx.set_layout(counts)
     ctx.load_state(state)
-    o = orc.Oracle(params, group_size=lib.GROUP_SIZE)
+    o = orc.Oracle(params, group_size=params.group_size or lib.GROUP_SIZE)
     o.set_layout(counts)
     o.load_state(state)
     for s in stages:
@@ -242,6 +242,35 @@ def test_gravity_parity_other_theta(lib, orc, theta):
     assert rel_rms(ag, ar) < 1e-5
     pg, pr = by_id(ctx, o, "phi", f64=True)
     assert rel_rms(pg, pr) < 1e-5
+
+
+@pytest.mark.parametrize("theta", [0.5, 0.7])
+def test_gravity_parity_reference_warp_groups(lib, orc, theta):
+    """group_size 32: the opening decision is shared by 32 particles, as by the reference's
+    warps (sph_gravity.cu:932-933).  The interaction count equals the oracle's at group 32 and
+    is below the count at group 64 (a node accepted by 64 particles is accepted by each half);
+    the accelerations are within 1e-5 of FP64.  Three segments, so that groups start at segment
+    offsets; theta 0.7 applies the AABB-inside test."""
+    from astraea_amd import scenario as S
+
+    specs = S.sweep_scenarios(count=3, n_min=2000, n_max=6000)
+    st, counts = S.build_sweep(specs)
+    stages = [("sort_and_build",), ("gravity",)]
+    ctx, o = run_pair(lib, orc, st, counts, S.plain_params(theta=theta, group_size=32), stages)
+    cg, co = ctx.counters(), o.counters()
+    assert o.group_size == 32 and cg["interactions"] == co["interactions"], (cg, co)
+    assert cg["forced_accepts"] == co["forced_accepts"] == 0
+    ag, ar = vec_by_id(ctx, o, ["acc_grav_x", "acc_grav_y", "acc_grav_z"])
+    assert rel_rms(ag, ar) < 1e-5
+    pg, pr = by_id(ctx, o, "phi", f64=True)
+    assert rel_rms(pg, pr) < 1e-5
+    assert ctx.params().group_size == 32
+    wide, _ = run_pair(lib, orc, st, counts, S.plain_params(theta=theta), stages[:1])
+    wide.gravity()
+    wide.synchronize()
+    assert cg["interactions"] < wide.counters()["interactions"]
+    with pytest.raises(lib.AstraeaError):
+        ctx.set_params(S.plain_params(group_size=16))
 
 
 def test_force_parity_segments(lib, orc):
