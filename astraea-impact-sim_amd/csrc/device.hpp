@@ -246,6 +246,14 @@ __device__ __forceinline__ uint32_t xcdBlock(uint32_t b, uint32_t nb) {
 
 // Squared distance, ((dx*dx) + dy*dy) + dz*dz as two FMAs: the operation order every
 // neighbour and opening decision uses (reproduced by the oracle's FP32 replica).
+// max(x, c) for c > 0 (or x, c both >= 0), as an integer max of the bit patterns: the bits of
+// non-negative floats order like the floats, and a negative x loses to c either way.  fmaxf on a
+// value loaded from memory costs an extra canonicalising instruction first; this does not (NaN
+// inputs are the only difference).
+__device__ __forceinline__ float maxPos(float x, float c) {
+    return __int_as_float(max(__float_as_int(x), __float_as_int(c)));
+}
+
 __device__ __forceinline__ float dist2f(float dx, float dy, float dz) {
 #pragma clang fp contract(off)
     float t = dx * dx;

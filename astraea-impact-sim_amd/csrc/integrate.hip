@@ -197,14 +197,14 @@ __global__ __launch_bounds__(kBlock) void kDrift(DeviceCtx c) {
         },
         [&](const Rec& n) {
             const float4 pj = n.p, vj = n.v;
-            const float Hj = 2.0f * fmaxf(n.h, 1.0e-6f);
+            const float Hj = 2.0f * maxPos(n.h, 1.0e-6f);
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float r2 = dist2f(dx, dy, dz);
             const float Hm = fmaxf(Hi, Hj);
             if (!(r2 < Hm * Hm)) return;
             const float r = fastSqrt(r2);
             const float wb = 0.5f * (wendlandS(r, sI) + wendlandS(r, supportOf(Hj)));
-            const float rb = 0.5f * (rhoI + fmaxf(vj.w, 1.0e-9f));
+            const float rb = 0.5f * (rhoI + maxPos(vj.w, 1.0e-9f));
             const float co = (pj.w * fastRcp(rb)) * wb;
             xs += co * (vj.x - vi.x);
             ys += co * (vj.y - vi.y);

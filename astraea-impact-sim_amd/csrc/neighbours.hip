@@ -147,7 +147,7 @@ __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
                 for (int u = 0; u < 32; ++u) {
                     const float4 q = L.cand[c0 + u];
                     const float r2 = dist2f(pi.x - q.x, pi.y - q.y, pi.z - q.z);
-                    const bool hit = r2 < Hb2 || r2 < q.w;
+                    const bool hit = r2 < maxPos(Hb2, q.w);  // both are squares
                     m |= hit ? (1u << u) : 0u;
                     any |= (ballot(hit) & actMask) != 0ull ? (1u << u) : 0u;
                 }

@@ -83,7 +83,7 @@ __global__ __launch_bounds__(kBlock) void kPass1(DeviceCtx c) {
             if (!(r2 < Hi2)) return;
             ++ngb;
             const float r = fastSqrt(r2);
-            const float volJ = pj.w * fastRcp(fmaxf(vj.w, 1.0e-9f));
+            const float volJ = pj.w * fastRcp(maxPos(vj.w, 1.0e-9f));
             const float w = wendlandS(r, sI);
             const float vw = volJ * w;
             shep += vw;
@@ -230,13 +230,13 @@ __global__ __launch_bounds__(kBlock) void kPass2(DeviceCtx c) {
         },
         [&](const Rec& n) {
             const float4 pj = n.p, nbj = n.n, vj = n.v;
-            const float hj = fmaxf(nbj.x, 1.0e-6f), Hj = 2.0f * hj;
+            const float hj = maxPos(nbj.x, 1.0e-6f), Hj = 2.0f * hj;
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float r2 = dist2f(dx, dy, dz);
             const float Hm = fmaxf(Hi, Hj);
             if (!(r2 < Hm * Hm)) return;
             ++pairs;
-            const float mj = pj.w, rhoJ = fmaxf(vj.w, 1.0e-9f);
+            const float mj = pj.w, rhoJ = maxPos(vj.w, 1.0e-9f);
             const float pr2J = nbj.y, csJ = nbj.z, fBalJ = nbj.w;
             const float dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
             const float vDotR = dvx * dx + dvy * dy + dvz * dz;
