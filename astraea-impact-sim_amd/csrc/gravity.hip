@@ -28,6 +28,8 @@ namespace astraea_hip {
 namespace {
 
 constexpr int kSCap = 1024;                 // per-wave node stack
+constexpr int kSub = 4;                     // sub-groups of a wavefront with boxes of their own
+constexpr int kSubLanes = kWave / kSub;
 constexpr int kFlushAt = 32;                // evaluate once either list holds more than this
 constexpr int kRCap = kFlushAt + 2 * kWave; // one round lists at most 128 records
 constexpr int kPend = kRCap - kWave;        // undecided nodes of one side, staged past the records
@@ -98,22 +100,22 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
     const float e2Node = __builtin_fmaf(epsI, epsI, 1.0e-20f);
     L.self[lane] = 0u;
 
-    // bounding boxes of the four sub-groups of 16 lanes over their active particles (a sub-group
+    // bounding boxes of the kSub sub-groups of kSubLanes lanes over their active particles (a sub-group
     // without one gets an inverted box: every node is apart from it and accepted by it)
-    float bl[4][3], bh[4][3];
+    float bl[kSub][3], bh[kSub][3];
     {
         float v[6] = {active ? pi.x : 3.0e38f, active ? pi.y : 3.0e38f, active ? pi.z : 3.0e38f,
                       active ? pi.x : -3.0e38f, active ? pi.y : -3.0e38f, active ? pi.z : -3.0e38f};
-        for (int o = 1; o < 16; o <<= 1) {
+        for (int o = 1; o < kSubLanes; o <<= 1) {
             for (int a = 0; a < 3; ++a) v[a] = fminf(v[a], __shfl_xor(v[a], o, 64));
             for (int a = 3; a < 6; ++a) v[a] = fmaxf(v[a], __shfl_xor(v[a], o, 64));
         }
 #pragma unroll
-        for (int g = 0; g < 4; ++g) {
+        for (int g = 0; g < kSub; ++g) {
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                bl[g][a] = readLaneF(v[a], 16 * g);
-                bh[g][a] = readLaneF(v[3 + a], 16 * g);
+                bl[g][a] = readLaneF(v[a], kSubLanes * g);
+                bh[g][a] = readLaneF(v[3 + a], kSubLanes * g);
             }
         }
     }
@@ -159,7 +161,7 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
                     // surely rejects it (all its particles fail), and tested exactly otherwise.
                     bool allPass = true, oneFails = false;
 #pragma unroll
-                    for (int g = 0; g < 4; ++g) {
+                    for (int g = 0; g < kSub; ++g) {
                         const float ux = bl[g][0] - cm.x, vx = cm.x - bh[g][0];
                         const float uy = bl[g][1] - cm.y, vy = cm.y - bh[g][1];
                         const float uz = bl[g][2] - cm.z, vz = cm.z - bh[g][2];
