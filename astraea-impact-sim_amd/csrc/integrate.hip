@@ -171,7 +171,7 @@ template <bool FP64, int S>
 __global__ __launch_bounds__(kBlock) void kDrift(DeviceCtx c) {
     struct Rec {
         float4 p, v;
-        float h;
+        float H, invH, wNorm;  // H_j = 2 max(h_j, 1e-6) and its kernel constants, once per record
     };
     __shared__ float sb[kWavesPerBlock][6];
     __shared__ Rec stageBuf[kWavesPerBlock][S];
@@ -193,17 +193,20 @@ __global__ __launch_bounds__(kBlock) void kDrift(DeviceCtx c) {
         [&](uint32_t j, Rec& r) {
             r.p = c.cur.pos[j];
             r.v = c.cur.vel[j];
-            r.h = c.d.nb[j].x;
+            r.H = 2.0f * maxPos(c.d.nb[j].x, 1.0e-6f);
+            const Support sj = supportOf(r.H);
+            r.invH = sj.invH;
+            r.wNorm = sj.wNorm;
         },
         [&](const Rec& n) {
             const float4 pj = n.p, vj = n.v;
-            const float Hj = 2.0f * maxPos(n.h, 1.0e-6f);
+            const float Hj = n.H;
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float r2 = dist2f(dx, dy, dz);
-            const float Hm = fmaxf(Hi, Hj);
+            const float Hm = maxPos(Hi, Hj);  // both positive
             if (!(r2 < Hm * Hm)) return;
             const float r = fastSqrt(r2);
-            const float wb = 0.5f * (wendlandS(r, sI) + wendlandS(r, supportOf(Hj)));
+            const float wb = 0.5f * (wendlandS(r, sI) + wendlandS(r, Support{n.invH, n.wNorm, 0.0f}));
             const float rb = 0.5f * (rhoI + maxPos(vj.w, 1.0e-9f));
             const float co = (pj.w * fastRcp(rb)) * wb;
             xs += co * (vj.x - vi.x);

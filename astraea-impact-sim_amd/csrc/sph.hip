@@ -165,8 +165,9 @@ __global__ __launch_bounds__(kBlock) void kEos(DeviceCtx c) {
 // smaller record lets five blocks share a CU instead of four).
 template <bool STRENGTH>
 struct Pass2Rec {
-    float4 p, n, v;
+    float4 p, n, v;  // n.x holds max(h_j, 1e-6)
     uint32_t mat;
+    float invH, gNorm;  // the kernel gradient's constants of H_j, once per staged record
 };
 template <>
 struct Pass2Rec<true> {
@@ -226,11 +227,16 @@ __global__ __launch_bounds__(kBlock) void kPass2(DeviceCtx c) {
             if constexpr (STRENGTH) {
                 r.s0 = c.d.sr0[j];
                 r.s1 = c.d.sr1[j];
+            } else {
+                r.n.x = maxPos(r.n.x, 1.0e-6f);
+                const Support sj = supportOf(2.0f * r.n.x);
+                r.invH = sj.invH;
+                r.gNorm = sj.gNorm;
             }
         },
         [&](const Rec& n) {
             const float4 pj = n.p, nbj = n.n, vj = n.v;
-            const float hj = maxPos(nbj.x, 1.0e-6f), Hj = 2.0f * hj;
+            const float hj = STRENGTH ? maxPos(nbj.x, 1.0e-6f) : nbj.x, Hj = 2.0f * hj;
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float r2 = dist2f(dx, dy, dz);
             const float Hm = fmaxf(Hi, Hj);
@@ -241,7 +247,10 @@ __global__ __launch_bounds__(kBlock) void kPass2(DeviceCtx c) {
             const float dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
             const float vDotR = dvx * dx + dvy * dy + dvz * dz;
             const float r = fastSqrt(r2);
-            const float gI = wendlandGradS(r, sI), gJ = wendlandGradS(r, supportOf(Hj));
+            Support sJ;
+            if constexpr (STRENGTH) sJ = supportOf(Hj);
+            else sJ = Support{n.invH, 0.0f, n.gNorm};
+            const float gI = wendlandGradS(r, sI), gJ = wendlandGradS(r, sJ);
             const float gb = 0.5f * (gI + gJ);
             const float gvx = gb * dx, gvy = gb * dy, gvz = gb * dz;
             const float vdg = dvx * gvx + dvy * gvy + dvz * gvz;
