@@ -121,10 +121,29 @@ __device__ __forceinline__ void waveSync() {
 // coalesced loads; each lane consumes its entries that fall in the round, pair(rec) in table
 // order, reading the next entry's record from LDS before it runs pair on the current one.
 // Past the end of a bank a round stages copies of its last slot (never read).
+//
+// The records sit in LDS quad-major: 16-byte quad q of record k at quad q * S + k of the buffer
+// (the same S * sizeof(Rec) bytes).  Lanes read records at scattered positions; with records
+// stored whole, a record of 2 or 4 quads would start on only 8 or 4 of the 16 four-bank groups,
+// while quad-major rows spread the lanes over all 16.
+template <int S, class Rec>
+__device__ __forceinline__ Rec ldsRecord(const float4* q, uint32_t k) {
+    constexpr int NQ = (int)(sizeof(Rec) / 16);
+    float4 t[NQ];
+#pragma unroll
+    for (int x = 0; x < NQ; ++x) t[x] = q[x * S + k];
+    Rec r;
+    __builtin_memcpy(&r, t, sizeof(Rec));
+    return r;
+}
+
 template <int S, class Rec, class Stage, class Pair>
 __device__ __forceinline__ void forEachNeighbour(const Neighbours& nb, uint32_t i, Rec* buf, Stage&& stage,
                                                  Pair&& pair) {
     static_assert(S % 64 == 0, "a round is whole wavefront loads");
+    static_assert(sizeof(Rec) % 16 == 0 && alignof(Rec) >= 16, "records are whole 16-byte quads");
+    constexpr int NQ = (int)(sizeof(Rec) / 16);
+    float4* quads = reinterpret_cast<float4*>(buf);
     constexpr int R = S / 64;
     constexpr uint32_t kPast = 0xffffffffu;  // an entry position beyond every round
     const uint32_t w = i >> 6, lane = i & 63u;
@@ -166,19 +185,26 @@ __device__ __forceinline__ void forEachNeighbour(const Neighbours& nb, uint32_t 
             for (int r = 0; r < R; ++r) id[r] = U[min(s0 + (uint32_t)(64 * r) + lane, len - 1u)];
             waveSync();
 #pragma unroll
-            for (int r = 0; r < R; ++r) stage(id[r], buf[64 * r + lane]);
+            for (int r = 0; r < R; ++r) {
+                Rec rec;
+                stage(id[r], rec);
+                float4 t[NQ];
+                __builtin_memcpy(t, &rec, sizeof(Rec));
+#pragma unroll
+                for (int x = 0; x < NQ; ++x) quads[x * S + 64 * r + lane] = t[x];
+            }
             waveSync();
             // two records alternate (a holds entry k when the loop body starts), so that a record
             // read ahead from LDS never has to be copied into the other's registers
-            Rec a = buf[min(e - s0, (uint32_t)S - 1u)];
+            Rec a = ldsRecord<S, Rec>(quads, min(e - s0, (uint32_t)S - 1u));
             while (e < s1) {
                 uint32_t en = advance();
-                const Rec b = buf[min(en - s0, (uint32_t)S - 1u)];
+                const Rec b = ldsRecord<S, Rec>(quads, min(en - s0, (uint32_t)S - 1u));
                 pair(a);
                 e = en;
                 if (e >= s1) break;
                 en = advance();
-                a = buf[min(en - s0, (uint32_t)S - 1u)];
+                a = ldsRecord<S, Rec>(quads, min(en - s0, (uint32_t)S - 1u));
                 pair(b);
                 e = en;
             }
