@@ -183,6 +183,15 @@ def test_force_parity_small_config(lib, orc):
     print(err)
 
 
+def test_force_parity_reference_warp_groups(lib, orc):
+    """Initial forces with group_size 32 (the reference's warp groups, sph_gravity.cu:932-933):
+    every stage against the oracle at group 32, with the same integer and FP bars."""
+    from astraea_amd import scenario as S
+
+    st = perturbed_small_state(seed=11)
+    force_parity(lib, orc, st, [len(st["mass"])], S.plain_params(group_size=32), FORCE_TOL)
+
+
 def test_force_parity_strength(lib, orc):
     from astraea_amd import scenario as S
 
