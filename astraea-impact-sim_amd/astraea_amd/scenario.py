@@ -57,8 +57,9 @@ def impact_pair(n_target: int, n_impactor: int, angle_deg: float, v_ratio: float
 
 
 def plain_params(**kw):
-    """Benchmark physics: Plummer softening = h, CFL 0.3, strength and damage off."""
-    base = dict(soft_factor=1.0, cfl_factor=0.3, strength_mask=0, precision=PREC_FP32)
+    """Benchmark physics: Plummer softening = h, CFL 0.3, strength and damage off, gravity
+    opening decisions shared by a whole wavefront of 64 particles."""
+    base = dict(soft_factor=1.0, cfl_factor=0.3, strength_mask=0, precision=PREC_FP32, group_size=64)
     base.update(kw)
     return default_params(**base)
 

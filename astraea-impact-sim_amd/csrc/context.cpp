@@ -501,6 +501,7 @@ void astraea_default_params(astraea_params* p) {
     p->cooling_min_temp_k = 0.0f;
     p->strength_mask = 0x17u;
     p->precision = ASTRAEA_PREC_FP32;
+    p->group_size = 32u;  // the reference's warp groups
 }
 
 int astraea_create(astraea_ctx** out, int device, uint32_t capacity, uint32_t maxSegments) {

@@ -155,7 +155,7 @@ def cpu_baseline(states, counts, params, min_particles):
         n += counts[k]
         k += 1
     merged = {f: np.ascontiguousarray(np.concatenate([s[f] for s in states[:k]])) for f in states[0]}
-    o = oracle.Oracle(params, group_size=64)
+    o = oracle.Oracle(params, group_size=params.group_size or 64)
     o.set_layout(counts[:k])
     o.load_state(merged)
     o.initial_forces()
@@ -192,7 +192,7 @@ def drift_vs_oracle(A, steps, specs):
     ctx.step(MAX_DT, steps, graph=True)
     ctx.synchronize()
     g1 = energy_total(ctx.energy())
-    o = oracle.Oracle(params, group_size=64)
+    o = oracle.Oracle(params, group_size=params.group_size or 64)
     o.set_layout(counts)
     o.load_state(state)
     o.initial_forces()

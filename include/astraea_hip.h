@@ -55,17 +55,17 @@ typedef struct astraea_params {
     uint32_t strength_mask;   /* bit m: material m carries deviatoric stress and damage;
                                  reference: every material with mu > 0 (sph_gravity.cu:42-73) = 0x17 */
     uint32_t precision;       /* ASTRAEA_PREC_FP32 (reference layout) or ASTRAEA_PREC_FP64_ACCUM */
-    uint32_t group_size;      /* particles sharing one gravity opening decision: 0 or 64 = one
-                                 wavefront (the default); 32 = the reference's warp groups
-                                 (sph_gravity.cu:932-933), reproducing its interaction lists with
-                                 lanes 32-63 of each wavefront idle in the traversal */
+    uint32_t group_size;      /* particles sharing one gravity opening decision: 32 = the
+                                 reference's warp groups (sph_gravity.cu:932-933), its interaction
+                                 lists, lanes 32-63 of each wavefront idle in the traversal (the
+                                 default); 0 or 64 = one wavefront, fewer traversals */
     uint32_t reserved[4];
 } astraea_params;
 
 #define ASTRAEA_PREC_FP32 0u       /* FP32 positions and velocities, as the reference */
 #define ASTRAEA_PREC_FP64_ACCUM 1u /* FP64 master positions/velocities for kick and drift */
 
-/* Default traversal group: the gravity opening decision is shared by one wavefront. */
+/* Traversal group of group_size 0: the gravity opening decision is shared by one wavefront. */
 #define ASTRAEA_GROUP_SIZE 64
 /* Each segment starts at a multiple of this many particle slots. */
 #define ASTRAEA_SEGMENT_ALIGN 256

@@ -51,7 +51,7 @@ def test_field_sizes_and_defaults(lib):
     # sph_gravity.cu:842 (0.8 h0), 1510 (0.25), eos.cu:93 (1.4), simulation.cpp:1729 (theta 0.5)
     assert (p.theta, p.cfl_factor, p.strength_mask, p.precision) == (0.5, 0.25, 0x17, 0)
     assert abs(p.soft_factor - 0.8) < 1e-7 and abs(p.gamma_hhe - 1.4) < 1e-7
-    assert p.group_size == 0 and lib.GROUP_SIZE == 64  # 0: one wavefront of 64
+    assert p.group_size == 32 and lib.GROUP_SIZE == 64  # the reference's warps; 0 means one wavefront
     assert ctypes_size(lib.Params) == 48 == ctypes_size(__import__("oracle").Params)
 
 

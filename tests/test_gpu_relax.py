@@ -57,7 +57,7 @@ def test_relax_two_bodies_matches_oracle(lib, orc, precision, anchored):
     anchor = (400.0 + 3.0 * (np.arange(n) % 11)) if anchored else None
     ctx = _ctx(lib, st, counts, p)
     g = ctx.relax(6, radii, anchor_temp=anchor)
-    o = orc.Oracle(p, group_size=lib.GROUP_SIZE)
+    o = orc.Oracle(p, group_size=p.group_size or lib.GROUP_SIZE)
     o.set_layout(counts)
     o.load_state(st)
     r = o.relax(6, radii, anchor_temp=anchor)
