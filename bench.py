@@ -277,6 +277,7 @@ def main():
             "theta": 0.5,
             "softening": "h",
             "cfl": 0.3,
+            "gravity_group": 64,
             "parallelism": "single GPU",
         },
         "energy_drift_rel": {"max_abs": float(np.max(np.abs(drift))), "median_abs": float(np.median(np.abs(drift))),
