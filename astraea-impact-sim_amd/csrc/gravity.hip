@@ -90,11 +90,15 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
     const float invTheta2 = theta2 > 0.0f ? 1.0f / theta2 : 0.0f;
     const bool pow2 = c.prm.theta2Pow2 != 0u;
     const float soft = c.prm.softFactor;
+    // the particle a lane evaluates: its own; at GROUP 32, lanes 32-63 take their partner's
+    // (lane - 32) and every other record, and the two partial sums are added at the end.  The
+    // walk's decisions use the group's lanes only (`active`, `am`).
+    const uint32_t ie = waveBase + (uint32_t)(lane % GROUP);
     float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float epsI = 1.0e-6f;
-    if (active) {
-        pi = c.cur.pos[i];
-        epsI = soft * fmaxf(c.cur.hu[i].y, 1.0e-6f);
+    if (ie - off < n) {
+        pi = c.cur.pos[ie];
+        epsI = soft * fmaxf(c.cur.hu[ie].y, 1.0e-6f);
     }
     const float halfEpsI = 0.5f * epsI;
     const float e2Node = __builtin_fmaf(epsI, epsI, 1.0e-20f);
@@ -123,6 +127,8 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
     float ax = 0.0f, ay = 0.0f, az = 0.0f, phi = 0.0f;
     uint32_t listed = 0, forced = 0;
     const uint64_t lt = (1ull << lane) - 1ull;
+    constexpr int kStride = kWave / GROUP;  // records a lane steps over in the evaluation
+    const int e0 = GROUP == kWave ? 0 : lane / GROUP;
 
     int sp = 1, nn = 0, nf = 0;
     L.stack[0] = (int)off;
@@ -266,7 +272,7 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
         // ---- evaluate the staged records ----
         listed += (uint32_t)(nn + nf);
 #pragma unroll 4
-        for (int e = 0; e < nn; ++e) {
+        for (int e = e0; e < nn; e += kStride) {
             const float4 q = L.node[e];
             const float dx = pi.x - q.x, dy = pi.y - q.y, dz = pi.z - q.z;
             const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, e2Node)));
@@ -279,7 +285,7 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
             phi -= t;
         }
 #pragma unroll 4
-        for (int e = 0; e < nf; ++e) {
+        for (int e = e0; e < nf; e += kStride) {
             const float4 q = L.leaf[e];
             const float eij = halfEpsI + L.leafEps[e];
             const float dx = pi.x - q.x, dy = pi.y - q.y, dz = pi.z - q.z;
@@ -300,6 +306,12 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
     }
     // the particle's own leaf: zero separation, so only its potential term is taken back
     // (the same expression as in the loop: eps_ii = eps_i)
+    if (GROUP < kWave) {
+        ax += __shfl_xor(ax, GROUP, kWave);
+        ay += __shfl_xor(ay, GROUP, kWave);
+        az += __shfl_xor(az, GROUP, kWave);
+        phi += __shfl_xor(phi, GROUP, kWave);
+    }
     const bool selfListed = L.self[lane] != 0u;
     if (active && selfListed) phi += (kGf * pi.w) * __builtin_amdgcn_rsqf(__builtin_fmaf(epsI, epsI, 1.0e-20f));
     if (active) c.d.grav[i] = make_float4(ax, ay, az, phi);

@@ -57,8 +57,8 @@ typedef struct astraea_params {
     uint32_t precision;       /* ASTRAEA_PREC_FP32 (reference layout) or ASTRAEA_PREC_FP64_ACCUM */
     uint32_t group_size;      /* particles sharing one gravity opening decision: 32 = the
                                  reference's warp groups (sph_gravity.cu:932-933), its interaction
-                                 lists, lanes 32-63 of each wavefront idle in the traversal (the
-                                 default); 0 or 64 = one wavefront, fewer traversals */
+                                 lists (the default); 0 or 64 = one wavefront, half the
+                                 traversals, more interactions */
     uint32_t reserved[4];
 } astraea_params;
 
