@@ -221,17 +221,18 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
                         if ((ballot(fail) & am) == 0ull) okMask |= 1ull << u;
                     }
                 } else {
-                    // four nodes per iteration: their LDS reads overlap; bits past the pending
-                    // count are never read
+                    // four nodes per iteration (eight at GROUP 32: lanes 32-63 test the odd ones
+                    // for their partners' particles): their LDS reads overlap; bits past the
+                    // pending count are never read
                     const int nu = __popcll(um);
-                    for (int u = 0; u < nu; u += 4) {
+                    for (int u = 0; u < nu; u += 4 * kStride) {
                         float4 q[4];
 #pragma unroll
                         for (int v = 0; v < 4; ++v) {
-                            const int at = kPend + u + v;  // slots past nu hold stale records
+                            const int at = kPend + u + kStride * v + e0;  // slots past nu hold stale records
                             q[v] = L.node[at < kRCap ? at : kRCap - 1];
                         }
-                        uint32_t nib = 0u;
+                        uint32_t bits = 0u;
 #pragma unroll
                         for (int v = 0; v < 4; ++v) {
                             bool fail;
@@ -240,9 +241,15 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
                                 const float r2 = dist2f(pi.x - q[v].x, pi.y - q[v].y, pi.z - q[v].z);
                                 fail = !(r2 >= q[v].w);
                             }
-                            nib |= ((ballot(fail) & am) == 0ull ? 1u : 0u) << v;
+                            const uint64_t b = ballot(fail);
+                            if constexpr (GROUP == kWave) {
+                                bits |= ((b & am) == 0ull ? 1u : 0u) << v;
+                            } else {
+                                bits |= ((b & am) == 0ull ? 1u : 0u) << (2 * v);
+                                bits |= (((b >> GROUP) & am) == 0ull ? 1u : 0u) << (2 * v + 1);
+                            }
                         }
-                        okMask |= (uint64_t)nib << u;
+                        okMask |= (uint64_t)bits << u;
                     }
                 }
                 if (undecided) accept = ((okMask >> myPend) & 1ull) != 0ull;
