@@ -70,6 +70,7 @@ __device__ __forceinline__ float acceptThreshold(float e2, float theta2, float i
 template <bool CHECK_INSIDE, int GROUP>
 __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
     static_assert(GROUP == kWave || GROUP == kWave / 2, "a group is a wavefront or half of one");
+    constexpr int kSubG = kSub * GROUP / kWave;  // the sub-groups that hold the group's lanes
     constexpr uint32_t kSlots = (uint32_t)(kWavesPerBlock * GROUP);
     __shared__ WaveLds lds[kWavesPerBlock];
     const uint32_t b = xcdBlock(blockIdx.x, gridDim.x);
@@ -115,7 +116,7 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
             for (int a = 3; a < 6; ++a) v[a] = fmaxf(v[a], __shfl_xor(v[a], o, 64));
         }
 #pragma unroll
-        for (int g = 0; g < kSub; ++g) {
+        for (int g = 0; g < kSubG; ++g) {
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 bl[g][a] = readLaneF(v[a], kSubLanes * g);
@@ -167,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
                     // surely rejects it (all its particles fail), and tested exactly otherwise.
                     bool allPass = true, oneFails = false;
 #pragma unroll
-                    for (int g = 0; g < kSub; ++g) {
+                    for (int g = 0; g < kSubG; ++g) {
                         const float ux = bl[g][0] - cm.x, vx = cm.x - bh[g][0];
                         const float uy = bl[g][1] - cm.y, vy = cm.y - bh[g][1];
                         const float uz = bl[g][2] - cm.z, vz = cm.z - bh[g][2];
