@@ -65,8 +65,9 @@ __device__ __forceinline__ float acceptThreshold(float e2, float theta2, float i
 }
 
 // GROUP particles share the opening decisions of one wavefront: 64 (all lanes), or 32, the
-// reference's warp groups, with lanes 32-63 idle.  A block covers kWavesPerBlock * GROUP slots,
-// which never straddle a segment (segments start at multiples of 256).
+// reference's warp groups, decided by lanes 0-31 while lanes 32-63 help with their partners'
+// exact tests and interactions.  A block covers kWavesPerBlock * GROUP slots, which never
+// straddle a segment (segments start at multiples of 256).
 template <bool CHECK_INSIDE, int GROUP>
 __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
     static_assert(GROUP == kWave || GROUP == kWave / 2, "a group is a wavefront or half of one");
@@ -105,8 +106,8 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
     const float e2Node = __builtin_fmaf(epsI, epsI, 1.0e-20f);
     L.self[lane] = 0u;
 
-    // bounding boxes of the kSub sub-groups of kSubLanes lanes over their active particles (a sub-group
-    // without one gets an inverted box: every node is apart from it and accepted by it)
+    // bounding boxes of the kSubG sub-groups of kSubLanes lanes over their active particles (a
+    // sub-group without one gets an inverted box: every node is apart from it and accepted by it)
     float bl[kSub][3], bh[kSub][3];
     {
         float v[6] = {active ? pi.x : 3.0e38f, active ? pi.y : 3.0e38f, active ? pi.z : 3.0e38f,
