@@ -52,6 +52,9 @@ def test_field_sizes_and_defaults(lib):
     assert (p.theta, p.cfl_factor, p.strength_mask, p.precision) == (0.5, 0.25, 0x17, 0)
     assert abs(p.soft_factor - 0.8) < 1e-7 and abs(p.gamma_hhe - 1.4) < 1e-7
     assert p.group_size == 32 and lib.GROUP_SIZE == 64  # the reference's warps; 0 means one wavefront
+    from astraea_amd import scenario as S
+
+    assert S.plain_params().group_size == 64  # the benchmark configurations: one wavefront
     assert ctypes_size(lib.Params) == 48 == ctypes_size(__import__("oracle").Params)
 
 
