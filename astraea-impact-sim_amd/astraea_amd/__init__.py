@@ -387,10 +387,10 @@ class Context:
         return out
 
     def counters(self) -> dict:
-        out = (C.c_uint64 * 7)()
-        _check(_lib.astraea_counters(self._h, out, 7))
+        out = (C.c_uint64 * 8)()
+        _check(_lib.astraea_counters(self._h, out, 8))
         keys = ["interactions", "forced_accepts", "nbr_entries", "nbr_stride", "pass1_pairs", "pass2_pairs",
-                "nbr_max"]
+                "nbr_max", "steps_redone"]
         return dict(zip(keys, [int(v) for v in out]))
 
     def set_neighbour_stride(self, stride: int) -> None:

@@ -92,6 +92,12 @@ struct astraea_ctx {
     uint32_t tracerCount = 0, tracerCap = 0;
     bool loaded = false, treeValid = false, listValid = false, dtRawValid = false, posNewPending = false;
     int parity = 0;  // number of cur/alt swaps mod 2
+    // steps queued on the stream: kStepEnd launches since the layout (the device counts the ones
+    // that completed, Counters kCntSteps), and the dt_max of every astraea_step step not yet
+    // settled, so that steps held by an incomplete neighbour table can be run again
+    uint64_t stepsQueued = 0;
+    std::vector<float> unsettled;
+    uint64_t stepsRedone = 0;      // held steps finished or run again after a table was widened
     // graph
     hipGraphExec_t graph = nullptr;
     float graphMaxDt = -1.0f;
@@ -464,6 +470,45 @@ void enqueueStep(astraea_ctx* c, float maxDt, bool timed) {
     c->dtRawValid = true;
 }
 
+// Queues `steps` full steps, eagerly or (pairs of steps) through the captured graph, recording
+// them for settle.
+int queueSteps(astraea_ctx* c, float maxDt, uint32_t steps, bool useGraph) {
+    uint32_t left = steps;
+    if (useGraph && !c->timing) {
+        if (c->graph && (c->graphMaxDt != maxDt || c->graphParity != c->parity)) freeGraph(c);
+        if (left >= 2 && !c->graph) {
+            hipGraph_t g = nullptr;
+            HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            const int parity0 = c->parity;
+            enqueueStep(c, maxDt, false);
+            enqueueStep(c, maxDt, false);
+            const hipError_t e = hipStreamEndCapture(c->stream, &g);
+            if (e != hipSuccess) return fail(ASTRAEA_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+            const hipError_t ei = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) return fail(ASTRAEA_E_HIP, "graph instantiate failed: %s", hipGetErrorString(ei));
+            c->graphMaxDt = maxDt;
+            c->graphParity = parity0;
+            // the capture recorded two steps without running them; the two swaps restored cur/alt
+            c->parity = parity0;
+        }
+        while (left >= 2 && c->graph) {
+            HIP_TRY(hipGraphLaunch(c->graph, c->stream));
+            c->stepsQueued += 2;
+            c->unsettled.insert(c->unsettled.end(), 2, maxDt);
+            left -= 2;
+        }
+    }
+    while (left > 0) {
+        enqueueStep(c, maxDt, c->timing);
+        c->stepsQueued += 1;
+        c->unsettled.push_back(maxDt);
+        --left;
+    }
+    c->dtRawValid = true;
+    return ASTRAEA_OK;
+}
+
 int requireLoaded(astraea_ctx* c) {
     if (!c) return fail(ASTRAEA_E_INVALID, "null context");
     if (!c->loaded) return fail(ASTRAEA_E_STATE, "no particle layout: call astraea_set_layout first");
@@ -476,6 +521,83 @@ int checkLaunch(astraea_ctx* c) {
     (void)c;
     return ASTRAEA_OK;
 }
+
+// Waits for the tree just enqueued; when a particle has more neighbour candidates than the table
+// holds, widens the table and rebuilds it from the same tree.  Used where the caller is not
+// inside a run of steps (a load or an explicit tree build), so no step is ever lost to it.
+int fitNeighbourTable(astraea_ctx* c) {
+    for (int round = 0;; ++round) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        unsigned long long v[8];
+        HIP_TRY(hipMemcpy(v, c->dc.cnt.v, sizeof(v), hipMemcpyDeviceToHost));
+        if (v[kCntHold] == 0ull) return ASTRAEA_OK;
+        // clear the hold and the overflow flags and sizes (the counts 0, 1 and 4 stay)
+        HIP_TRY(hipMemset(c->dc.cnt.v + kCntHold, 0, 2 * sizeof(unsigned long long)));
+        HIP_TRY(hipMemset(c->dc.cnt.v + 5, 0, 3 * sizeof(unsigned long long)));
+        if (v[6] != 0ull) {
+            c->listValid = false;
+            return fail(ASTRAEA_E_OVERFLOW, "neighbour traversal stack overflow");
+        }
+        if (round > 0 || v[7] > kMaxNeighbourStride || v[5] > (1ull << 16)) {
+            c->listValid = false;
+            return fail(ASTRAEA_E_OVERFLOW,
+                        "a particle needs a table of %llu entries and a wavefront a union of %llu, more than the table "
+                        "can hold (%u and %u)",
+                        (unsigned long long)v[7], (unsigned long long)v[5], kMaxNeighbourStride, 1u << 16);
+        }
+        TRY(allocNeighbourList(c, std::max<uint32_t>(roundUp((uint32_t)wantedStride(v), 64), c->dc.nb.stride)));
+        launchNeighbours(c->dc, c->stream);
+    }
+}
+
+// Waits for the work queued on the context stream.  When the neighbour build of a queued step held
+// the step sequence (its table came out incomplete: stepHeld), the state is that of the held step
+// after its sort, with the step's dt: the table is widened and built again from the same tree, the
+// rest of that step runs (pass 1, gravity, pass 2, second kick, epilogue), and then the steps that
+// were queued after it, again until none is held.  The result is the run a table wide enough from
+// the start would have given.
+int settle(astraea_ctx* c) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (!c->loaded) return ASTRAEA_OK;
+    for (;;) {
+        unsigned long long v[8];
+        HIP_TRY(hipMemcpy(v, c->dc.cnt.v, sizeof(v), hipMemcpyDeviceToHost));
+        if (v[kCntHold] == 0ull) {
+            c->unsettled.clear();
+            return ASTRAEA_OK;
+        }
+        // the held step and the steps queued after it (all from astraea_step: every other path
+        // fits its table before it goes on)
+        const uint64_t lost = c->stepsQueued - v[kCntSteps];
+        if (lost == 0ull || lost > c->unsettled.size()) {
+            HIP_TRY(hipMemset(c->dc.cnt.v + kCntHold, 0, 2 * sizeof(unsigned long long)));
+            HIP_TRY(hipMemset(c->dc.cnt.v + 5, 0, 3 * sizeof(unsigned long long)));
+            c->stepsQueued = v[kCntSteps];
+            c->unsettled.clear();
+            c->listValid = false;
+            return fail(ASTRAEA_E_OVERFLOW, "neighbour table overflow outside a step: evaluate the forces again");
+        }
+        std::vector<float> rest(c->unsettled.end() - (ptrdiff_t)lost, c->unsettled.end());
+        c->unsettled.clear();
+        c->stepsQueued -= lost;
+        const int rc = fitNeighbourTable(c);
+        if (rc != ASTRAEA_OK) return rc;  // listValid is false: the forces must be evaluated again
+        launchPass1(c->dc, c->stream);
+        launchGravity(c->dc, c->stream);
+        launchPass2(c->dc, c->stream);
+        launchKick(c->dc, c->stream, true, false);
+        launchStepEnd(c->dc, c->stream);
+        c->stepsQueued += 1;
+        c->stepsRedone += lost;
+        c->dtRawValid = true;
+        TRY(checkLaunch(c));
+        for (size_t k = 1; k < rest.size(); ++k) TRY(queueSteps(c, rest[k], 1, false));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+}
+
+// settle, when steps may be pending; every entry point but astraea_step starts with it.
+int settleQueued(astraea_ctx* c) { return c->unsettled.empty() ? ASTRAEA_OK : settle(c); }
 
 }  // namespace
 
@@ -616,6 +738,7 @@ int astraea_set_params(astraea_ctx* c, const astraea_params* p) {
         return fail(ASTRAEA_E_INVALID, "unknown precision mode %u", p->precision);
     if (p->group_size != 0u && p->group_size != 32u && p->group_size != 64u)
         return fail(ASTRAEA_E_INVALID, "group_size %u is not 0, 32 or 64", p->group_size);
+    if (c->loaded) TRY(settleQueued(c));
     const bool wasFp64 = c->params.precision == ASTRAEA_PREC_FP64_ACCUM;
     c->params = *p;
     syncDerivedFlags(c);
@@ -717,6 +840,8 @@ int astraea_set_layout(astraea_ctx* c, uint32_t ns, const uint32_t* counts) {
     HIP_TRY(hipMemset(d.ss.dtRawBits, 0x7f, sizeof(uint32_t) * ns));
     HIP_TRY(hipMemset(d.cnt.v, 0, 8 * sizeof(unsigned long long)));
     HIP_TRY(hipDeviceSynchronize());
+    c->stepsQueued = 0;
+    c->unsettled.clear();
     c->loaded = true;
     c->treeValid = c->listValid = c->dtRawValid = c->posNewPending = false;
     return ASTRAEA_OK;
@@ -732,6 +857,7 @@ int astraea_num_particles(astraea_ctx* c, uint32_t* out) {
 
 int astraea_upload_field(astraea_ctx* c, int f, const void* src) {
     TRY(requireLoaded(c));
+    TRY(settleQueued(c));
     HIP_TRY(hipSetDevice(c->device));
     freeGraph(c);
     return transferField(c, f, const_cast<void*>(src), true);
@@ -739,40 +865,23 @@ int astraea_upload_field(astraea_ctx* c, int f, const void* src) {
 
 int astraea_download_field(astraea_ctx* c, int f, void* dst) {
     TRY(requireLoaded(c));
+    TRY(settleQueued(c));
     HIP_TRY(hipSetDevice(c->device));
     return transferField(c, f, dst, false);
 }
 
 int astraea_clear_stats(astraea_ctx* c) {
     TRY(requireLoaded(c));
+    TRY(settleQueued(c));
     launchClear(c->dc, c->stream);
     c->dtRawValid = false;
     return checkLaunch(c);
 }
 
-// Waits for the tree just enqueued; when a particle has more neighbour candidates than the table
-// holds, widens the table and rebuilds it from the same tree.  Used where the caller is not
-// inside a run of steps (a load or an explicit tree build), so no step is ever lost to it.
-static int fitNeighbourTable(astraea_ctx* c) {
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    unsigned long long v[8];
-    HIP_TRY(hipMemcpy(v, c->dc.cnt.v, sizeof(v), hipMemcpyDeviceToHost));
-    if (v[3] == 0ull) return ASTRAEA_OK;
-    if (v[7] > kMaxNeighbourStride || v[5] > (1ull << 16))
-        return fail(ASTRAEA_E_OVERFLOW,
-                    "a particle needs a table of %llu entries and a wavefront a union of %llu, more than the table "
-                    "can hold (%u and %u)",
-                    (unsigned long long)v[7], (unsigned long long)v[5], kMaxNeighbourStride, 1u << 16);
-    HIP_TRY(hipMemset(c->dc.cnt.v + 3, 0, sizeof(unsigned long long)));
-    HIP_TRY(hipMemset(c->dc.cnt.v + 5, 0, sizeof(unsigned long long)));
-    HIP_TRY(hipMemset(c->dc.cnt.v + 7, 0, sizeof(unsigned long long)));
-    TRY(allocNeighbourList(c, std::max<uint32_t>(roundUp((uint32_t)wantedStride(v), 64), c->dc.nb.stride)));
-    launchNeighbours(c->dc, c->stream);
-    return ASTRAEA_OK;
-}
 
 int astraea_sort_and_build_bvh(astraea_ctx* c) {
     TRY(requireLoaded(c));
+    TRY(settleQueued(c));
     if (!c->posNewPending) {
         launchResetBounds(c->dc, c->stream);
         launchPrepareSort(c->dc, c->stream);
@@ -786,6 +895,7 @@ int astraea_sort_and_build_bvh(astraea_ctx* c) {
 
 int astraea_sph_pass1(astraea_ctx* c) {
     TRY(requireLoaded(c));
+    TRY(settleQueued(c));
     if (!c->listValid) return fail(ASTRAEA_E_STATE, "pass 1 needs a tree: call astraea_sort_and_build_bvh");
     launchPass1(c->dc, c->stream);
     c->dtRawValid = false;
@@ -794,6 +904,7 @@ int astraea_sph_pass1(astraea_ctx* c) {
 
 int astraea_evaluate_eos(astraea_ctx* c) {
     TRY(requireLoaded(c));
+    TRY(settleQueued(c));
     launchEos(c->dc, c->stream);
     c->dtRawValid = false;
     return checkLaunch(c);
@@ -801,6 +912,7 @@ int astraea_evaluate_eos(astraea_ctx* c) {
 
 int astraea_gravity(astraea_ctx* c) {
     TRY(requireLoaded(c));
+    TRY(settleQueued(c));
     if (!c->treeValid) return fail(ASTRAEA_E_STATE, "gravity needs a tree: call astraea_sort_and_build_bvh");
     launchGravity(c->dc, c->stream);
     c->dtRawValid = false;
@@ -809,6 +921,7 @@ int astraea_gravity(astraea_ctx* c) {
 
 int astraea_sph_pass2(astraea_ctx* c) {
     TRY(requireLoaded(c));
+    TRY(settleQueued(c));
     if (!c->listValid) return fail(ASTRAEA_E_STATE, "pass 2 needs a tree: call astraea_sort_and_build_bvh");
     launchPass2(c->dc, c->stream);
     c->dtRawValid = false;
@@ -817,6 +930,7 @@ int astraea_sph_pass2(astraea_ctx* c) {
 
 int astraea_reduce_dt(astraea_ctx* c, float maxDt, float* out) {
     TRY(requireLoaded(c));
+    TRY(settleQueued(c));
     if (!(maxDt > 0.0f)) return fail(ASTRAEA_E_INVALID, "max_dt must be > 0");
     launchDtCandidates(c->dc, c->stream);
     launchStepBeginNoBounds(c->dc, c->stream, maxDt);
@@ -830,6 +944,7 @@ int astraea_reduce_dt(astraea_ctx* c, float maxDt, float* out) {
 
 int astraea_kick_half(astraea_ctx* c, int clamp) {
     TRY(requireLoaded(c));
+    TRY(settleQueued(c));
     launchKick(c->dc, c->stream, false, clamp != 0);
     c->dtRawValid = false;
     return checkLaunch(c);
@@ -837,6 +952,7 @@ int astraea_kick_half(astraea_ctx* c, int clamp) {
 
 int astraea_drift(astraea_ctx* c) {
     TRY(requireLoaded(c));
+    TRY(settleQueued(c));
     if (c->posNewPending) return fail(ASTRAEA_E_STATE, "a drift must be followed by astraea_sort_and_build_bvh");
     launchResetBounds(c->dc, c->stream);
     launchDrift(c->dc, c->stream);
@@ -847,6 +963,7 @@ int astraea_drift(astraea_ctx* c) {
 
 int astraea_evaluate_initial_forces(astraea_ctx* c) {
     TRY(requireLoaded(c));
+    TRY(settleQueued(c));
     HIP_TRY(hipSetDevice(c->device));
     hipStream_t s = c->stream;
     launchClear(c->dc, s);
@@ -874,66 +991,19 @@ int astraea_step(astraea_ctx* c, float maxDt, uint32_t steps, int useGraph) {
         launchDtCandidates(c->dc, c->stream);
         c->dtRawValid = true;
     }
-    uint32_t left = steps;
-    if (useGraph && !c->timing) {
-        if (c->graph && (c->graphMaxDt != maxDt || c->graphParity != c->parity)) freeGraph(c);
-        if (left >= 2 && !c->graph) {
-            hipGraph_t g = nullptr;
-            HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            const int parity0 = c->parity;
-            enqueueStep(c, maxDt, false);
-            enqueueStep(c, maxDt, false);
-            const hipError_t e = hipStreamEndCapture(c->stream, &g);
-            if (e != hipSuccess) return fail(ASTRAEA_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
-            const hipError_t ei = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (ei != hipSuccess) return fail(ASTRAEA_E_HIP, "graph instantiate failed: %s", hipGetErrorString(ei));
-            c->graphMaxDt = maxDt;
-            c->graphParity = parity0;
-            // the capture recorded two steps without running them; the two swaps restored cur/alt
-            c->parity = parity0;
-        }
-        while (left >= 2 && c->graph) {
-            HIP_TRY(hipGraphLaunch(c->graph, c->stream));
-            left -= 2;
-        }
-    }
-    while (left > 0) {
-        enqueueStep(c, maxDt, c->timing);
-        --left;
-    }
-    c->dtRawValid = true;
+    TRY(queueSteps(c, maxDt, steps, useGraph != 0));
     return checkLaunch(c);
 }
 
 int astraea_synchronize(astraea_ctx* c) {
     if (!c) return fail(ASTRAEA_E_INVALID, "null context");
     HIP_TRY(hipSetDevice(c->device));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    if (!c->loaded) return ASTRAEA_OK;
-    unsigned long long v[8];
-    HIP_TRY(hipMemcpy(v, c->dc.cnt.v, sizeof(v), hipMemcpyDeviceToHost));
-    if (v[3] != 0ull || v[6] != 0ull) {
-        // clear the sticky flags; a table too narrow is widened for the following steps
-        HIP_TRY(hipMemset(c->dc.cnt.v + 3, 0, sizeof(unsigned long long)));
-        HIP_TRY(hipMemset(c->dc.cnt.v + 5, 0, sizeof(unsigned long long)));
-        HIP_TRY(hipMemset(c->dc.cnt.v + 6, 0, 2 * sizeof(unsigned long long)));
-        if (v[3] != 0ull) {
-            const uint32_t was = c->dc.nb.stride;
-            TRY(allocNeighbourList(c, std::max<uint32_t>(roundUp((uint32_t)wantedStride(v), 64), was)));
-            c->listValid = false;
-            return fail(ASTRAEA_E_OVERFLOW,
-                        "neighbour table overflow: a particle needs %llu entries and a wavefront a union of %llu, the "
-                        "table holds %u per particle (widened to %u; the affected step is invalid)",
-                        (unsigned long long)v[7], (unsigned long long)v[5], was, c->dc.nb.stride);
-        }
-        return fail(ASTRAEA_E_OVERFLOW, "neighbour traversal stack overflow");
-    }
-    return ASTRAEA_OK;
+    return settle(c);
 }
 
 int astraea_get_dt(astraea_ctx* c, float* out) {
     TRY(requireLoaded(c));
+    TRY(settleQueued(c));
     if (!out) return fail(ASTRAEA_E_INVALID, "null output");
     HIP_TRY(hipMemcpyAsync(out, c->dc.ss.dt, sizeof(float) * c->numSeg, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -942,6 +1012,7 @@ int astraea_get_dt(astraea_ctx* c, float* out) {
 
 int astraea_energy(astraea_ctx* c, double* out) {
     TRY(requireLoaded(c));
+    TRY(settleQueued(c));
     if (!out) return fail(ASTRAEA_E_INVALID, "null output");
     HIP_TRY(hipSetDevice(c->device));
     launchEnergy(c->dc, c->stream, c->dEnergy);
@@ -965,6 +1036,7 @@ int astraea_energy(astraea_ctx* c, double* out) {
 
 int astraea_classify(astraea_ctx* c, const double* targetRadius, double* out) {
     TRY(requireLoaded(c));
+    TRY(settleQueued(c));
     if (!targetRadius || !out) return fail(ASTRAEA_E_INVALID, "null argument");
     const uint32_t S = c->numSeg;
     for (uint32_t s = 0; s < S; ++s)
@@ -1063,12 +1135,14 @@ static int enqueueRelaxKdk(astraea_ctx* c) {
     launchPass2(c->dc, s);
     launchKick(c->dc, s, true, true);
     launchStepEnd(c->dc, s);
+    c->stepsQueued += 1;
     return checkLaunch(c);
 }
 
 int astraea_relax(astraea_ctx* c, uint32_t steps, const float* anchorTempK, const double* nominalRadius,
                   double* out) {
     TRY(requireLoaded(c));
+    TRY(settleQueued(c));
     if (!nominalRadius || !out) return fail(ASTRAEA_E_INVALID, "null argument");
     HIP_TRY(hipSetDevice(c->device));
     const uint32_t S = c->numSeg;
@@ -1144,6 +1218,7 @@ static int ensureRenderBuffers(astraea_ctx* c) {
 
 int astraea_pack_render(astraea_ctx* c, const double* comPsu, uint8_t* records, uint8_t* speed) {
     TRY(requireLoaded(c));
+    TRY(settleQueued(c));
     if (!comPsu || !records) return fail(ASTRAEA_E_INVALID, "null argument");
     HIP_TRY(hipSetDevice(c->device));
     TRY(ensureRenderBuffers(c));
@@ -1169,6 +1244,7 @@ int astraea_pack_render(astraea_ctx* c, const double* comPsu, uint8_t* records, 
 int astraea_set_tracers(astraea_ctx* c, uint32_t count, const uint32_t* particleId, const float* rest,
                         const uint8_t* ocean) {
     TRY(requireLoaded(c));
+    TRY(settleQueued(c));
     if (count > 0 && (!particleId || !rest || !ocean)) return fail(ASTRAEA_E_INVALID, "null argument");
     if (count > 65535u) return fail(ASTRAEA_E_INVALID, "%u tracers: the record's anchor index is 16 bits", count);
     HIP_TRY(hipSetDevice(c->device));
@@ -1202,6 +1278,7 @@ int astraea_set_tracers(astraea_ctx* c, uint32_t count, const uint32_t* particle
 
 int astraea_extract_tracers(astraea_ctx* c, const double* livingComPsu, const double* sysComPsu, uint8_t* records) {
     TRY(requireLoaded(c));
+    TRY(settleQueued(c));
     if (!livingComPsu || !sysComPsu || (c->tracerCount > 0 && !records)) return fail(ASTRAEA_E_INVALID, "null argument");
     if (c->tracerCount == 0) return ASTRAEA_OK;
     HIP_TRY(hipSetDevice(c->device));
@@ -1229,6 +1306,7 @@ int astraea_set_neighbour_stride(astraea_ctx* c, uint32_t stride) {
         return fail(ASTRAEA_E_INVALID, "neighbour stride %u outside [1, %u]", stride, kMaxNeighbourStride);
     stride = roundUp(stride, 8 * kBanks);  // kBanks lists per slot, in chunks of eight entries
     HIP_TRY(hipSetDevice(c->device));
+    if (c->loaded) TRY(settleQueued(c));
     HIP_TRY(hipStreamSynchronize(c->stream));
     TRY(allocNeighbourList(c, stride));
     c->listValid = false;
@@ -1237,11 +1315,12 @@ int astraea_set_neighbour_stride(astraea_ctx* c, uint32_t stride) {
 
 int astraea_counters(astraea_ctx* c, uint64_t* out, int n) {
     TRY(requireLoaded(c));
+    TRY(settleQueued(c));
     if (!out || n <= 0) return fail(ASTRAEA_E_INVALID, "null output");
     HIP_TRY(hipStreamSynchronize(c->stream));
     unsigned long long v[8];
     HIP_TRY(hipMemcpy(v, c->dc.cnt.v, sizeof(v), hipMemcpyDeviceToHost));
-    uint64_t vals[7] = {v[0], v[1], 0, c->dc.nb.stride, 0, 0, 0};
+    uint64_t vals[8] = {v[0], v[1], 0, c->dc.nb.stride, 0, 0, 0, c->stepsRedone};
     {
         const uint32_t waves = c->slots / kWave;
         std::vector<uint32_t> ws(waves), wm(waves);
@@ -1261,7 +1340,7 @@ int astraea_counters(astraea_ctx* c, uint64_t* out, int n) {
             vals[5] += (uint64_t)b[s].w;
         });
     }
-    for (int k = 0; k < n && k < 7; ++k) out[k] = vals[k];
+    for (int k = 0; k < n && k < 8; ++k) out[k] = vals[k];
     // interaction and forced-acceptance counts restart from zero
     HIP_TRY(hipMemset(c->dc.cnt.v, 0, 2 * sizeof(unsigned long long)));
     return ASTRAEA_OK;
@@ -1270,6 +1349,7 @@ int astraea_counters(astraea_ctx* c, uint64_t* out, int n) {
 int astraea_download_tree(astraea_ctx* c, uint32_t seg, uint64_t* keys, uint32_t* perm, int32_t* left, int32_t* right,
                           int32_t* pI, int32_t* pL, float* cm, float* mn, float* mx) {
     TRY(requireLoaded(c));
+    TRY(settleQueued(c));
     if (seg >= c->numSeg) return fail(ASTRAEA_E_INVALID, "segment %u out of range", seg);
     if (!c->treeValid) return fail(ASTRAEA_E_STATE, "no tree: call astraea_sort_and_build_bvh");
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1343,6 +1423,7 @@ int astraea_autotune(astraea_ctx* c, float maxDt, uint32_t stepsPerVariant, astr
         const astraea_tuning t = {p1[order2[r]], p2[order2[r]], dr[order4[r]]};
         rc = astraea_set_tuning(c, &t);
         if (rc == ASTRAEA_OK) rc = astraea_step(c, maxDt, stepsPerVariant, 0);
+        if (rc == ASTRAEA_OK) rc = settle(c);
         double stage[ASTRAEA_STAGE_COUNT_];
         uint64_t n[ASTRAEA_STAGE_COUNT_];
         if (rc == ASTRAEA_OK) rc = astraea_stage_times(c, stage, n);

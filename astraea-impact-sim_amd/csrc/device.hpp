@@ -234,10 +234,13 @@ struct SegScalars {
 };
 
 struct Counters {
-    unsigned long long* v;  // [8]: 0 interactions, 1 stack overflows (gravity), 3 nbr overflow flag,
+    unsigned long long* v;  // [8]: 0 interactions, 1 stack overflows (gravity), 2 hold flag (see
+                            // stepHeld), 3 nbr overflow flag, 4 steps completed (kStepEnd),
                             // 5 largest union a wavefront wanted (sticky max), 6 nbr stack overflow,
                             // 7 table width wanted: kBanks x the fullest bank of a slot (sticky max)
 };
+constexpr int kCntHold = 2;
+constexpr int kCntSteps = 4;
 
 struct Params {
     float theta2;

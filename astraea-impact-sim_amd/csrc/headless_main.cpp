@@ -50,7 +50,9 @@ bool finiteSnapshot(const TelemetrySnapshot& s) {
                         s.radiatedEnergyJ, s.totalEnergyJ, s.initialTotalEnergyJ, s.energyDriftRel,
                         s.momentumKgMS[0], s.momentumKgMS[1], s.momentumKgMS[2], s.angularMomentumKgM2S[0],
                         s.angularMomentumKgM2S[1], s.angularMomentumKgM2S[2], s.virialRatio, s.maxTempK,
-                        s.maxPressureGPa, s.boundMassFrac, s.diskMassFrac, s.ejectaMassFrac, s.qRatioQrd, s.gpuStepMs};
+                        s.maxPressureGPa, s.boundMassFrac, s.diskMassFrac, s.ejectaMassFrac, s.qRatioQrd, s.gpuStepMs,
+                        s.livingBodyComKm[0], s.livingBodyComKm[1], s.livingBodyComKm[2], s.livingBodyRadiusKm,
+                        s.livingBodySpinRadS, s.impactorComKm[0], s.impactorComKm[1], s.impactorComKm[2]};
     for (double d : v)
         if (!std::isfinite(d)) return false;
     return true;

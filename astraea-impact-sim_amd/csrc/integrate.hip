@@ -54,6 +54,7 @@ __device__ __forceinline__ void reduceDtRad(const DeviceCtx& c, uint32_t sg, flo
 
 template <bool SECOND, bool CLAMP, bool STRENGTH, bool FP64>
 __global__ __launch_bounds__(kBlock) void kKick(DeviceCtx c) {
+    if (stepHeld(c)) return;
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t sg = c.seg.slotSeg[i];
     const bool active = i - c.seg.off[sg] < c.seg.n[sg];
@@ -175,6 +176,7 @@ __global__ __launch_bounds__(kBlock) void kDrift(DeviceCtx c) {
     };
     __shared__ float sb[kWavesPerBlock][6];
     __shared__ Rec stageBuf[kWavesPerBlock][S];
+    if (stepHeld(c)) return;
     const uint32_t b = xcdBlock(blockIdx.x, gridDim.x);
     const uint32_t i = b * kBlock + threadIdx.x;
     const int lane = laneId(), wave = threadIdx.x >> 6;
@@ -249,7 +251,7 @@ __global__ __launch_bounds__(kBlock) void kDrift(DeviceCtx c) {
 // clamp, sph_gravity.cu:2137-2160), then reset the accumulators of this step.
 __global__ void kStepBegin(DeviceCtx c, float maxDt, int resetBounds) {
     const uint32_t sg = blockIdx.x * blockDim.x + threadIdx.x;
-    if (sg >= c.seg.count) return;
+    if (sg >= c.seg.count || stepHeld(c)) return;
     const float floorDt = fminf(1.0e-5f, maxDt);
     const float prev = c.ss.dtPrev[sg];
     const float upper = prev > 0.0f ? fminf(prev * 1.15f, maxDt) : maxDt;
@@ -264,10 +266,12 @@ __global__ void kStepBegin(DeviceCtx c, float maxDt, int resetBounds) {
     if (resetBounds) c.ss.living[sg] = 0u;
 }
 
-// Step epilogue: radiated energy by the trapezoid rule (simulation.cpp:1741-1744) and time.
+// Step epilogue: radiated energy by the trapezoid rule (simulation.cpp:1741-1744) and time; the
+// count of completed steps (the host compares it with the steps it queued).
 __global__ void kStepEnd(DeviceCtx c) {
     const uint32_t sg = blockIdx.x * blockDim.x + threadIdx.x;
-    if (sg >= c.seg.count) return;
+    if (sg >= c.seg.count || stepHeld(c)) return;
+    if (sg == 0u) c.cnt.v[kCntSteps] += 1ull;
     const double dtSec = (double)c.ss.dt[sg] * 1.0e3;
     const double cur = c.ss.radSum[sg];
     c.ss.erad[sg] += 0.5 * (c.ss.radPrev[sg] + cur) * dtSec / 1.0e30;

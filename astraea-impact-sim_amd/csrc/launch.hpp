@@ -50,6 +50,17 @@ struct DeviceCtx {
     Tuning tune;
 };
 
+// The hold flag.  The neighbour build sets it when the table of a step came out incomplete (a
+// bank, a union or the walk's stack too small).  From then on every kernel of the step sequence
+// that would advance the state returns at once, and the permutation copies the state in place of
+// gathering it, so the state, the tree and the step's dt stay those of the incomplete step while
+// the graph or the queue of steps runs out.  The host then widens the table and finishes that
+// step.  The flag only changes between launches on the context stream, so a kernel reads it once,
+// uniformly, at its start.
+__device__ __forceinline__ bool stepHeld(const DeviceCtx& c) {
+    return __hip_atomic_load(&c.cnt.v[kCntHold], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0ull;
+}
+
 // sort.hip: stable LSD radix sort of (key, value) over `bits` key bits, then (segments > 1)
 // one stable pass on the segment of each value.  Result in keyA/valA.
 void sortPairs(const DeviceCtx& c, hipStream_t s);

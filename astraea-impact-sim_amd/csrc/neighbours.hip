@@ -18,7 +18,9 @@
 // particle and bank, the ascending 16-bit positions of its neighbours in that bank, one 16-bit
 // store each (the table's lines fill in L2).  A bank of a slot that wants more than stride / kBanks entries keeps the
 // first ones; a union longer than `unionCap` is cut; the overflow flag and the sizes wanted are
-// left for the host, which widens the table.
+// left for the host, which widens the table.  An incomplete table also raises the hold flag
+// (stepHeld): the rest of the step sequence waits for the host, which widens the table, builds it
+// again from the same tree and finishes the step.
 #include "launch.hpp"
 
 namespace astraea_hip {
@@ -58,6 +60,7 @@ struct WaveLds {
 // any lane form the wave's union list; a lane's entry is the hit's position in it.
 __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
     __shared__ WaveLds lds[kWavesPerBlock];
+    if (stepHeld(c)) return;
     const uint32_t b = xcdBlock(blockIdx.x, gridDim.x);
     const int wave = threadIdx.x >> 6, lane = laneId();
     const uint32_t i = b * kBlock + threadIdx.x;
@@ -287,6 +290,7 @@ __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
             atomicMax(&c.cnt.v[5], (unsigned long long)ulen);
         }
         if (full) atomicOr(&c.cnt.v[6], 1ull);
+        if (want > K || ulen > UC || full) atomicOr(&c.cnt.v[kCntHold], 1ull);
     }
 }
 

@@ -14,8 +14,9 @@
 // collision outcome from the same rules as simulation.cpp:1229-1281.
 //
 // Bodies come from the hydrostatic planet builder (astraea_build_planet) and are relaxed on the
-// device (astraea_relax), both bodies at once, anchored to their profile temperatures; render
-// frames and tracers are not produced.
+// device (astraea_relax), both bodies at once, anchored to their profile temperatures.  Render
+// frames (astraea_pack_render) and crust tracer records (astraea_extract_tracers) come from the
+// device as in simulation.cpp:1805-1867.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

@@ -50,6 +50,7 @@ __global__ __launch_bounds__(kBlock) void kPass1(DeviceCtx c) {
         float4 p, v;
     };
     __shared__ Rec stageBuf[kWavesPerBlock][S];
+    if (stepHeld(c)) return;
     const uint32_t b = xcdBlock(blockIdx.x, gridDim.x);
     const uint32_t i = b * kBlock + threadIdx.x;
     const uint32_t sg = c.seg.slotSeg[i];
@@ -180,6 +181,7 @@ template <bool STRENGTH, int S>
 __global__ __launch_bounds__(kBlock) void kPass2(DeviceCtx c) {
     using Rec = Pass2Rec<STRENGTH>;
     __shared__ Rec stageBuf[kWavesPerBlock][S];
+    if (stepHeld(c)) return;
     const uint32_t b = xcdBlock(blockIdx.x, gridDim.x);
     const uint32_t i = b * kBlock + threadIdx.x;
     const uint32_t sg = c.seg.slotSeg[i];

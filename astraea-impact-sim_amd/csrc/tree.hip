@@ -92,15 +92,18 @@ __global__ __launch_bounds__(kBlock) void kMorton(DeviceCtx c) {
 }
 
 // Gather the persistent state into the sorted order and build the Karras tree of each segment.
+// While the step sequence is held (stepHeld) the state is copied in place and the tree is kept:
+// the host swaps the two sets after this kernel either way.
 __global__ __launch_bounds__(kBlock) void kPermuteKarras(DeviceCtx c) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t sg = c.seg.slotSeg[i];
     const uint32_t off = c.seg.off[sg], n = c.seg.n[sg];
     const uint32_t local = i - off;
     if (local >= n) return;
-    const uint32_t src = c.sb.valA[i];
+    const bool hold = stepHeld(c);
+    const uint32_t src = hold ? i : c.sb.valA[i];
     const size_t cap = (size_t)c.stride64;
-    float4 p = c.d.posNew[src];
+    float4 p = hold ? c.cur.pos[i] : c.d.posNew[src];
     if (c.fp64) {
         const double x = c.cur.p64[src], y = c.cur.p64[cap + src], z = c.cur.p64[2 * cap + src];
         c.alt.p64[i] = x; c.alt.p64[cap + i] = y; c.alt.p64[2 * cap + i] = z;
@@ -116,6 +119,7 @@ __global__ __launch_bounds__(kBlock) void kPermuteKarras(DeviceCtx c) {
     const uint32_t meta = c.cur.meta[src];
     c.alt.meta[i] = meta;
     c.alt.oid[i] = c.cur.oid[src];
+    if (hold) return;
     if (((meta >> 16) & 0xffu) != 0u && hu.w < 0.8f) atomicOr(&c.ss.living[sg], 1u);
 
     if (n == 1u) {
@@ -243,6 +247,7 @@ __device__ __forceinline__ bool inChunk(const DeviceCtx& c, int p, int base) {
 __global__ __launch_bounds__(kChunk) void kMoments(DeviceCtx c) {
     __shared__ uint32_t arrive[kChunk];
     __shared__ NodeRec rec[kChunk];
+    if (stepHeld(c)) return;
     const int base = (int)(blockIdx.x * kChunk);
     const uint32_t i = (uint32_t)base + threadIdx.x;
     arrive[threadIdx.x] = 0u;

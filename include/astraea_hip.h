@@ -175,7 +175,13 @@ int astraea_evaluate_initial_forces(astraea_ctx* ctx);
  * `use_graph` the sequence is captured once in a hipGraph and replayed. */
 int astraea_step(astraea_ctx* ctx, float max_dt_psu, uint32_t steps, int use_graph);
 
-/* Block until the context stream is idle; reports device-side overflows. */
+/* Block until the context stream is idle.  A step whose neighbour table came out incomplete
+ * (see astraea_set_neighbour_stride) holds the rest of the queued steps on the device; this call
+ * (and every other entry point except astraea_step, which only queues) then widens the table,
+ * finishes that step and runs the held steps, so the result is the run a wide enough table gives.
+ * ASTRAEA_E_OVERFLOW only when the table cannot hold what a particle needs (more than 65536
+ * entries, or a wavefront union above 65536) or the neighbour walk's stack overflowed: the state
+ * then stays at the step after its sort, and the forces must be evaluated again. */
 int astraea_synchronize(astraea_ctx* ctx);
 
 /* Per-segment scalars after the last dt reduction: dt (PSU) of the last step. */
@@ -283,7 +289,9 @@ int astraea_relax(astraea_ctx* ctx, uint32_t steps, const float* anchor_temp_k,
  * sph_gravity.cu:933), `out[2]` = neighbour candidates of the last table (all
  * particles), `out[3]` = table entries per particle, `out[4]` = sum of pass-1
  * neighbour counts (self excluded), `out[5]` = sum of pass-2 pairs, `out[6]` =
- * largest candidate count of one particle in the last table.  Up to `n` values. */
+ * largest candidate count of one particle in the last table, `out[7]` = steps held by an
+ * incomplete neighbour table and run again after it was widened (since the context was created).
+ * Up to `n` values. */
 int astraea_counters(astraea_ctx* ctx, uint64_t* out, int n);
 
 /* Width of the neighbour table, in entries per particle, rounded up to a multiple
@@ -292,8 +300,9 @@ int astraea_counters(astraea_ctx* ctx, uint64_t* out, int n);
  * min(32 * stride, 65536) ids, 4 bytes each, in four banks) and each particle 16-bit
  * positions into them, stride / 4 per bank, so the table takes 4 * stride * capacity
  * bytes.  A particle with more candidates in a bank than stride / 4, or a wavefront
- * with more than its union holds, makes astraea_synchronize report ASTRAEA_E_OVERFLOW
- * for the step and widen the table. */
+ * with more than its union holds, holds the queued steps until the next synchronising call
+ * widens the table and runs them (astraea_synchronize).  A force evaluation at load widens
+ * it the same way. */
 #define ASTRAEA_DEFAULT_NEIGHBOUR_STRIDE 320
 int astraea_set_neighbour_stride(astraea_ctx* ctx, uint32_t stride);
 

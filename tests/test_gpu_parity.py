@@ -172,7 +172,7 @@ def force_parity(lib, orc, st, counts, params, tol):
 # Tolerances (relative RMS vs FP64).  Gravity, density rate and du/dt: 1e-5 (the north-star
 # bar).  The SPH acceleration is a sum of O(100) pressure terms that cancel to a few percent
 # of each term; FP32 rounding of the terms then shows up at ~1e-6..1e-5 of the result.
-FORCE_TOL = {"default": 1e-5, "acc_sph": 2e-5, "pressure": 1e-6, "sound_speed": 1e-6, "temp_k": 1e-6, "h": 1e-6}
+FORCE_TOL = {"default": 1e-5, "acc_sph": 1e-5, "pressure": 1e-6, "sound_speed": 1e-6, "temp_k": 1e-6, "h": 1e-6}
 
 
 def test_force_parity_small_config(lib, orc):
@@ -539,35 +539,47 @@ def test_neighbour_table_widens_at_load(lib):
     assert np.array_equal(got, want)
 
 
-def test_neighbour_table_overflow_in_steps_is_reported(lib):
+@pytest.mark.parametrize("graph", [False, True])
+def test_neighbour_table_overflow_in_steps_is_recovered(lib, graph):
     """Smoothing lengths far below equilibrium grow by 8 % per step, and the candidate counts with
-    them; once a particle outgrows the table, synchronize reports the overflow, widens the table and
-    asks for a new force evaluation."""
+    them.  Thirty steps are queued at once on a table fitted to the loaded state: a step outgrows
+    it, holds the queued steps on the device, and the synchronising call widens the table,
+    finishes that step and runs the held ones.  The state after the run equals, bit for bit, the
+    run on a table wide enough from the start (the table's width never changes a pair's order)."""
     from astraea_amd import scenario as S
 
     st = S.config_small()
     st["h"] = (st["h"] * 0.35).astype(np.float32)
     n = len(st["mass"])
-    ctx = lib.Context(n)
-    ctx.set_params(S.plain_params())
-    ctx.set_layout([n])
-    ctx.set_neighbour_stride(32)
-    ctx.load_state(st)
-    ctx.initial_forces()  # widens the table to what the loaded state needs, with a quarter to spare
-    ctx.synchronize()
-    width = ctx.counters()["nbr_stride"]
-    assert ctx.counters()["nbr_max"] <= width
-    with pytest.raises(lib.AstraeaError) as e:
-        for _ in range(30):
-            ctx.step(MAX_DT, 1, graph=False)
-            ctx.synchronize()
-    assert e.value.code == -5 and "neighbour table overflow" in str(e.value)
-    assert ctx.counters()["nbr_stride"] > width
-    with pytest.raises(lib.AstraeaError):
-        ctx.step(MAX_DT, 1)   # the table was rebuilt empty: forces must be evaluated again
-    ctx.initial_forces()
-    ctx.step(MAX_DT, 2)
-    ctx.synchronize()
+    fields = ("pos_x", "pos_y", "pos_z", "vel_x", "vel_y", "vel_z", "rho", "u", "h", "acc_x", "du_dt",
+              "original_id")
+
+    def run(stride):
+        ctx = lib.Context(n)
+        ctx.set_params(S.plain_params())
+        ctx.set_layout([n])
+        ctx.set_neighbour_stride(stride)
+        ctx.load_state(st)
+        ctx.initial_forces()  # fits the table to the loaded state, with a quarter to spare
+        ctx.synchronize()
+        width = ctx.counters()["nbr_stride"]
+        ctx.step(MAX_DT, 30, graph=graph)
+        ctx.synchronize()
+        c = ctx.counters()
+        out = {k: ctx.download(k) for k in fields}
+        out["dt"] = ctx.dt()
+        return out, c, width, ctx.energy()
+
+    want, cw, _, ew = run(2048)
+    assert cw["steps_redone"] == 0 and cw["nbr_max"] <= 2048
+    got, cg, width, eg = run(32)
+    assert cg["steps_redone"] > 0, "the run never outgrew its table: the case tests nothing"
+    assert cg["nbr_stride"] > width
+    for k in want:
+        assert np.array_equal(got[k], want[k]), k
+    # the FP64 sums (block partials added by atomics in any order) agree to rounding; momenta and
+    # centres are cancelling sums of terms of order one, hence the absolute part
+    assert np.all(np.abs(eg - ew) <= 1e-11 * np.abs(ew) + 1e-14), (eg, ew)
 
 
 def test_sim_headless_verify_conservation():
