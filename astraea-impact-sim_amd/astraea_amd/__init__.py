@@ -166,6 +166,7 @@ _sig = {
     "astraea_energy": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "astraea_classify": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "astraea_pack_render": (C.c_int, [_P, _P, _P, _P]),
+    "astraea_pack_render_noalloc": (C.c_int, [_P, _P, _P, _P]),
     "astraea_set_tracers": (C.c_int, [_P, C.c_uint32, _P, _P, _P]),
     "astraea_extract_tracers": (C.c_int, [_P, _P, _P, _P]),
     "astraea_select_tracers": (C.c_int, [C.c_uint32] + [_P] * 5 + [C.c_uint32] + [_P] * 5),
@@ -367,6 +368,13 @@ class Context:
         spd = np.zeros(n, np.uint8) if speed else None
         _check(_lib.astraea_pack_render(self._h, com.ctypes.data, rec.ctypes.data, None if spd is None else spd.ctypes.data))
         return (rec, spd) if speed else rec
+
+    def pack_render_noalloc(self, com_psu, records_dev: int, speed_dev: int | None = None):
+        """Allocation-free render records into the caller's device buffers (device addresses)."""
+        ns = len(self.counts)
+        com = np.ascontiguousarray(np.broadcast_to(np.asarray(com_psu, np.float64).reshape(-1, 3), (ns, 3)))
+        _check(_lib.astraea_pack_render_noalloc(self._h, com.ctypes.data, C.c_void_p(records_dev),
+                                                None if speed_dev is None else C.c_void_p(speed_dev)))
 
     def set_tracers(self, particle_id, rest, ocean):
         """Tracer t on the particle of ORIGINAL_ID particle_id[t]; rest: (count, 4) radius + normal."""

@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -52,6 +53,34 @@ struct FieldLoc {
     int comp;        // component index (for u8 inside meta: byte index)
 };
 
+// Table of the allocation-free render entry point (astraea_pack_render_noalloc): for each of at
+// most ASTRAEA_NOALLOC_MAX_SEGMENTS segments, the centre (3 floats, the first 3 x max entries) and
+// the output offset (1 u32, the last max entries).  4 KiB per GPU, held until the process ends.
+// Host code never uses this name as an address: each device's copy is reached through the
+// address hipGetSymbolAddress returns with that device current (scratchAddress).
+constexpr uint32_t kScratchWords = 4u * ASTRAEA_NOALLOC_MAX_SEGMENTS;
+static __device__ __align__(256) uint32_t scratch[kScratchWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex gScratchMutex;
+uint32_t* gScratchAddr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device`, which must be the current device; looked up once
+// per device.
+int scratchAddress(int device, uint32_t** out) {
+    if (device < 0 || device >= kMaxDevices) return fail(ASTRAEA_E_INVALID, "device %d beyond %d", device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(gScratchMutex);
+    if (!gScratchAddr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(ASTRAEA_E_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", device, hipGetErrorString(e));
+        gScratchAddr[device] = static_cast<uint32_t*>(p);
+    }
+    *out = gScratchAddr[device];
+    return ASTRAEA_OK;
+}
+
 }  // namespace
 
 int astraea_hip::reportError(int code, const char* fmt, ...) {
@@ -66,6 +95,7 @@ int astraea_hip::reportError(int code, const char* fmt, ...) {
 
 struct astraea_ctx {
     int device = 0;
+    uint32_t* scratchDev = nullptr;        // this device's address of the file-scope `scratch` table
     hipStream_t stream = nullptr;
     uint32_t capSlots = 0;
     uint32_t maxSeg = 0;
@@ -648,6 +678,10 @@ int astraea_create(astraea_ctx** out, int device, uint32_t capacity, uint32_t ma
     };
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(ASTRAEA_E_HIP, "hipStreamCreate failed"));
+    {
+        const int rs = scratchAddress(device, &c->scratchDev);
+        if (rs != ASTRAEA_OK) return cleanup(rs);
+    }
     const uint32_t C = c->capSlots;
     DeviceCtx& d = c->dc;
     int r = ASTRAEA_OK;
@@ -1238,6 +1272,44 @@ int astraea_pack_render(astraea_ctx* c, const double* comPsu, uint8_t* records, 
     HIP_TRY(hipMemcpyAsync(records, c->dRender, 20 * (size_t)total, hipMemcpyDeviceToHost, c->stream));
     if (speed) HIP_TRY(hipMemcpyAsync(speed, c->dSpeed, total, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return ASTRAEA_OK;
+}
+
+int astraea_pack_render_noalloc(astraea_ctx* c, const double* comPsu, uint8_t* recordsDev, uint8_t* speedDev) {
+    TRY(requireLoaded(c));
+    if (!comPsu || !recordsDev) return fail(ASTRAEA_E_INVALID, "null argument");
+    const uint32_t S = c->numSeg;
+    if (S > ASTRAEA_NOALLOC_MAX_SEGMENTS)
+        return fail(ASTRAEA_E_INVALID, "%u segments: the allocation-free render takes at most %u", S,
+                    (unsigned)ASTRAEA_NOALLOC_MAX_SEGMENTS);
+    if (!c->scratchDev) return fail(ASTRAEA_E_STATE, "no scratch table on device %d", c->device);
+    HIP_TRY(hipSetDevice(c->device));
+    if (!c->unsettled.empty()) {
+        // queued steps: finishing a held one would widen (allocate) the neighbour table
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        unsigned long long hold = 0ull;
+        HIP_TRY(hipMemcpy(&hold, c->dc.cnt.v + kCntHold, sizeof(hold), hipMemcpyDeviceToHost));
+        if (hold != 0ull)
+            return fail(ASTRAEA_E_STATE, "a queued step waits for a wider neighbour table: call astraea_synchronize");
+        c->unsettled.clear();
+    }
+    uint32_t table[kScratchWords];
+    float* com = reinterpret_cast<float*>(table);
+    uint32_t* base = table + 3u * ASTRAEA_NOALLOC_MAX_SEGMENTS;
+    uint32_t total = 0;
+    for (uint32_t k = 0; k < S; ++k) {
+        for (int a = 0; a < 3; ++a) com[3 * k + a] = (float)comPsu[3 * k + a];
+        base[k] = total;
+        total += c->counts[k];
+    }
+    uint32_t* dCom = c->scratchDev;
+    uint32_t* dBase = c->scratchDev + 3u * ASTRAEA_NOALLOC_MAX_SEGMENTS;
+    HIP_TRY(hipMemcpyAsync(dCom, com, sizeof(float) * 3 * S, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(dBase, base, sizeof(uint32_t) * S, hipMemcpyHostToDevice, c->stream));
+    launchPackRender(c->dc, c->stream, reinterpret_cast<const float*>(dCom), dBase,
+                     reinterpret_cast<uint32_t*>(recordsDev), speedDev);
+    TRY(checkLaunch(c));
+    HIP_TRY(hipStreamSynchronize(c->stream));  // the host table is read before it goes out of scope
     return ASTRAEA_OK;
 }
 

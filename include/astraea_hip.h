@@ -253,6 +253,22 @@ int astraea_classify(astraea_ctx* ctx, const double* target_radius_psu, double* 
  * current particle order (ORIGINAL_ID gives the input index), segments concatenated. */
 int astraea_pack_render(astraea_ctx* ctx, const double* com_psu, uint8_t* records, uint8_t* speed);
 
+/* Allocation-free variant of astraea_pack_render.  It has no counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops such as a render stream), where astraea_pack_render allocates its staging buffers on the
+ * first call.  `records_dev` (20 bytes per particle, 4-byte aligned) and `speed_dev` (NULL or one
+ * byte per particle) are the caller's device buffers; they receive exactly what astraea_pack_render
+ * returns, in the same order.  The per-segment centres and output offsets go through a fixed table
+ * in the library's device memory (ASTRAEA_NOALLOC_MAX_SEGMENTS segments, 4 KiB per GPU, held until
+ * the process ends); a layout with more segments is rejected with ASTRAEA_E_INVALID.  No device
+ * memory is allocated or freed.  A queued step held by an incomplete neighbour table is not
+ * finished here (that widens the table): ASTRAEA_E_STATE asks for astraea_synchronize first.
+ * Every call on one GPU uses that GPU's table, which keeps the last call's contents: calls that
+ * overlap in time on one GPU (from several threads or contexts) race.  Returns when the records
+ * are written. */
+#define ASTRAEA_NOALLOC_MAX_SEGMENTS 256
+int astraea_pack_render_noalloc(astraea_ctx* ctx, const double* com_psu, uint8_t* records_dev, uint8_t* speed_dev);
+
 /* Crust tracers (the tracer fields of the reference's particle buffers, types.hpp:150-156,
  * filled by planet_builder.cpp:737-906).  Tracer t sits on the particle whose ORIGINAL_ID is
  * particle_id[t]; rest = 4 floats per tracer (rest radius PSU, rest unit normal x, y, z about
