@@ -8,20 +8,23 @@ run the same stage sequence on both and compare.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
 import numpy as np
 
 _HERE = Path(__file__).resolve().parent
-LIB_PATH = _HERE / "liboracle.so"
+# ASTRAEA_ORACLE_LIB names another build of oracle.cpp: tests/test_oracle_mutants.py loads copies
+# with one constant or sign changed by hand to show that the pinning tests catch the change.
+LIB_PATH = Path(os.environ.get("ASTRAEA_ORACLE_LIB", str(_HERE / "liboracle.so")))
 
 
 def build():
     subprocess.run(["make", "-s", "-C", str(_HERE)], check=True)
 
 
-if not LIB_PATH.exists():
+if not LIB_PATH.exists() and "ASTRAEA_ORACLE_LIB" not in os.environ:
     build()
 _lib = C.CDLL(str(LIB_PATH))
 

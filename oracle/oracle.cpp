@@ -15,10 +15,22 @@
 //   moments                      root mass = sum m, root COM = system COM, boxes contain children.
 //   gravity                      brute-force FP64 N^2 (reference AC3 thresholds,
 //                                tests/cpp/test_gravity_sph.cpp:293-395) and theta -> 0 limit = N^2.
-//   pass1 / pass2 / drift        invariants: uniform translation -> div v = 0, drho/dt = 0; pairwise
-//                                antisymmetry -> sum m a_sph = 0; isolated-particle cooling closed
-//                                form (tests/cpp/test_gravity_sph.cpp:646-670); Wendland normalisation.
-//   kick / dt                    closed form per particle.
+//   pass1                        linear velocity fields on a lattice: grad v = A M_i from brute-force
+//                                lattice sums, M_i = identity to lattice accuracy; Shepard sums and
+//                                f_surf by brute force (interior 0, free surface > 0); the h update on
+//                                configurations of known neighbour count, with both clamps.
+//   pass2                        isolated pairs: Monaghan viscosity x Balsara (with and without curl),
+//                                tensile term = R_i + R_j at a quarter of the mean support, stress
+//                                force and work (pair energy conserved); lattice: delta-SPH as a
+//                                Laplacian estimate (2 kappa rho0) restricted to one material, Jaumann
+//                                rate = 2 mu' strain + W S - S W (rigid rotation: co-rotation only);
+//                                Grady-Kipp rate at the failure strain (numpy principal stresses);
+//                                cooling: isolated closed form (tests/cpp/test_gravity_sph.cpp:646-670),
+//                                255 K ambient of an intact living body, the threshold knob.
+//   drift                        XSPH: zero for linear fields, 0.25 kappa H^2/15 for v = kappa y^2.
+//   kick / dt                    closed form per particle; von Mises radial return by hand.
+//   (tests/test_oracle_mutants.py builds copies with a constant, sign or index of these changed and
+//    checks that the pinning tests fail on each.)
 //   step                         conservation gates of engine/headless_main.cpp:263-281.
 //   relax                        closed forms: steps = 0 gives v_rms / v_esc of the loaded state;
 //                                force-free particles keep 0.85^steps of their velocity about the
