@@ -131,6 +131,22 @@ def gas_giant_sweep(count: int = 16, n_per: int = 500_000, master_seed: int = 42
     return out
 
 
+def build_gas_giant_sweep(specs, threads: int | None = None):
+    """Concatenated state of the gas-giant scenarios `specs` (gas_giant_sweep) and their particle
+    counts, built on `threads` host threads."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(sp):
+        return gas_giant_pair(sp["n_target"], sp["n_impactor"], sp["angle_deg"], sp["v_ratio"], seed=sp["seed"])
+
+    threads = threads or min(16, os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=threads) as pool:
+        states = list(pool.map(one, specs))
+    counts = [sp["n_target"] + sp["n_impactor"] for sp in specs]
+    merged = {k: np.ascontiguousarray(np.concatenate([s[k] for s in states])) for k in states[0]}
+    return merged, counts
+
+
 def config_giant(n_target: int = 900_000, n_impactor: int = 100_000, seed: int = 42) -> dict:
     """Giant impact, 900k + 100k particles, 45 deg, 1.1 v_esc."""
     return impact_pair(n_target, n_impactor, 45.0, 1.1, seed=seed)

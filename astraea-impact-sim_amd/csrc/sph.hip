@@ -65,9 +65,16 @@ __global__ __launch_bounds__(kBlock) void kPass1(DeviceCtx c) {
     const float hi = fmaxf(hu.x, 1.0e-6f), h0i = fmaxf(hu.y, 1.0e-6f), rhoI = fmaxf(vi.w, 1.0e-9f);
     const float Hi = 2.0f * hi;
     const float Hi2 = Hi * Hi;
-    float shep = (pi.w / rhoI) * (21.0f / (2.0f * kPiF * Hi * Hi * Hi));
+    // f_surf takes 1 - C and |n| / h against the 0.15 threshold, which multiplies their errors by
+    // up to 1.6 C / f_surf (hundreds, at a layer boundary inside a body where C dips to 0.9).  The
+    // Shepard terms are therefore evaluated in FP64 from the FP32 inputs (the pair's separation
+    // exactly, the kernel polynomial, the volume correctly rounded once per staged record), and
+    // the Shepard sum and the normal accumulate in FP64.  wNorm applies once, at the end.
+    const double pxd = (double)pi.x, pyd = (double)pi.y, pzd = (double)pi.z;
+    const double invHd = 1.0 / (double)Hi;
+    double shepW = 0.0;
     uint32_t ngb = 1;
-    float nsx = 0.0f, nsy = 0.0f, nsz = 0.0f;
+    double nsx = 0.0, nsy = 0.0, nsz = 0.0;
     float gxx = 0.0f, gxy = 0.0f, gxz = 0.0f, gyx = 0.0f, gyy = 0.0f, gyz = 0.0f, gzx = 0.0f, gzy = 0.0f, gzz = 0.0f;
 
     const Support sI = supportOf(Hi);
@@ -76,6 +83,7 @@ __global__ __launch_bounds__(kBlock) void kPass1(DeviceCtx c) {
         [&](uint32_t j, Rec& r) {
             r.p = c.cur.pos[j];
             r.v = c.cur.vel[j];
+            r.v.w = __fdiv_rn(r.p.w, fmaxf(r.v.w, 1.0e-9f));  // V_j = m_j / rho_j, once per record
         },
         [&](const Rec& n) {
             const float4 pj = n.p, vj = n.v;
@@ -84,13 +92,20 @@ __global__ __launch_bounds__(kBlock) void kPass1(DeviceCtx c) {
             if (!(r2 < Hi2)) return;
             ++ngb;
             const float r = fastSqrt(r2);
-            const float volJ = pj.w * fastRcp(maxPos(vj.w, 1.0e-9f));
-            const float w = wendlandS(r, sI);
-            const float vw = volJ * w;
-            shep += vw;
-            nsx -= vw * dx;
-            nsy -= vw * dy;
-            nsz -= vw * dz;
+            const float volJ = vj.w;
+            const double ex = pxd - (double)pj.x, ey = pyd - (double)pj.y, ez = pzd - (double)pj.z;
+            const double r2d = fma(ez, ez, fma(ey, ey, ex * ex));
+            // r = r2 / sqrt(r2): the hardware FP32 reciprocal square root (about 1 ulp) refined by one
+            // FP64 Newton step (relative error ~1e-14); a pair at zero separation has q = 0
+            const double y0 = (double)__builtin_amdgcn_rsqf(r2);
+            const double y1 = y0 * fma(-0.5 * r2d, y0 * y0, 1.5);
+            const double qd = r2 > 0.0f ? r2d * y1 * invHd : 0.0;
+            const double od = fmax(0.0, 1.0 - qd), o2d = od * od;
+            const double vw = (double)volJ * (o2d * o2d) * fma(4.0, qd, 1.0);
+            shepW += vw;
+            nsx -= vw * ex;
+            nsy -= vw * ey;
+            nsz -= vw * ez;
             const float gc = volJ * wendlandGradS(r, sI);
             const float gx = gc * dx, gy = gc * dy, gz = gc * dz;
             const float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
@@ -100,12 +115,16 @@ __global__ __launch_bounds__(kBlock) void kPass1(DeviceCtx c) {
         });
     if (!active) return;
 
-    const float normNs = sqrtf(nsx * nsx + nsy * nsy + nsz * nsz);
-    float fs = fminf(1.0f, fmaxf(0.0f, 1.6f * fmaxf(1.0f - shep, normNs / hi) - 0.15f));
+    const double wNormD = 21.0 / (2.0 * 3.14159265358979323846) * invHd * invHd * invHd;
+    const double shep = ((double)pi.w / (double)rhoI + shepW) * wNormD;
+    const double normNs = wNormD * sqrt(nsx * nsx + nsy * nsy + nsz * nsz);
+    float fs = (float)fmin(1.0, fmax(0.0, 1.6 * fmax(1.0 - shep, normNs / (double)hi) - 0.15));
     if (ngb <= 2u) fs = 1.0f;
     const float divV = gxx + gyy + gzz;
     const float cux = gzy - gyz, cuy = gxz - gzx, cuz = gyx - gxy;
-    const float factor = fminf(1.08f, fmaxf(0.92f, cbrtf(56.0f / (float)ngb)));
+    // the factor in FP64, rounded once: (float)cbrt(56/n) is the correctly rounded cube root for
+    // every n where the clamp does not bind, so the oracle reproduces h' bit for bit
+    const float factor = fminf(1.08f, fmaxf(0.92f, (float)cbrt(56.0 / (double)ngb)));
     const float hNew = fminf(6.0f * h0i, fmaxf(0.35f * h0i, hi * factor));
     hu.x = hNew;
     c.cur.hu[i] = hu;
@@ -120,7 +139,7 @@ __global__ __launch_bounds__(kBlock) void kPass1(DeviceCtx c) {
     const float invR2 = 1.0f / (rhoI * rhoI);
     const float pEff = effectivePressure(P, e.phase);
     c.d.nb[i] = make_float4(hNew, pEff * invR2, csF, fBal);
-    c.d.p1a[i] = make_float4(shep, fs, divV, (float)ngb);
+    c.d.p1a[i] = make_float4((float)shep, fs, divV, (float)ngb);
     c.d.p1b[i] = make_float4(cux, cuy, cuz, fBal);
     if (STRENGTH) {
         const float4 s0 = c.cur.str0[i];

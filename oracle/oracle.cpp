@@ -632,7 +632,7 @@ void pass1(or_ctx* c, Seg& s) {
 #pragma omp parallel for schedule(dynamic, 256) reduction(+ : pairs)
     for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
         const uint32_t i = (uint32_t)ii;
-        const double hi = std::max(s.h[i], 1.0e-6), h0i = std::max(s.h0[i], 1.0e-6), rhoI = std::max(s.rho[i], 1.0e-9);
+        const double hi = std::max(s.h[i], 1.0e-6), rhoI = std::max(s.rho[i], 1.0e-9);
         const double Hi = 2.0 * hi;
         double shep = (s.m[i] / rhoI) * (21.0 / (2.0 * kPi * Hi * Hi * Hi));
         int ngb = 1;
@@ -668,8 +668,11 @@ void pass1(or_ctx* c, Seg& s) {
         s.cu[1][i] = g[0][2] - g[2][0];
         s.cu[2][i] = g[1][0] - g[0][1];
         s.ngb[i] = (uint32_t)ngb;
-        const double fac = std::min(1.08, std::max(0.92, std::cbrt(56.0 / (double)ngb)));
-        hNew[i] = std::min(6.0 * h0i, std::max(0.35 * h0i, hi * fac));
+        // h' as the FP32 replica of the kernels' update (every later neighbour decision reads it):
+        // the factor rounded once from FP64, then the product and the clamps in FP32
+        const float hif = std::fmax((float)s.h[i], 1.0e-6f), h0f = std::fmax((float)s.h0[i], 1.0e-6f);
+        const float facf = std::fmin(1.08f, std::fmax(0.92f, (float)std::cbrt(56.0 / (double)ngb)));
+        hNew[i] = (double)std::fmin(6.0f * h0f, std::fmax(0.35f * h0f, hif * facf));
     }
     s.h = hNew;
     s.pairs1 = pairs;
@@ -790,9 +793,12 @@ void pass2(or_ctx* c, Seg& s) {
     const uint32_t n = s.n;
     uint64_t pairs = 0;
     std::vector<double> Hs(n), pE(n), csF(n), fB(n);
+    std::vector<float> Hf(n), xf(n), yf(n), zf(n);  // FP32 replicas for the pair decision
     for (uint32_t i = 0; i < n; ++i) {
         const double hi = std::max(s.h[i], 1.0e-6);
         Hs[i] = 2.0 * hi;
+        Hf[i] = 2.0f * std::fmax((float)s.h[i], 1.0e-6f);
+        xf[i] = (float)s.x[i]; yf[i] = (float)s.y[i]; zf[i] = (float)s.z[i];
         pE[i] = effP(s.P[i], s.phase[i]);
         csF[i] = std::max(s.cs[i], 0.05);
         fB[i] = balsara(s.divv[i], s.cu[0][i], s.cu[1][i], s.cu[2][i], csF[i], hi);
@@ -813,10 +819,13 @@ void pass2(or_ctx* c, Seg& s) {
         double a[3] = {0, 0, 0}, dC = 0, dDf = 0, duP = 0, duAv = 0, duT = 0, duS = 0;
         double g[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
         forCandidates(s, i, Hi, true, [&](uint32_t j) {
+            // FP32 replica of the kernels' decision r^2 < max(H_i, H_j)^2
+            const float r2f = dist2f(xf[i] - xf[j], yf[i] - yf[j], zf[i] - zf[j]);
+            const float Hmf = std::fmax(Hf[i], Hf[j]);
+            if (!(r2f < Hmf * Hmf)) return;
             const double dx = s.x[i] - s.x[j], dy = s.y[i] - s.y[j], dz = s.z[i] - s.z[j];
             const double r2 = dx * dx + dy * dy + dz * dz;
-            const double Hj = Hs[j], Hm = std::max(Hi, Hj);
-            if (!(r2 < Hm * Hm)) return;
+            const double Hj = Hs[j];
             ++pairs;
             const double hj = 0.5 * Hj;
             const double mj = s.m[j], rhoJ = std::max(s.rho[j], 1.0e-9), pJ = pE[j], csJ = csF[j];
@@ -970,7 +979,12 @@ void drift(or_ctx* c, Seg& s, double dt) {
     const uint32_t n = s.n;
     std::vector<double> nx(n), ny(n), nz(n);
     std::vector<double> Hs(n);
-    for (uint32_t i = 0; i < n; ++i) Hs[i] = 2.0 * std::max(s.h[i], 1.0e-6);
+    std::vector<float> Hf(n), xf(n), yf(n), zf(n);  // FP32 replicas for the pair decision
+    for (uint32_t i = 0; i < n; ++i) {
+        Hs[i] = 2.0 * std::max(s.h[i], 1.0e-6);
+        Hf[i] = 2.0f * std::fmax((float)s.h[i], 1.0e-6f);
+        xf[i] = (float)s.x[i]; yf[i] = (float)s.y[i]; zf[i] = (float)s.z[i];
+    }
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
         const uint32_t i = (uint32_t)ii;
@@ -978,10 +992,11 @@ void drift(or_ctx* c, Seg& s, double dt) {
         double xs[3] = {0, 0, 0};
         if (s.tree && n >= 2) {
             forCandidates(s, i, Hi, true, [&](uint32_t j) {
+                const float r2f = dist2f(xf[i] - xf[j], yf[i] - yf[j], zf[i] - zf[j]);
+                const float Hmf = std::fmax(Hf[i], Hf[j]);
+                if (!(r2f < Hmf * Hmf)) return;
                 const double dx = s.x[i] - s.x[j], dy = s.y[i] - s.y[j], dz = s.z[i] - s.z[j];
                 const double r2 = dx * dx + dy * dy + dz * dz;
-                const double Hm = std::max(Hi, Hs[j]);
-                if (!(r2 < Hm * Hm)) return;
                 const double r = std::sqrt(r2);
                 const double wb = 0.5 * (wendland(r, Hi) + wendland(r, Hs[j]));
                 const double rb = 0.5 * (rhoI + std::max(s.rho[j], 1.0e-9));

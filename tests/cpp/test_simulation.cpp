@@ -6,14 +6,30 @@
 //   render    tests/cpp/test_conservation.cpp:260-369 (AC12): render-frame layout, quantisation and
 //             the optional speed array, and the PCLS frame of it (server/binary_protocol.cpp);
 //   pcls      the PCLS header and payload layout on synthetic records (no GPU needed).
-// Usage: test_simulation {outcomes|relax|render|pcls}.  Prints one line per case; the exit status
-// is the number of failed expectations (200 + k on an exception in case k).
+//   ac4       tests/cpp/test_gravity_sph.cpp:397-496: a 10k LivingTerra body alone for 200 steps keeps
+//             its core inside the mantle without a gap, a density jump > 1.8 at the core-mantle
+//             boundary and an rms speed below 0.015 v_esc;
+//   ac5       tests/cpp/test_conservation.cpp:123-158: the 20k Earth-Theia impact for 150 steps within
+//             the conservation bars;
+//   ac6       tests/cpp/test_gravity_sph.cpp:498-589: the 15k Mercury mantle strip for 650 steps melts
+//             granite and damages the crater region, not the far field;
+//   ac8       tests/cpp/test_conservation.cpp:160-195: 100k particles in under 512 MiB of device
+//             memory, median step time within 33.3 ms;
+//   ac9       tests/cpp/test_conservation.cpp:197-258: every crust tracer once in each frame over 100
+//             steps of re-sorting, every field finite and in range;
+//   ac10      tests/cpp/test_conservation.cpp:371-416: controls, history and the CSV / JSON exports.
+// Usage: test_simulation {outcomes|relax|render|pcls|ac4|ac5|ac6|ac8|ac9|ac10}.  Prints one line per
+// case; the exit status is the number of failed expectations (200 + k on an exception in case k).
+#include <hip/hip_runtime.h>
+
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <exception>
-
+#include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/astraea/binary_protocol.hpp"
@@ -311,11 +327,304 @@ int pcls() {
     return fails;
 }
 
+// ---- render-frame records, decoded as the reference's tests decode them ----
+struct Rec {
+    double x, y, z, tempK, pGPa, damage, radiusKm, speedKmS;
+    uint8_t material, body, phase;
+};
+
+std::vector<Rec> decode(const std::vector<uint8_t>& p, const std::vector<uint8_t>* speed) {
+    const double ln20000 = std::log(20000.0), ln1e7 = std::log(1.0e7), ln2000 = std::log(2000.0);
+    std::vector<Rec> out(p.size() / 20);
+    for (size_t i = 0; i < out.size(); ++i) {
+        const uint8_t* r = p.data() + 20 * i;
+        Rec& o = out[i];
+        o.x = f32(r);
+        o.y = f32(r + 4);
+        o.z = f32(r + 8);
+        o.tempK = 10.0 * std::exp(u16(r + 12) / 65535.0 * ln20000);
+        const uint16_t pc = u16(r + 14);
+        o.pGPa = pc == 0 ? 0.0 : 1.0e-3 * std::exp((pc - 1) / 65534.0 * ln1e7);
+        o.material = r[16];
+        o.body = r[17] >> 4;
+        o.phase = r[17] & 0x0f;
+        o.damage = r[18] / 255.0;
+        o.radiusKm = std::exp(r[19] / 255.0 * ln2000);
+        o.speedKmS = speed ? 0.05 * std::exp((*speed)[i] / 255.0 * ln2000) : 0.0;
+    }
+    return out;
+}
+
+std::array<double, 3> bodyCom(const std::vector<Rec>& recs, uint8_t body, uint32_t* count = nullptr) {
+    double s[3] = {0, 0, 0};
+    uint32_t n = 0;
+    for (const Rec& r : recs)
+        if (r.body == body) {
+            s[0] += r.x; s[1] += r.y; s[2] += r.z;
+            ++n;
+        }
+    if (count) *count = n;
+    return n ? std::array<double, 3>{s[0] / n, s[1] / n, s[2] / n} : std::array<double, 3>{0, 0, 0};
+}
+
+int ac4() {
+    try {
+        CollisionScenarioConfig cfg;
+        cfg.presetName = "CmbStabilityTest";
+        cfg.target = {PlanetArchetype::LivingTerra, units::M_EARTH_SI, 288.0, 5500.0, 0.0, 80};
+        cfg.impactor.massKg = 0.0;
+        cfg.impactVelocityRatioVesc = 0.0;
+        cfg.impactParamB = 0.0;
+        cfg.initialSepRadii = 50.0;
+        cfg.totalParticles = 10000;
+        cfg.tracerGridRes = 8;
+        cfg.randomSeed = 42;
+        Simulation sim(12000);
+        sim.loadScenario(cfg);
+        for (int s = 0; s < 200; ++s) sim.step(5.0);
+        std::vector<uint8_t> pb, tb, sb;
+        sim.extractRenderFrame(pb, tb, &sb);
+        const std::vector<Rec> recs = decode(pb, &sb);
+        uint32_t n0 = 0;
+        const std::array<double, 3> c = bodyCom(recs, 0, &n0);
+        expect(n0 > 5000u, "ac4: target particles > 5000");
+        // (radius, volume-equivalent radius) of iron and granite; particles of one body share a
+        // mass, so their densities go as 1 / r_vol^3
+        std::vector<std::pair<double, double>> iron, granite;
+        double sumV2 = 0.0;
+        for (const Rec& r : recs) {
+            if (r.body != 0) continue;
+            const double rad = std::sqrt((r.x - c[0]) * (r.x - c[0]) + (r.y - c[1]) * (r.y - c[1]) + (r.z - c[2]) * (r.z - c[2]));
+            sumV2 += r.speedKmS * r.speedKmS;
+            if (r.material == (uint8_t)MaterialId::Iron) iron.push_back({rad, r.radiusKm});
+            else if (r.material == (uint8_t)MaterialId::Granite) granite.push_back({rad, r.radiusKm});
+        }
+        expect(iron.size() > 100u && granite.size() > 200u, "ac4: iron and granite present");
+        std::sort(iron.begin(), iron.end());
+        std::sort(granite.begin(), granite.end());
+        const double ironP95 = iron[(size_t)(0.95 * (iron.size() - 1))].first;
+        const double granP05 = granite[(size_t)(0.05 * (granite.size() - 1))].first;
+        const double granP95 = granite[(size_t)(0.95 * (granite.size() - 1))].first;
+        const double spacing = std::cbrt((4.0 * M_PI / 3.0) * std::pow(granP95, 3) / (double)n0);
+        expect(ironP95 < granP05 + 1.2 * spacing, "ac4: core inside the mantle");
+        expect(granP05 - ironP95 < 2.5 * spacing, "ac4: no gap at the core-mantle boundary");
+        const size_t ne = std::max<size_t>(10, iron.size() / 10), ng = std::max<size_t>(10, granite.size() / 10);
+        double invIron = 0.0, invGran = 0.0;  // mean 1 / r_vol^3 on each side of the boundary
+        for (size_t i = iron.size() - ne; i < iron.size(); ++i) invIron += 1.0 / std::pow(iron[i].second, 3);
+        for (size_t i = 0; i < ng; ++i) invGran += 1.0 / std::pow(granite[i].second, 3);
+        const double ratio = (invIron / ne) / (invGran / ng);
+        expect(ratio > 1.8, "ac4: density jump > 1.8 at the core-mantle boundary");
+        const double vRms = std::sqrt(sumV2 / n0);
+        const double vEsc = std::sqrt(2.0 * units::G_SI * units::M_EARTH_SI / units::R_EARTH_SI) / 1000.0;
+        expect(vRms / vEsc < 0.015, "ac4: rms speed < 0.015 v_esc");
+        std::printf("ac4 ironP95=%.1f km graniteP05=%.1f km spacing=%.1f km density ratio=%.2f vrms/vesc=%.4f %s\n",
+                    ironP95, granP05, spacing, ratio, vRms / vEsc, fails ? "FAIL" : "ok");
+    } catch (const std::exception& e) {
+        std::printf("exception in ac4: %s\n", e.what());
+        return 200;
+    }
+    return fails;
+}
+
+double norm3d(const double v[3]) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+double diff3d(const double a[3], const double b[3]) {
+    const double d[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+    return norm3d(d);
+}
+
+int ac5() {
+    try {
+        const CollisionScenarioConfig cfg = makePresetConfig(ScenarioPresetId::EarthTheiaMoonForming, 20000);
+        Simulation sim(25000);
+        sim.loadScenario(cfg);
+        const TelemetrySnapshot t0 = sim.computeTelemetry();
+        expect(t0.initialTotalEnergyJ != 0.0 && t0.particleCount == 20000u, "ac5: loaded");
+        for (int s = 0; s < 150; ++s) sim.step(5.0);
+        const TelemetrySnapshot t1 = sim.computeTelemetry();
+        expect(t1.frameSeq >= 150u && t1.simTimeSec > 0.0, "ac5: frames and time advance");
+        expect(std::isfinite(t1.energyDriftRel) && std::fabs(t1.energyDriftRel) < 0.01, "ac5: |energy drift| < 0.01");
+        const double manual = (t1.totalEnergyJ + t1.radiatedEnergyJ - t0.initialTotalEnergyJ) / std::fabs(t0.initialTotalEnergyJ);
+        expect(std::fabs(manual) < 0.01, "ac5: manual drift < 0.01");
+        const double sumE = t1.kineticEnergyJ + t1.potentialEnergyJ + t1.internalEnergyJ;
+        expect(std::fabs(t1.totalEnergyJ - sumE) <= 1.0e-5 * std::fabs(t1.totalEnergyJ) + 1.0e20, "ac5: E = K + U + Phi");
+        const double mTot = cfg.target.massKg + cfg.impactor.massKg;
+        const double vEsc = std::sqrt(2.0 * units::G_SI * mTot / (1.4 * units::R_EARTH_SI));
+        const double dP = diff3d(t1.momentumKgMS, t0.momentumKgMS) / (mTot * vEsc);
+        const double L0 = norm3d(t0.angularMomentumKgM2S);
+        const double dL = diff3d(t1.angularMomentumKgM2S, t0.angularMomentumKgM2S) / L0;
+        expect(L0 > 0.0 && dP < 1.0e-4 && dL < 0.005, "ac5: momentum and angular momentum");
+        std::printf("ac5 drift=%.3e manual=%.3e dP=%.3e dL=%.3e t=%.0f s %s\n", t1.energyDriftRel, manual, dP, dL,
+                    t1.simTimeSec, fails ? "FAIL" : "ok");
+    } catch (const std::exception& e) {
+        std::printf("exception in ac5: %s\n", e.what());
+        return 200;
+    }
+    return fails;
+}
+
+int ac6() {
+    try {
+        CollisionScenarioConfig cfg = makePresetConfig(ScenarioPresetId::MercuryMantleStrip, 15000);
+        cfg.initialSepRadii = 1.02;
+        Simulation sim(20000);
+        sim.loadScenario(cfg);
+        const TelemetrySnapshot t0 = sim.computeTelemetry();
+        std::vector<uint8_t> pb, tb;
+        sim.extractRenderFrame(pb, tb, nullptr);
+        const std::array<double, 3> c0 = bodyCom(decode(pb, nullptr), 0);
+        const double off[3] = {t0.impactorComKm[0] - c0[0], t0.impactorComKm[1] - c0[1], t0.impactorComKm[2] - c0[2]};
+        for (int s = 0; s < 650; ++s) sim.step(5.0);
+        const TelemetrySnapshot t = sim.computeTelemetry();
+        expect(t.maxTempK > 1600.0, "ac6: max temperature > 1600 K");
+        sim.extractRenderFrame(pb, tb, nullptr);
+        const std::vector<Rec> recs = decode(pb, nullptr);
+        const std::array<double, 3> c = bodyCom(recs, 0);
+        const double crater[3] = {c[0] + off[0], c[1] + off[1], c[2] + off[2]};
+        uint32_t molten = 0, damaged = 0, intact = 0, n0 = 0;
+        double dDam = 0.0, dInt = 0.0;
+        for (const Rec& r : recs) {
+            if (r.body != 0) continue;
+            ++n0;
+            if (r.material == (uint8_t)MaterialId::Granite && r.tempK > 1600.0 &&
+                (r.phase == (uint8_t)PhaseId::Molten || r.phase == (uint8_t)PhaseId::Vapor))
+                ++molten;
+            const double d = std::sqrt((r.x - crater[0]) * (r.x - crater[0]) + (r.y - crater[1]) * (r.y - crater[1]) +
+                                       (r.z - crater[2]) * (r.z - crater[2]));
+            // the record's damage byte is round(255 D): D > 0.8 and D < 0.2 as the reference's float tests
+            if (r.damage > 0.8) { ++damaged; dDam += d; }
+            else if (r.damage < 0.2) { ++intact; dInt += d; }
+        }
+        expect(n0 > 5000u, "ac6: target particles > 5000");
+        expect(molten > 10u, "ac6: molten or vapour granite > 10");
+        expect(damaged > 20u, "ac6: high damage > 20");
+        expect(intact > n0 / 10, "ac6: intact far field > 10 %");
+        const double mDam = damaged ? dDam / damaged : 1e300, mInt = intact ? dInt / intact : 0.0;
+        expect(mDam < mInt, "ac6: damage closer to the crater than intact material");
+        std::printf("ac6 maxT=%.0f K molten=%u damaged=%u intact=%u of %u mean dist damaged=%.0f intact=%.0f km %s\n",
+                    t.maxTempK, molten, damaged, intact, n0, mDam, mInt, fails ? "FAIL" : "ok");
+    } catch (const std::exception& e) {
+        std::printf("exception in ac6: %s\n", e.what());
+        return 200;
+    }
+    return fails;
+}
+
+int ac8() {
+    try {
+        if (hipFree(nullptr) != hipSuccess) throw std::runtime_error("hipFree(nullptr) failed");
+        size_t freeBefore = 0, total = 0, freeAfter = 0;
+        if (hipMemGetInfo(&freeBefore, &total) != hipSuccess) throw std::runtime_error("hipMemGetInfo failed");
+        const CollisionScenarioConfig cfg = makePresetConfig(ScenarioPresetId::LivingGaiaAsteroidCataclysm, 100000);
+        Simulation sim(110000);
+        sim.loadScenario(cfg);
+        for (int i = 0; i < 3; ++i) sim.step(5.0);
+        std::vector<double> ms;
+        for (int i = 0; i < 20; ++i) {
+            sim.step(5.0);
+            const TelemetrySnapshot snap = sim.computeTelemetry();
+            expect(std::isfinite(snap.gpuStepMs), "ac8: finite step time");
+            ms.push_back(snap.gpuStepMs);
+        }
+        if (hipMemGetInfo(&freeAfter, &total) != hipSuccess) throw std::runtime_error("hipMemGetInfo failed");
+        const double usedMiB = freeBefore > freeAfter ? (double)(freeBefore - freeAfter) / (1024.0 * 1024.0) : 0.0;
+        expect(usedMiB < 512.0, "ac8: device memory < 512 MiB at 100k");
+        std::sort(ms.begin(), ms.end());
+        const double median = ms[ms.size() / 2];
+        expect(median > 0.0 && median <= 33.3, "ac8: median step <= 33.3 ms");
+        std::printf("ac8 used=%.1f MiB median step=%.3f ms %s\n", usedMiB, median, fails ? "FAIL" : "ok");
+    } catch (const std::exception& e) {
+        std::printf("exception in ac8: %s\n", e.what());
+        return 200;
+    }
+    return fails;
+}
+
+int ac9() {
+    try {
+        const CollisionScenarioConfig cfg = makePresetConfig(ScenarioPresetId::LivingGaiaAsteroidCataclysm, 30000);
+        expect(cfg.tracerGridRes == 16u, "ac9: grid 16");
+        const uint32_t nt = 6u * 16u * 16u;
+        Simulation sim(35000);
+        sim.loadScenario(cfg);
+        std::vector<uint8_t> pb, tb;
+        uint32_t frames = 0;
+        for (int step = 0; step <= 100; ++step) {
+            if (step % 10 == 0) {
+                sim.extractRenderFrame(pb, tb, nullptr);
+                expect(tb.size() == 40u * nt, "ac9: tracer record count");
+                std::vector<uint32_t> seen(nt, 0u);
+                bool ok = true;
+                for (uint32_t t = 0; t < nt && 40u * (t + 1) <= tb.size(); ++t) {
+                    const uint8_t* r = tb.data() + 40u * t;
+                    const uint16_t idx = u16(r), flags = u16(r + 2);
+                    const float x = f32(r + 4), y = f32(r + 8), z = f32(r + 12), disp = f32(r + 16), dn = f32(r + 20),
+                                T = f32(r + 24), dmg = f32(r + 28), op = f32(r + 32), ts = f32(r + 36);
+                    if (idx < nt) ++seen[idx]; else ok = false;
+                    ok = ok && flags <= 3u && std::isfinite(x) && std::isfinite(y) && std::isfinite(z) &&
+                         std::sqrt(x * x + y * y + z * z) < 5.0e5f && std::isfinite(disp) && std::isfinite(dn) &&
+                         dn >= 0.0f && dn <= 3.2f && std::isfinite(T) && T >= 3.0f && T <= 300000.0f &&
+                         std::isfinite(dmg) && dmg >= 0.0f && dmg <= 1.0f && std::isfinite(op) && op >= 0.0f &&
+                         std::isfinite(ts) && ts > -1.0e5f && ts < 1.0e6f;
+                }
+                for (uint32_t c : seen) ok = ok && c == 1u;
+                expect(ok, "ac9: every tracer once, fields finite and in range");
+                ++frames;
+            }
+            if (step < 100) sim.step(5.0);
+        }
+        std::printf("ac9 frames=%u tracers=%u %s\n", frames, nt, fails ? "FAIL" : "ok");
+    } catch (const std::exception& e) {
+        std::printf("exception in ac9: %s\n", e.what());
+        return 200;
+    }
+    return fails;
+}
+
+int ac10() {
+    try {
+        const CollisionScenarioConfig cfg = makePresetConfig(ScenarioPresetId::LivingGaiaAsteroidCataclysm, 4000);
+        Simulation sim(5000);
+        sim.loadScenario(cfg);
+        expect(sim.activeConfig().presetName == cfg.presetName, "ac10: activeConfig preset");
+        expect(sim.activeConfig().totalParticles == 4000u, "ac10: activeConfig particles");
+        sim.setTimeScale(1.0);
+        sim.setSlicePlane({true, 0.0f, 1.0f, 0.0f, 0.0f});
+        sim.setSlicePlane({false, 0.0f, 1.0f, 0.0f, 0.0f});
+        for (int i = 0; i < 10; ++i) sim.step(5.0);
+        expect(sim.telemetryHistory().size() >= 10u, "ac10: history");
+        const std::string csv = sim.exportTelemetryCsv();
+        for (const char* f : {"frameSeq", "simTimeSec", "kineticEnergyJ", "potentialEnergyJ", "internalEnergyJ",
+                              "radiatedEnergyJ", "energyDriftRel"})
+            expect(csv.find(f) != std::string::npos, f);
+        expect(std::count(csv.begin(), csv.end(), '\n') >= 10, "ac10: csv lines");
+        const std::string json = sim.exportTelemetryJson();
+        const char* fields[29] = {"frameSeq", "simTimeSec", "dtSec", "outcome", "kineticEnergyJ", "potentialEnergyJ",
+                                  "internalEnergyJ", "radiatedEnergyJ", "totalEnergyJ", "initialTotalEnergyJ",
+                                  "energyDriftRel", "momentumKgMS", "angularMomentumKgM2S", "virialRatio", "maxTempK",
+                                  "maxPressureGPa", "boundMassFrac", "diskMassFrac", "ejectaMassFrac", "qRatioQrd",
+                                  "gpuStepMs", "livingBodyComKm", "livingBodyRadiusKm", "livingBodySpinRadS",
+                                  "impactorComKm", "particleCount", "tracerCount", "tracerGridRes", "livingBodyPresent"};
+        for (const char* f : fields) expect(json.find(f) != std::string::npos, f);
+        std::printf("ac10 history=%zu csv=%zu bytes json=%zu bytes %s\n", sim.telemetryHistory().size(), csv.size(),
+                    json.size(), fails ? "FAIL" : "ok");
+    } catch (const std::exception& e) {
+        std::printf("exception in ac10: %s\n", e.what());
+        return 200;
+    }
+    return fails;
+}
+
 int main(int argc, char** argv) {
     if (argc == 2 && std::strcmp(argv[1], "outcomes") == 0) return outcomes();
     if (argc == 2 && std::strcmp(argv[1], "relax") == 0) return relax();
     if (argc == 2 && std::strcmp(argv[1], "render") == 0) return render();
     if (argc == 2 && std::strcmp(argv[1], "pcls") == 0) return pcls();
-    std::fprintf(stderr, "usage: %s {outcomes|relax|render|pcls}\n", argv[0]);
+    if (argc == 2 && std::strcmp(argv[1], "ac4") == 0) return ac4();
+    if (argc == 2 && std::strcmp(argv[1], "ac5") == 0) return ac5();
+    if (argc == 2 && std::strcmp(argv[1], "ac6") == 0) return ac6();
+    if (argc == 2 && std::strcmp(argv[1], "ac8") == 0) return ac8();
+    if (argc == 2 && std::strcmp(argv[1], "ac9") == 0) return ac9();
+    if (argc == 2 && std::strcmp(argv[1], "ac10") == 0) return ac10();
+    std::fprintf(stderr, "usage: %s {outcomes|relax|render|pcls|ac4|ac5|ac6|ac8|ac9|ac10}\n", argv[0]);
     return 2;
 }

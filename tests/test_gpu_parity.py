@@ -144,6 +144,11 @@ def test_tree_bit_exact_segments(lib, orc):
 
 def force_parity(lib, orc, st, counts, params, tol):
     ctx, o = run_pair(lib, orc, st, counts, params, [("initial_forces",)])
+    return check_forces(ctx, o, tol)
+
+
+def check_forces(ctx, o, tol):
+    """Initial-force parity of a context and an oracle loaded with the same state."""
     cg, co = ctx.counters(), o.counters()
     # integer results of the decision replicas
     assert cg["interactions"] == co["interactions"], (cg, co)

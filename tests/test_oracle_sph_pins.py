@@ -220,12 +220,16 @@ def test_pass1_h_update_counts_against_56(orc, k, h, h0):
     (sph_gravity.cu:785-787); with at most one neighbour f_surf = 1."""
     st = _sphere_config(k, h, h0, np.random.default_rng(k))
     o = run(orc, st, ("sort_and_build", "pass1"))
-    hf, h0f = float(np.float32(h)), float(np.float32(h0))
+    hf, h0f = np.float32(h), np.float32(h0)
     ngb = k + 1
     ids = o.download("original_id")
     assert o.download("ngb_count")[np.argsort(ids)][0] == ngb
-    want = min(6 * h0f, max(0.35 * h0f, hf * min(1.08, max(0.92, (56.0 / ngb) ** (1.0 / 3.0)))))
-    assert abs(by_id(o, "h")[0] - want) < 1e-12 * want
+    # the reference's kernel evaluates the update in FP32 (sph_gravity.cu:785-787)
+    fac = min(np.float32(1.08), max(np.float32(0.92), np.float32(np.cbrt(56.0 / ngb))))
+    want = min(np.float32(6.0) * h0f, max(np.float32(0.35) * h0f, hf * fac))
+    assert by_id(o, "h")[0] == float(want), (by_id(o, "h")[0], want)
+    exact = min(6.0 * h0, max(0.35 * h0, h * min(1.08, max(0.92, (56.0 / ngb) ** (1.0 / 3.0)))))
+    assert abs(float(want) / exact - 1.0) < 1e-6
     if k <= 1:
         assert by_id(o, "surf_factor")[0] == 1.0
 
@@ -622,3 +626,30 @@ def test_cooling_ambient_and_threshold(orc, case):
         want = np.where(T > 500.0, 0.9 * SIGMA * math.pi * rp ** 2 * (T ** 4 - 255.0 ** 4), 0.0)
         assert T[0] < 500.0 < T[1]
     assert np.allclose(pr, want, rtol=1e-9, atol=0.0), (pr, want)
+
+
+def test_h_update_factor_is_correctly_rounded():
+    """The h update's factor clamp(cbrt(56/n), 0.92, 1.08) binds except for 45 <= n <= 71.  For those
+    n, (float)cbrt(56/n) evaluated in FP64 (the kernels and the oracle) is the correctly rounded
+    cube root, and the exact value lies so far from every float32 rounding boundary that any FP64
+    cube root within a few ulp rounds to the same float: the oracle's h' equals the kernels'."""
+    from decimal import Decimal, getcontext
+
+    getcontext().prec = 60
+    for n in range(40, 80):
+        x = Decimal(56) / Decimal(n)
+        y = Decimal(float(x) ** (1.0 / 3.0))
+        for _ in range(8):  # Newton on y^3 = x
+            y = y - (y * y * y - x) / (3 * y * y)
+        clamped = not (Decimal("0.92") < y < Decimal("1.08"))
+        assert clamped == (n < 45 or n > 71), n
+        if clamped:
+            continue
+        f = np.float32(np.cbrt(56.0 / n))
+        lo, hi = np.nextafter(f, np.float32(0)), np.nextafter(f, np.float32(2))
+        # the float nearest to y is f, and y is well inside f's rounding interval
+        below = (Decimal(float(f)) + Decimal(float(lo))) / 2
+        above = (Decimal(float(f)) + Decimal(float(hi))) / 2
+        assert below < y < above, n
+        margin = min(y - below, above - y) / y
+        assert margin > Decimal(2) ** -45, (n, margin)
