@@ -395,10 +395,10 @@ class Context:
         return out
 
     def counters(self) -> dict:
-        out = (C.c_uint64 * 8)()
-        _check(_lib.astraea_counters(self._h, out, 8))
+        out = (C.c_uint64 * 10)()
+        _check(_lib.astraea_counters(self._h, out, 10))
         keys = ["interactions", "forced_accepts", "nbr_entries", "nbr_stride", "pass1_pairs", "pass2_pairs",
-                "nbr_max", "steps_redone"]
+                "nbr_max", "steps_redone", "spilled_waves", "spill_bytes"]
         return dict(zip(keys, [int(v) for v in out]))
 
     def set_neighbour_stride(self, stride: int) -> None:

@@ -211,11 +211,15 @@ int allocNeighbourList(astraea_ctx* c, uint32_t stride) {
     return ASTRAEA_OK;
 }
 
-// Table width for the sizes a walk wanted (counter 7: the width the fullest bank of a slot needs,
-// counter 5: the largest union of a wavefront), with a quarter to spare.
-uint64_t wantedStride(const unsigned long long* v) {
-    const uint64_t need = std::max<uint64_t>(v[7], (v[5] + 31ull) / 32ull);
-    return std::min<uint64_t>(need + need / 4, kMaxNeighbourStride);
+// The spill pool of the wavefronts the regular region cannot hold (a captured step holds its
+// address, so the graph goes with it).
+int allocSpillPool(astraea_ctx* c, unsigned long long bytes) {
+    releaseArray(c, c->dc.nb.pool);
+    bytes = std::max<unsigned long long>((bytes + 255ull) & ~255ull, 256ull);
+    TRY(devAlloc(c, c->dc.nb.pool, (size_t)bytes));
+    c->dc.nb.poolBytes = bytes;
+    freeGraph(c);
+    return ASTRAEA_OK;
 }
 
 Params toDeviceParams(const astraea_params& p) {
@@ -558,8 +562,9 @@ int checkLaunch(astraea_ctx* c) {
 int fitNeighbourTable(astraea_ctx* c) {
     for (int round = 0;; ++round) {
         HIP_TRY(hipStreamSynchronize(c->stream));
-        unsigned long long v[8];
+        unsigned long long v[8], q[4];
         HIP_TRY(hipMemcpy(v, c->dc.cnt.v, sizeof(v), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(q, c->dc.nb.q, sizeof(q), hipMemcpyDeviceToHost));
         if (v[kCntHold] == 0ull) return ASTRAEA_OK;
         // clear the hold and the overflow flags and sizes (the counts 0, 1 and 4 stay)
         HIP_TRY(hipMemset(c->dc.cnt.v + kCntHold, 0, 2 * sizeof(unsigned long long)));
@@ -568,14 +573,14 @@ int fitNeighbourTable(astraea_ctx* c) {
             c->listValid = false;
             return fail(ASTRAEA_E_OVERFLOW, "neighbour traversal stack overflow");
         }
-        if (round > 0 || v[7] > kMaxNeighbourStride || v[5] > (1ull << 16)) {
+        if (round > 2 || v[5] > (1ull << 16)) {
             c->listValid = false;
             return fail(ASTRAEA_E_OVERFLOW,
-                        "a particle needs a table of %llu entries and a wavefront a union of %llu, more than the table "
-                        "can hold (%u and %u)",
-                        (unsigned long long)v[7], (unsigned long long)v[5], kMaxNeighbourStride, 1u << 16);
+                        "a wavefront's neighbours form a union of %llu ids, more than the 65536 a table can address",
+                        (unsigned long long)v[5]);
         }
-        TRY(allocNeighbourList(c, std::max<uint32_t>(roundUp((uint32_t)wantedStride(v), 64), c->dc.nb.stride)));
+        // the spill pool was too small: q[1] holds what every listed wavefront reserved
+        TRY(allocSpillPool(c, std::max<unsigned long long>(q[1] + q[1] / 4, 2 * c->dc.nb.poolBytes)));
         launchNeighbours(c->dc, c->stream);
     }
 }
@@ -712,7 +717,11 @@ int astraea_create(astraea_ctx** out, int device, uint32_t capacity, uint32_t ma
     A(devAlloc(c, d.nb.count, C));
     A(devAlloc(c, d.nb.waveSum, C / kWave));
     A(devAlloc(c, d.nb.waveMax, C / kWave));
+    A(devAlloc(c, d.nb.waveNeed, C / kWave));
     A(devAlloc(c, d.nb.unionLen, C / kWave));
+    A(devAlloc(c, d.nb.tab, C / kWave));
+    A(devAlloc(c, d.nb.spill, C / kWave));
+    A(devAlloc(c, d.nb.q, 4));
     A(devAlloc(c, d.seg.off, maxSegments + 1));
     A(devAlloc(c, d.seg.n, maxSegments));
     A(devAlloc(c, d.seg.slotSeg, C));
@@ -741,6 +750,7 @@ int astraea_create(astraea_ctx** out, int device, uint32_t capacity, uint32_t ma
     A(devAlloc(c, c->dRem, (size_t)24 * maxSegments));
     A(devAlloc(c, c->dSeed, maxSegments));
     if (r == ASTRAEA_OK) r = allocNeighbourList(c, ASTRAEA_DEFAULT_NEIGHBOUR_STRIDE);
+    if (r == ASTRAEA_OK) r = allocSpillPool(c, 1ull << 20);
     if (r != ASTRAEA_OK) return cleanup(r);
     d.stride64 = C;
     if (hipMemsetAsync(d.cnt.v, 0, 8 * sizeof(unsigned long long), c->stream) != hipSuccess)
@@ -863,6 +873,8 @@ int astraea_set_layout(astraea_ctx* c, uint32_t ns, const uint32_t* counts) {
     HIP_TRY(hipMemset(d.nb.count, 0, sizeof(uint2) * c->slots));
     HIP_TRY(hipMemset(d.nb.waveSum, 0, sizeof(uint32_t) * (c->slots / kWave)));
     HIP_TRY(hipMemset(d.nb.waveMax, 0, sizeof(uint32_t) * (c->slots / kWave)));
+    HIP_TRY(hipMemset(d.nb.waveNeed, 0, sizeof(uint32_t) * (c->slots / kWave)));
+    HIP_TRY(hipMemset(d.nb.q, 0, 4 * sizeof(unsigned long long)));
     HIP_TRY(hipMemset(d.nb.unionLen, 0, sizeof(uint32_t) * (c->slots / kWave)));
     HIP_TRY(hipMemset(d.ss.dtPrev, 0, sizeof(float) * ns));
     HIP_TRY(hipMemset(d.ss.dt, 0, sizeof(float) * ns));
@@ -1392,7 +1404,10 @@ int astraea_counters(astraea_ctx* c, uint64_t* out, int n) {
     HIP_TRY(hipStreamSynchronize(c->stream));
     unsigned long long v[8];
     HIP_TRY(hipMemcpy(v, c->dc.cnt.v, sizeof(v), hipMemcpyDeviceToHost));
-    uint64_t vals[8] = {v[0], v[1], 0, c->dc.nb.stride, 0, 0, 0, c->stepsRedone};
+    unsigned long long q[4];
+    HIP_TRY(hipMemcpy(q, c->dc.nb.q, sizeof(q), hipMemcpyDeviceToHost));
+    uint64_t vals[10] = {v[0], v[1], 0, c->dc.nb.stride, 0, 0, 0, c->stepsRedone, q[0],
+                         std::min<unsigned long long>(q[1], c->dc.nb.poolBytes)};
     {
         const uint32_t waves = c->slots / kWave;
         std::vector<uint32_t> ws(waves), wm(waves);
@@ -1412,7 +1427,7 @@ int astraea_counters(astraea_ctx* c, uint64_t* out, int n) {
             vals[5] += (uint64_t)b[s].w;
         });
     }
-    for (int k = 0; k < n && k < 8; ++k) out[k] = vals[k];
+    for (int k = 0; k < n && k < 10; ++k) out[k] = vals[k];
     // interaction and forced-acceptance counts restart from zero
     HIP_TRY(hipMemset(c->dc.cnt.v, 0, 2 * sizeof(unsigned long long)));
     return ASTRAEA_OK;

@@ -72,36 +72,45 @@ struct Tree {
     float4* bmax;         // (max x, y, z, max support bound of the subtree)
 };
 
-// Banks of the neighbour table (see tableChunks).
+// Banks of the neighbour table (see WaveTable).
 constexpr uint32_t kBanks = 4;
+
+// Where a wavefront's part of the neighbour table lives.  Its union: the slots that are a
+// neighbour of at least one of its particles; the q-th slot the walk found goes to bank q % kBanks
+// at position q / kBanks, bank b at uni + b ucap / kBanks.  A particle's neighbours come in runs of
+// slots the walk found one after another, so dealing the union out to the banks like cards gives
+// every particle about the same number of entries in each bank, and the rounds of a pass (a bank
+// at a time, staged in LDS) keep the lanes equally busy; in walk order a particle's neighbours
+// would crowd into a few rounds.  An entry is a 16-bit position in its bank, and a particle's
+// entries of one bank ascend.  Each slot's `stride` entries are kBanks lists of stride / kBanks;
+// entry k of bank b of lane l is half-word (k & 7) of chunk 64 (b stride / (8 kBanks) + (k >> 3)) + l
+// of `list`: a lane's entries come in 16-byte chunks of eight, chunk r of the 64 lanes is one
+// contiguous kilobyte.
+struct WaveTable {
+    uint16_t* list;
+    uint32_t* uni;
+    uint32_t stride;      // entries per slot, a multiple of 8 kBanks
+    uint32_t ucap;        // union ids, a multiple of kBanks (<= 65536)
+};
 
 struct Neighbours {
     uint2* count;         // entries of each slot per bank (four u16, bank b in half b & 1 of word b >> 1)
     uint32_t* waveSum;    // entries wanted by each wavefront's particles (uncapped)
     uint32_t* waveMax;    // largest entry count one particle of the wavefront wanted
-    uint16_t* list;       // see tableChunks
-    uint32_t* unionIds;   // per wavefront: the union of its particles' neighbours (slots), unionCap each
-    uint32_t* unionLen;   // per wavefront: length of that union (uncapped)
-    uint32_t stride;      // entries per slot, a multiple of 8 kBanks (stride / kBanks per bank)
-    uint32_t unionCap;    // union entries per wavefront, a multiple of kBanks (<= 65536)
+    uint32_t* waveNeed;   // table width the wavefront needs: kBanks x its fullest bank, rounded up to 32
+    uint32_t* unionLen;   // per wavefront: length of its union (uncapped)
+    WaveTable* tab;       // per wavefront: where its table lives
+    uint16_t* list;       // regular region: wavefront w at list + 64 w stride
+    uint32_t* unionIds;   // regular region: wavefront w at unionIds + w unionCap
+    uint32_t stride;      // regular entries per slot
+    uint32_t unionCap;    // regular union ids per wavefront
+    uint32_t* spill;      // wavefronts the regular region cannot hold, in the order they were found
+    unsigned long long* q;  // [4]: 0 wavefronts listed, 1 pool bytes reserved (also past the end),
+                            //      2 and 3 unused
+    uint8_t* pool;        // the spill pool
+    unsigned long long poolBytes;
 };
 
-// Table layout.  Each wavefront w owns a union list: the slots that are a neighbour of at least
-// one of its particles (or one of them).  The q-th slot the walk found goes to bank q % kBanks at
-// position q / kBanks; bank b of w's union starts at unionIds + w unionCap + b unionCap / kBanks.
-// A particle's neighbours come in runs of slots the walk found one after another, so dealing the
-// union out to the banks like cards gives every particle about the same number of entries in
-// each bank, and the rounds of a pass (a bank at a time, staged in LDS) keep the lanes equally
-// busy; in walk order a particle's neighbours would crowd into a few rounds.
-// An entry is a 16-bit position in its bank, and a particle's entries of one bank ascend.  Each
-// slot's `stride` entries are kBanks lists of stride / kBanks; entry k of bank b of slot
-// i = 64 w + l is half-word (k & 7) of chunk 64 (b stride / (8 kBanks) + (k >> 3)) + l of the
-// wavefront's block, which starts at half-word 64 w stride: a lane's entries come in 16-byte
-// chunks of eight, chunk r of the 64 lanes is one contiguous kilobyte.
-__device__ __forceinline__ uint4* tableChunks(const Neighbours& nb, uint32_t i) {
-    return reinterpret_cast<uint4*>(nb.list + (uint64_t)(i >> 6) * (64ull * nb.stride)) + (i & 63u);
-}
-__device__ __forceinline__ uint32_t bankChunks(const Neighbours& nb) { return nb.stride / (8u * kBanks); }
 __device__ __forceinline__ uint32_t bankCount(uint2 c, uint32_t b) {
     return ((b < 2u ? c.x : c.y) >> (16u * (b & 1u))) & 0xffffu;
 }
@@ -147,20 +156,21 @@ __device__ __forceinline__ void forEachNeighbour(const Neighbours& nb, uint32_t 
     constexpr int R = S / 64;
     constexpr uint32_t kPast = 0xffffffffu;  // an entry position beyond every round
     const uint32_t w = i >> 6, lane = i & 63u;
-    const uint32_t total = __builtin_amdgcn_readfirstlane(min(nb.unionLen[w], nb.unionCap));
-    const uint32_t bankCap = nb.unionCap / kBanks;
+    const WaveTable t = nb.tab[w];
+    const uint32_t total = __builtin_amdgcn_readfirstlane(min(nb.unionLen[w], t.ucap));
+    const uint32_t bankCap = t.ucap / kBanks;
     const uint2 counts = nb.count[i];
-    const uint32_t perBank = bankChunks(nb);
+    const uint32_t perBank = t.stride / (8u * kBanks);
     const uint32_t lastChunk = perBank - 1u;
 #pragma unroll 1
     for (uint32_t bank = 0; bank < kBanks; ++bank) {
         const uint32_t len = (total + (kBanks - 1u) - bank) / kBanks;
         if (len == 0u) break;  // banks fill in turn: the later ones are empty too
-        const uint32_t* U = nb.unionIds + (uint64_t)w * nb.unionCap + bank * bankCap;
+        const uint32_t* U = t.uni + bank * bankCap;
         const uint32_t cnt = bankCount(counts, bank);
         // chunks are loaded one ahead, unconditionally (clamped to the bank's last chunk: entries
         // past cnt are never used), so that no load result has to be merged with an older value
-        const uint4* ch = tableChunks(nb, i) + 64ull * (bank * perBank);
+        const uint4* ch = reinterpret_cast<const uint4*>(t.list) + lane + 64ull * (bank * perBank);
         uint4 cur = ch[0];
         uint4 nxt = ch[64ull * min(1u, lastChunk)];
         uint32_t k = 0;

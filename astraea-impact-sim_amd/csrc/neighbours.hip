@@ -13,14 +13,17 @@
 // sets are exactly the reference's.  Each lane then loops over its own list: the pair
 // physics runs only for real pairs.
 //
-// Layout (tableChunks): each wavefront writes the union of its particles' neighbour sets (the hit
+// Layout (WaveTable): each wavefront writes the union of its particles' neighbour sets (the hit
 // candidates, in the order the walk found them, dealt out to kBanks banks) and, for every
 // particle and bank, the ascending 16-bit positions of its neighbours in that bank, one 16-bit
-// store each (the table's lines fill in L2).  A bank of a slot that wants more than stride / kBanks entries keeps the
-// first ones; a union longer than `unionCap` is cut; the overflow flag and the sizes wanted are
-// left for the host, which widens the table.  An incomplete table also raises the hold flag
-// (stepHeld): the rest of the step sequence waits for the host, which widens the table, builds it
-// again from the same tree and finishes the step.
+// store each (the table's lines fill in L2).  A wavefront first builds into the regular region,
+// `stride` entries per slot and `unionCap` union ids.  One that needs more (a particle with many
+// neighbours, as an envelope particle of a gas giant with h ten times its core's) is listed, and a
+// second kernel builds it again into the spill pool with exactly the width it needs, so memory
+// follows the entries the particles have, not the worst particle times every slot.  Only a pool
+// too small for the listed wavefronts (or a union beyond 65536 ids, or the walk's stack) leaves
+// the table incomplete: then the hold flag (stepHeld) stops the rest of the step sequence, and the
+// host grows the pool, builds the table again from the same tree and finishes the step.
 #include "launch.hpp"
 
 namespace astraea_hip {
@@ -58,25 +61,26 @@ struct WaveLds {
 // 32 candidates into a hit mask, then the lane appends its hits in candidate order, so the
 // bookkeeping of an entry runs once per hit rather than once per candidate.  The candidates hit by
 // any lane form the wave's union list; a lane's entry is the hit's position in it.
-__global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
-    __shared__ WaveLds lds[kWavesPerBlock];
-    if (stepHeld(c)) return;
-    const uint32_t b = xcdBlock(blockIdx.x, gridDim.x);
-    const int wave = threadIdx.x >> 6, lane = laneId();
-    const uint32_t i = b * kBlock + threadIdx.x;
-    const uint32_t gw = i >> 6;
-    const uint32_t sg = uniformU32(c.seg.slotSeg[b * kBlock]);
+struct WaveBuild {
+    uint64_t cnt4;    // entries per bank of this lane's particle: bank b in bits 16 b .. 16 b + 15
+    uint32_t ulen;    // length of the wavefront's union (uncapped)
+    bool stackFull;
+};
+
+// Builds the table of wavefront gw (slots 64 gw .. 64 gw + 63) into `list` (64 lanes x K entries in
+// chunks, WaveTable) and `U` (UC union ids).  Every lane of the wavefront calls it.
+__device__ __forceinline__ WaveBuild buildWave(const DeviceCtx& c, WaveLds& L, uint32_t gw, int lane, uint16_t* list,
+                                               uint32_t K, uint32_t* U, uint32_t UC) {
+    const uint32_t i = gw * kWave + (uint32_t)lane;
+    const uint32_t sg = uniformU32(c.seg.slotSeg[gw * kWave]);
     const uint32_t off = uniformU32(c.seg.off[sg]), n = uniformU32(c.seg.n[sg]);
     const bool active = i - off < n;
-    const uint32_t K = c.nb.stride, Kb = K / kBanks, perBank = bankChunks(c.nb);
-    uint4* out = tableChunks(c.nb, i);
-    // entries per bank: bank b in bits 16 b .. 16 b + 15
+    const uint32_t Kb = K / kBanks, perBank = K / (8u * kBanks);
+    uint4* out = reinterpret_cast<uint4*>(list) + lane;
     uint64_t cnt4 = 0;
     uint32_t ulen = 0;
     bool stackFull = false;
-    WaveLds& L = lds[wave];
-    const uint32_t UC = c.nb.unionCap, UCb = UC / kBanks;
-    uint32_t* U = c.nb.unionIds + (uint64_t)gw * UC;
+    const uint32_t UCb = UC / kBanks;
     const uint64_t actMask = ballot(active);
     if (actMask != 0ull && n >= 2u) {
         float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -170,10 +174,11 @@ __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
                     const uint32_t jj = (uint32_t)L.candId[c0 + u];
                     if (jj == i) continue;
                     const uint32_t q = ulen + (uint32_t)__popc(any & ((1u << u) - 1u));
-                    if (q >= UC) continue;  // the union overflowed: the host widens the table
                     const uint32_t bk = q % kBanks, sh = bk << 4;
                     const uint32_t cb = (uint32_t)(cnt4 >> sh) & 0xffffu;
-                    if (cb < Kb)
+                    // past the union's capacity the entry is only counted: the wavefront is built
+                    // again in the spill pool with the width the counts ask for
+                    if (cb < Kb && q < UC)
                         reinterpret_cast<uint16_t*>(out + 64ull * (bk * perBank + (cb >> 3)))[cb & 7u] =
                             (uint16_t)(q / kBanks);
                     cnt4 += 1ull << sh;
@@ -265,37 +270,124 @@ __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
             if (sp == 0) break;
         }
     }
-    // the stored counts are capped at the bank's capacity
+    return WaveBuild{cnt4, ulen, stackFull};
+}
+
+// Per-slot counts (capped at the bank's capacity) and per-wavefront sizes of a build; true when
+// every bank and the union fit.
+__device__ __forceinline__ bool finishWave(const DeviceCtx& c, uint32_t gw, int lane, const WaveBuild& w, uint32_t K,
+                                           uint32_t UC) {
+    const uint32_t i = gw * kWave + (uint32_t)lane;
+    const uint32_t Kb = K / kBanks;
     uint32_t cnt = 0, maxBank = 0, stored[kBanks];
 #pragma unroll
     for (uint32_t bk = 0; bk < kBanks; ++bk) {
-        const uint32_t cb = (uint32_t)(cnt4 >> (16u * bk)) & 0xffffu;
+        const uint32_t cb = (uint32_t)(w.cnt4 >> (16u * bk)) & 0xffffu;
         cnt += cb;
         maxBank = max(maxBank, cb);
         stored[bk] = min(cb, Kb);
     }
     c.nb.count[i] = make_uint2(stored[0] | (stored[1] << 16), stored[2] | (stored[3] << 16));
-    if (lane == 0) c.nb.unionLen[gw] = ulen;
     uint32_t sum = cnt;
     for (int o = 32; o > 0; o >>= 1) sum += (uint32_t)__shfl_xor((int)sum, o, 64);
     const uint32_t mxc = waveMaxU(cnt);
-    const uint32_t want = waveMaxU(maxBank) * kBanks;  // the stride the fullest bank needs
-    const bool full = ballot(stackFull) != 0ull;
+    const uint32_t want = waveMaxU(maxBank) * kBanks;  // the width the fullest bank needs
+    const bool full = ballot(w.stackFull) != 0ull;
     if (lane == 0) {
+        c.nb.unionLen[gw] = w.ulen;
         c.nb.waveSum[gw] = sum;
         c.nb.waveMax[gw] = mxc;
-        if (want > K || ulen > UC) {
-            atomicOr(&c.cnt.v[3], 1ull);
-            atomicMax(&c.cnt.v[7], (unsigned long long)want);
-            atomicMax(&c.cnt.v[5], (unsigned long long)ulen);
+        c.nb.waveNeed[gw] = (want + 31u) & ~31u;
+        atomicMax(&c.cnt.v[7], (unsigned long long)want);
+        atomicMax(&c.cnt.v[5], (unsigned long long)w.ulen);
+        if (full) {
+            atomicOr(&c.cnt.v[6], 1ull);
+            atomicOr(&c.cnt.v[kCntHold], 1ull);
         }
-        if (full) atomicOr(&c.cnt.v[6], 1ull);
-        if (want > K || ulen > UC || full) atomicOr(&c.cnt.v[kCntHold], 1ull);
+        if (w.ulen > (1u << 16)) {  // beyond what 16-bit bank positions address
+            atomicOr(&c.cnt.v[3], 1ull);
+            atomicOr(&c.cnt.v[kCntHold], 1ull);
+        }
     }
+    return want <= K && w.ulen <= UC;
+}
+
+// First build: every wavefront into the regular region; the ones that need more are listed.
+__global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
+    __shared__ WaveLds lds[kWavesPerBlock];
+    if (stepHeld(c)) return;
+    const uint32_t b = xcdBlock(blockIdx.x, gridDim.x);
+    const int wave = threadIdx.x >> 6, lane = laneId();
+    const uint32_t gw = (b * kBlock) / kWave + (uint32_t)wave;
+    const uint32_t K = c.nb.stride, UC = c.nb.unionCap;
+    uint16_t* list = c.nb.list + (uint64_t)gw * (64ull * K);
+    uint32_t* U = c.nb.unionIds + (uint64_t)gw * UC;
+    const WaveBuild w = buildWave(c, lds[wave], gw, lane, list, K, U, UC);
+    const bool fits = finishWave(c, gw, lane, w, K, UC);
+    if (lane == 0) {
+        if (fits) {
+            c.nb.tab[gw] = WaveTable{list, U, K, UC};
+        } else {
+            const unsigned long long at = atomicAdd(&c.nb.q[0], 1ull);
+            c.nb.spill[at] = gw;
+        }
+    }
+}
+
+// Second build: listed wavefront k of the first build is built by wavefront k of this grid (as
+// many as the first build's, so every listed one has its own), into a region of the spill pool
+// exactly as wide as the first build found it needs.  The region is reserved with one atomic
+// addition; when the pool is too small the wavefront stays unbuilt and the hold flag is raised,
+// and q[1] (which sums what every listed wavefront reserved) is the size the host grows the pool
+// to.  No wavefront loops here: every branch below is uniform across the wavefront (lane 0 alone
+// does the atomics, then every lane reads its result), so no lane can be left behind while the
+// others read lane 0.
+__global__ __launch_bounds__(kBlock) void kNeighboursSpill(DeviceCtx c) {
+    __shared__ WaveLds lds[kWavesPerBlock];
+    if (stepHeld(c)) return;
+    const int wave = threadIdx.x >> 6, lane = laneId();
+    const uint32_t k = uniformU32(blockIdx.x * (uint32_t)kWavesPerBlock + (uint32_t)wave);
+    const uint32_t listed =
+        uniformU32((uint32_t)__hip_atomic_load(&c.nb.q[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    if (k >= listed) return;
+    const uint32_t gw = uniformU32(c.nb.spill[k]);
+    const uint32_t K = uniformU32(c.nb.waveNeed[gw]);
+    const uint32_t UC = uniformU32((min(c.nb.unionLen[gw], 1u << 16) + 63u) & ~63u);
+    const uint64_t listBytes = (128ull * K + 255ull) & ~255ull, bytes = listBytes + 4ull * UC;
+    unsigned long long at = 0;
+    if (lane == 0) at = atomicAdd(&c.nb.q[1], (unsigned long long)bytes);
+    at = ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(at >> 32), 0) << 32) |
+         (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)at, 0);
+    if (at + bytes > c.nb.poolBytes) {
+        if (lane == 0) {
+            atomicOr(&c.cnt.v[3], 1ull);
+            atomicOr(&c.cnt.v[kCntHold], 1ull);
+        }
+        return;
+    }
+    uint16_t* list = reinterpret_cast<uint16_t*>(c.nb.pool + at);
+    uint32_t* U = reinterpret_cast<uint32_t*>(c.nb.pool + at + listBytes);
+    const WaveBuild w = buildWave(c, lds[wave], gw, lane, list, K, U, UC);
+    const bool fits = finishWave(c, gw, lane, w, K, UC);
+    if (lane == 0) {
+        c.nb.tab[gw] = WaveTable{list, U, K, UC};
+        if (!fits) {  // K and UC are what the first build counted: a second walk finds the same set
+            atomicOr(&c.cnt.v[3], 1ull);
+            atomicOr(&c.cnt.v[kCntHold], 1ull);
+        }
+    }
+}
+
+__global__ void kNeighboursReset(DeviceCtx c) {
+    if (threadIdx.x < 2) c.nb.q[threadIdx.x] = 0ull;
 }
 
 }  // namespace
 
-void launchNeighbours(const DeviceCtx& c, hipStream_t s) { kNeighbours<<<c.blocks, kBlock, 0, s>>>(c); }
+void launchNeighbours(const DeviceCtx& c, hipStream_t s) {
+    kNeighboursReset<<<1, 64, 0, s>>>(c);
+    kNeighbours<<<c.blocks, kBlock, 0, s>>>(c);
+    kNeighboursSpill<<<c.blocks, kBlock, 0, s>>>(c);
+}
 
 }  // namespace astraea_hip

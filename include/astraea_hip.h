@@ -306,19 +306,21 @@ int astraea_relax(astraea_ctx* ctx, uint32_t steps, const float* anchor_temp_k,
  * particles), `out[3]` = table entries per particle, `out[4]` = sum of pass-1
  * neighbour counts (self excluded), `out[5]` = sum of pass-2 pairs, `out[6]` =
  * largest candidate count of one particle in the last table, `out[7]` = steps held by an
- * incomplete neighbour table and run again after it was widened (since the context was created).
+ * incomplete neighbour table and run again after it was widened (since the context was created),
+ * `out[8]` = wavefronts of the last table built in the spill pool, `out[9]` = pool bytes they use.
  * Up to `n` values. */
 int astraea_counters(astraea_ctx* ctx, uint64_t* out, int n);
 
-/* Width of the neighbour table, in entries per particle, rounded up to a multiple
- * of 32 (no counterpart in the reference, which walks the tree in every pass).  Each
+/* Width of the neighbour table's regular region, in entries per particle, rounded up to a
+ * multiple of 32 (no counterpart in the reference, which walks the tree in every pass).  Each
  * wavefront of 64 particles keeps the union of its particles' candidates (up to
  * min(32 * stride, 65536) ids, 4 bytes each, in four banks) and each particle 16-bit
- * positions into them, stride / 4 per bank, so the table takes 4 * stride * capacity
- * bytes.  A particle with more candidates in a bank than stride / 4, or a wavefront
- * with more than its union holds, holds the queued steps until the next synchronising call
- * widens the table and runs them (astraea_synchronize).  A force evaluation at load widens
- * it the same way. */
+ * positions into them, stride / 4 per bank, so the regular region takes 4 * stride * capacity
+ * bytes.  A wavefront with a particle that has more candidates in a bank than stride / 4, or
+ * more candidates than its union holds, is built again in a spill pool exactly as wide as it
+ * needs, so memory follows the candidates the particles have.  The pool grows when a table does
+ * not fit it: at a load, or for queued steps at the next synchronising call, which holds the
+ * steps until then and runs them (astraea_synchronize). */
 #define ASTRAEA_DEFAULT_NEIGHBOUR_STRIDE 320
 int astraea_set_neighbour_stride(astraea_ctx* ctx, uint32_t stride);
 

@@ -515,9 +515,10 @@ def test_neighbour_table_size_matches_brute_force(lib, orc, which):
     assert g["nbr_max"] <= g["nbr_stride"]
 
 
-def test_neighbour_table_widens_at_load(lib):
-    """A table narrower than a particle's candidate count is widened when forces are evaluated at
-    load (a synchronisation point), and the forces equal those of a context that never overflowed."""
+def test_neighbour_table_spills_at_load(lib):
+    """With a regular region narrower than the particles' candidate counts, every wavefront is built
+    in the spill pool (grown at the load, a synchronisation point), and the forces equal, bit for
+    bit, those of a context whose regular region holds them all."""
     from astraea_amd import scenario as S
 
     st = S.config_small()
@@ -540,17 +541,20 @@ def test_neighbour_table_widens_at_load(lib):
     ctx.set_neighbour_stride(16)
     got = forces(ctx)
     c = ctx.counters()
-    assert c["nbr_max"] > 16 and c["nbr_stride"] >= c["nbr_max"]
+    assert c["nbr_max"] > 16 and c["nbr_stride"] == 32
+    assert c["spilled_waves"] > 0 and c["spill_bytes"] > 0
+    assert ref.counters()["spilled_waves"] == 0
     assert np.array_equal(got, want)
 
 
 @pytest.mark.parametrize("graph", [False, True])
 def test_neighbour_table_overflow_in_steps_is_recovered(lib, graph):
     """Smoothing lengths far below equilibrium grow by 8 % per step, and the candidate counts with
-    them.  Thirty steps are queued at once on a table fitted to the loaded state: a step outgrows
-    it, holds the queued steps on the device, and the synchronising call widens the table,
-    finishes that step and runs the held ones.  The state after the run equals, bit for bit, the
-    run on a table wide enough from the start (the table's width never changes a pair's order)."""
+    them.  With a narrow regular region every wavefront spills, and the pool is sized to the loaded
+    state.  Thirty steps are queued at once: a step outgrows the pool, holds the queued steps on the
+    device, and the synchronising call grows the pool, finishes that step and runs the held ones.
+    The state after the run equals, bit for bit, the run on a regular region wide enough from the
+    start (where a wavefront's table lives never changes a pair's order)."""
     from astraea_amd import scenario as S
 
     st = S.config_small()
@@ -565,9 +569,9 @@ def test_neighbour_table_overflow_in_steps_is_recovered(lib, graph):
         ctx.set_layout([n])
         ctx.set_neighbour_stride(stride)
         ctx.load_state(st)
-        ctx.initial_forces()  # fits the table to the loaded state, with a quarter to spare
+        ctx.initial_forces()  # fits the pool to the loaded state, with a quarter to spare
         ctx.synchronize()
-        width = ctx.counters()["nbr_stride"]
+        width = ctx.counters()["spill_bytes"]
         ctx.step(MAX_DT, 30, graph=graph)
         ctx.synchronize()
         c = ctx.counters()
@@ -576,10 +580,10 @@ def test_neighbour_table_overflow_in_steps_is_recovered(lib, graph):
         return out, c, width, ctx.energy()
 
     want, cw, _, ew = run(2048)
-    assert cw["steps_redone"] == 0 and cw["nbr_max"] <= 2048
+    assert cw["steps_redone"] == 0 and cw["spilled_waves"] == 0
     got, cg, width, eg = run(32)
-    assert cg["steps_redone"] > 0, "the run never outgrew its table: the case tests nothing"
-    assert cg["nbr_stride"] > width
+    assert cg["steps_redone"] > 0, "the run never outgrew its pool: the case tests nothing"
+    assert cg["spill_bytes"] > width
     for k in want:
         assert np.array_equal(got[k], want[k]), k
     # the FP64 sums (block partials added by atomics in any order) agree to rounding; momenta and
