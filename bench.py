@@ -17,9 +17,28 @@ Beside it, the plain FP32 forms the metric is compared with:
     scenario.
 Timed regions: K steps bracketed by stream synchronisation.  A second region of K eager steps
 of the batched run records HIP events around every stage on the context stream; it gives the
-per-stage times and the gravity kernel's average launch duration used for the roofline.  The
-CPU baseline times the FP64 OpenMP oracle on a bounded sample of the batched workload (the
-first scenarios of the sweep, one step after their initial force evaluation).
+per-stage times and the gravity kernel's average launch duration used for the roofline.
+
+Two rooflines are printed.  `roofline` is the dominant kernel's (gravity: FP32 VALU, 23 FLOP per
+interaction).  `step_roofline` is the whole step's: achieved = sum over stages of
+max(B_s / 8 TB/s, F_s / 157.3 TFLOP/s) / measured step time, with the algorithmic bytes B_s and
+FLOPs F_s per particle of each stage (hydro step, 1,183 B + 120 B for the FP64 accumulators;
+F = 64 n1 + 100 n2 + 40 n2 + 23 I + 2 x 60) and n1, n2, I measured in the same run.  The
+neighbour-table build has no counterpart in the algorithm (B = F = 0): its time counts against
+the step.  `traffic_ratio` gives, per kernel, HBM bytes read and written (rocprofv3 FETCH_SIZE +
+WRITE_SIZE, calibrated) over its algorithmic bytes, from the committed counter summary.
+
+The energy drift of the timed run is reported normalised by |E0| and by |Phi0| (at 2-3 v_esc,
+K + Phi + U nearly cancels in some scenarios, so |E0| can be 1 % of |Phi0|).  The comparison
+with the FP64 oracle over 1000 steps reports each drift twice: with the Barnes-Hut potential
+(1/2 sum m phi_BH, what the step computes) and with the exact N^2 potential of the same
+positions (the oracle's direct sum), which separates the potential's measurement error from
+the integration's own energy error.
+
+The CPU baseline runs the FP64 OpenMP oracle on the host: config 0 in full (2 x 8,192
+particles, 100 adaptive KDK steps) and the 1,000,000-particle giant impact for 5 steps after
+its initial force evaluation (the reported value), with the core count, OpenMP threads and CPU
+model.
 
 Prints one JSON line.
 """
@@ -46,6 +65,15 @@ PEAK_HBM_GBS = 8000.0         # MI355X published HBM3E bandwidth
 # neighbour records cache-served, FP32 state): sort = 8 + 24 per 8-bit digit pass
 ALG_BYTES = {"sort": 200, "pass1": 126, "pass2": 151, "drift": 88}
 TRAFFIC_PROFILE = ROOT / "profiles" / "pmc_gravity_batched_sweep.md"
+KERNEL_TRAFFIC_PROFILE = ROOT / "profiles" / "pmc_step_batched_sweep.md"
+# Algorithmic bytes per particle-step of each timed stage (hydro step, FP32 state): kick 1 61,
+# drift + XSPH 88, bounds + Morton 36, radix sort 200, permutation 88 + Karras/leaves 123, moment
+# climb 123, pass 1 + EOS 126, gravity 68, pass 2 151, kick 2 + EOS 78 and the dt candidates 41
+# (computed in kick 2; the dt stage only clamps per segment), 1,183 in all.  The FP64
+# position/velocity accumulators add 120 (kick 1 24, drift 48, permutation 24, kick 2 24).
+STEP_BYTES = {"dt": 0, "kick1": 61 + 24, "drift": 88 + 48, "morton": 36, "sort": 200,
+              "permute_tree": 88 + 123 + 24, "moments": 123, "neighbours": 0, "pass1": 126, "gravity": 68,
+              "pass2": 151, "kick2": 78 + 41 + 24}
 
 
 def log(msg):
@@ -67,6 +95,47 @@ def gravity_traffic():
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
     return float(m.group(1)) if m else None
+
+
+def step_flops(stage, n1, n2, inter):
+    """Algorithmic FLOPs per particle of a stage: 64 per pass-1 pair, 100 per pass-2 pair, 40 per
+    XSPH pair, 23 per gravity interaction, 60 per EOS evaluation (after pass 1 and in kick 2)."""
+    return {"drift": 40.0 * n2, "pass1": 64.0 * n1 + 60.0, "gravity": 23.0 * inter, "pass2": 100.0 * n2,
+            "kick2": 60.0}.get(stage, 0.0)
+
+
+def step_roofline(stages, n, n1, n2, inter, step_ms):
+    """Sum over stages of max(B_s / HBM peak, F_s / FP32 peak) over the measured step time."""
+    rows, t_min = {}, 0.0
+    for k, b in STEP_BYTES.items():
+        if k not in stages:
+            continue
+        f = step_flops(k, n1, n2, inter)
+        t_b = n * b / (PEAK_HBM_GBS * 1e9) * 1e3
+        t_f = n * f / (PEAK_FP32_TFLOPS * 1e12) * 1e3
+        t = stages[k][0] / max(stages[k][1], 1)
+        rows[k] = {"bytes_per_particle": b, "flop_per_particle": f, "t_min_ms": max(t_b, t_f),
+                   "bound": "hbm" if t_b >= t_f else "fp32", "t_ms": t, "frac": max(t_b, t_f) / t if t > 0 else None}
+        t_min += max(t_b, t_f)
+    return {"definition": "sum_s max(B_s/8 TB/s, F_s/157.3 TFLOP/s) / step time", "achieved": t_min / step_ms,
+            "t_min_ms": t_min, "step_ms": step_ms, "n1": n1, "n2": n2, "I": inter,
+            "bytes_per_particle": sum(STEP_BYTES[k] for k in rows),
+            "flop_per_particle": sum(r["flop_per_particle"] for r in rows.values()), "stages": rows}
+
+
+def kernel_traffic_ratios():
+    """Per kernel: counter bytes (FETCH_SIZE + WRITE_SIZE, calibrated) over algorithmic bytes per
+    launch, from the committed rocprofv3 --pmc summary of this workload (rows
+    `ratio: <kernel> <counter bytes> <algorithmic bytes>`), or None."""
+    try:
+        text = KERNEL_TRAFFIC_PROFILE.read_text()
+    except OSError:
+        return None
+    out = {}
+    for m in re.finditer(r"^ratio:\s*(\S+)\s+([0-9.eE+]+)\s+([0-9.eE+]+)", text, re.M):
+        c, a = float(m.group(2)), float(m.group(3))
+        out[m.group(1)] = {"counter_bytes": c, "algorithmic_bytes": a, "ratio": c / a if a > 0 else None}
+    return out or None
 
 
 def time_steps(ctx, steps, graph=True):
@@ -101,9 +170,11 @@ def run_batched(A, state, counts, steps, warmup, autotune=True):
     ctx.counters()
     log(f"[bench] batched: timed {steps} steps")
     wall = time_steps(ctx, steps)
-    e_end = energy_total(ctx.energy())
+    e_full = ctx.energy()
+    e_end = energy_total(e_full)
     cnt = ctx.counters()
     drift = (e_end - e_start) / np.maximum(np.abs(e_init), 1.0)
+    drift_phi = (e_end - e_start) / np.abs(e_full[:, 2])
 
     log(f"[bench] batched: instrumented {steps} steps")
     ctx.enable_stage_timing(True)
@@ -112,7 +183,7 @@ def run_batched(A, state, counts, steps, warmup, autotune=True):
     stages = ctx.stage_times()
     ctx.enable_stage_timing(False)
     cnt2 = ctx.counters()
-    return dict(wall=wall, drift=drift, cnt=cnt, cnt2=cnt2, stages=stages, n=n, tune=tune)
+    return dict(wall=wall, drift=drift, drift_phi=drift_phi, cnt=cnt, cnt2=cnt2, stages=stages, n=n, tune=tune)
 
 
 def run_sequential_plain(A, states, counts, steps, warmup):
@@ -145,31 +216,87 @@ def run_single_plain(A, steps, warmup, n_target, n_impactor):
     return n * steps / time_steps(ctx, steps)
 
 
-def cpu_baseline(states, counts, params, min_particles):
-    """FP64 OpenMP oracle on the first scenarios of the sweep (at least `min_particles` in all):
-    initial forces, then one full step timed."""
+def host_info():
+    import os
     import oracle
 
-    k, n = 0, 0
-    while k < len(counts) and n < min_particles:
-        n += counts[k]
-        k += 1
-    merged = {f: np.ascontiguousarray(np.concatenate([s[f] for s in states[:k]])) for f in states[0]}
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"nproc": usable, "cpu_count": os.cpu_count(), "omp_max_threads": oracle.num_threads(), "cpu_model": model}
+
+
+def cpu_baseline(A):
+    """FP64 OpenMP oracle on the host: config 0 in full (2 x 8,192, 100 adaptive KDK steps) and the
+    1,000,000-particle giant impact for 5 steps after its initial force evaluation.  Wall clock
+    around the step loop only."""
+    import oracle
+    from astraea_amd import scenario as S
+
+    params = S.plain_params(precision=A.PREC_FP64_ACCUM)
+    info = host_info()
+    log("[bench] CPU baseline: config 0, 2 x 8,192 particles, 100 steps (FP64 oracle)")
+    st = S.config_small()
+    n0 = len(st["mass"])
     o = oracle.Oracle(params, group_size=params.group_size or 64)
-    o.set_layout(counts[:k])
-    o.load_state(merged)
+    o.set_layout([n0])
+    o.load_state(st)
+    o.initial_forces()
+    e0 = o.energy()
+    t0 = time.perf_counter()
+    o.step(MAX_DT, 100)
+    c0_s = time.perf_counter() - t0
+    e1 = o.energy()
+    tot0, tot1 = float(energy_total(e0)[0]), float(energy_total(e1)[0])
+    c0 = {"particles": n0, "steps": 100, "seconds": c0_s, "particle_steps_per_s": n0 * 100 / c0_s,
+          "energy_drift_rel_E0": (tot1 - tot0) / abs(tot0), "energy_drift_rel_Phi0": (tot1 - tot0) / abs(e0[0, 2])}
+    del o
+    log("[bench] CPU baseline: 1,000,000-particle giant impact, 5 steps (FP64 oracle)")
+    st = S.config_giant()
+    n1 = len(st["mass"])
+    o = oracle.Oracle(params, group_size=params.group_size or 64)
+    o.set_layout([n1])
+    o.load_state(st)
     o.initial_forces()
     t0 = time.perf_counter()
-    o.step(MAX_DT, 1)
-    dt = time.perf_counter() - t0
+    o.step(MAX_DT, 5)
+    c1_s = time.perf_counter() - t0
+    del o
     return {
-        "value": n / dt,
+        "value": n1 * 5 / c1_s,
         "unit": "particle-steps/s",
-        "cores": oracle.num_threads(),
+        "cores": info["omp_max_threads"],
         "kind": "port",
-        "sample": f"1 full KDK step of the first {k} scenarios of the sweep ({n:,} particles) after their initial "
-                  f"force evaluation, FP64, {oracle.num_threads()} OpenMP threads ({dt:.1f} s)",
+        "sample": f"FP64 OpenMP oracle, {info['omp_max_threads']} threads: the 1,000,000-particle giant impact, 5 full "
+                  f"KDK steps after its initial force evaluation ({c1_s:.1f} s, {1e3 * c1_s / 5:.0f} ms per step); "
+                  f"config 0 in full beside it",
+        "config1": {"particles": n1, "steps": 5, "seconds": c1_s, "ms_per_step": 1e3 * c1_s / 5,
+                    "particle_steps_per_s": n1 * 5 / c1_s},
+        "config0": c0,
+        "host": info,
     }
+
+
+def exact_potential(oracle, state64, params, counts):
+    """1/2 sum m phi of the direct N^2 sum (the oracle's gravity at theta = 0, Plummer softening
+    soft_factor x h0 as in the step), per segment, for positions given in FP64."""
+    q = oracle.default_params(theta=0.0, soft_factor=params.soft_factor)
+    e = oracle.Oracle(q, 64)
+    e.set_layout(counts)
+    e.load_state({"pos_x": state64["pos_x"].astype(np.float32), "pos_y": state64["pos_y"].astype(np.float32),
+                  "pos_z": state64["pos_z"].astype(np.float32), "mass": state64["mass"], "h": state64["h0"],
+                  "h0": state64["h0"]})
+    for k in ("x", "y", "z"):
+        e.upload("pos64_" + k, state64["pos_" + k])
+    e.sort_and_build()
+    e.gravity()
+    return e.energy()[:, 2]
 
 
 def drift_vs_oracle(A, steps, specs):
@@ -189,21 +316,60 @@ def drift_vs_oracle(A, steps, specs):
     ctx.initial_forces()
     ctx.synchronize()
     g0 = energy_total(ctx.energy())
+    ge0 = ctx.energy()
+
+    def gpu_state():
+        # the segment's particles in original-id order, FP64 positions (the accumulators)
+        ids = ctx.download("original_id")
+        out = {}
+        for k, src in (("pos_x", "pos64_x"), ("pos_y", "pos64_y"), ("pos_z", "pos64_z"), ("mass", "mass"),
+                       ("h0", "h0")):
+            v = ctx.download(src)
+            w = np.empty_like(v)
+            w[ids] = v
+            out[k] = w
+        return out
+
+    px0 = exact_potential(oracle, gpu_state(), params, counts)
     ctx.step(MAX_DT, steps, graph=True)
     ctx.synchronize()
     g1 = energy_total(ctx.energy())
+    ge1 = ctx.energy()
+    px1 = exact_potential(oracle, gpu_state(), params, counts)
     o = oracle.Oracle(params, group_size=params.group_size or 64)
     o.set_layout(counts)
     o.load_state(state)
     o.initial_forces()
     r0 = energy_total(o.energy())
+    re0 = o.energy()
+
+    def oracle_state():
+        ids = o.download("original_id")
+        out = {}
+        for k, src in (("pos_x", "pos64_x"), ("pos_y", "pos64_y"), ("pos_z", "pos64_z"), ("mass", "mass"),
+                       ("h0", "h0")):
+            v = o.download64(src) if src.startswith("pos64") else o.download(src)
+            w = np.empty_like(v)
+            w[ids] = v
+            out[k] = w
+        return out
+
+    rx0 = exact_potential(oracle, oracle_state(), params, counts)
     t0 = time.perf_counter()
     o.step(MAX_DT, steps)
     t_oracle = time.perf_counter() - t0
     r1 = energy_total(o.energy())
+    re1 = o.energy()
+    rx1 = exact_potential(oracle, oracle_state(), params, counts)
     dg = (g1 - g0) / np.maximum(np.abs(g0), 1.0)
     dr = (r1 - r0) / np.maximum(np.abs(r0), 1.0)
     diff = np.abs(dg - dr)
+    phi0 = np.abs(re0[:, 2])
+    # drifts relative to |Phi0|, with the BH potential and with the exact one
+    bh_g = (g1 - g0) / phi0
+    bh_r = (r1 - r0) / phi0
+    ex_g = ((g1 - ge1[:, 2] + px1) - (g0 - ge0[:, 2] + px0)) / phi0
+    ex_r = ((r1 - re1[:, 2] + rx1) - (r0 - re0[:, 2] + rx0)) / phi0
     return {
         "steps": steps,
         "scenarios": len(counts),
@@ -212,6 +378,13 @@ def drift_vs_oracle(A, steps, specs):
         "median_abs_vs_fp64": float(np.median(diff)),
         "gpu_max_abs": float(np.abs(dg).max()),
         "fp64_max_abs": float(np.abs(dr).max()),
+        "relative_to_abs_phi0": {
+            "bh_potential": {"gpu_max_abs": float(np.abs(bh_g).max()), "fp64_max_abs": float(np.abs(bh_r).max()),
+                             "max_abs_gpu_minus_fp64": float(np.abs(bh_g - bh_r).max())},
+            "exact_potential": {"gpu_max_abs": float(np.abs(ex_g).max()), "fp64_max_abs": float(np.abs(ex_r).max()),
+                                "max_abs_gpu_minus_fp64": float(np.abs(ex_g - ex_r).max())},
+            "bh_potential_error_at_start_max": float(np.abs((re0[:, 2] - rx0) / phi0).max()),
+        },
         "oracle_seconds": t_oracle,
     }
 
@@ -224,7 +397,6 @@ def main():
     ap.add_argument("--scenarios", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP32 comparison runs")
-    ap.add_argument("--cpu-particles", type=int, default=1_000_000)
     ap.add_argument("--drift-steps", type=int, default=1000,
                     help="steps of the energy-drift comparison with the FP64 oracle (0: skip)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -253,6 +425,12 @@ def main():
     inter_per_launch = b["cnt2"]["interactions"] / max(grav_n, 1)
     achieved = FLOP_PER_INTERACTION * inter_per_launch / (grav_avg_ms * 1e-3) / 1e12
     drift = b["drift"]
+    drift_phi = b["drift_phi"]
+    cnt = b["cnt2"]
+    steps_instr = K
+    n1 = cnt["pass1_pairs"] / n
+    n2 = cnt["pass2_pairs"] / n
+    inter_pp = cnt["interactions"] / (n * steps_instr)
 
     out = {
         "metric": METRIC,
@@ -281,7 +459,9 @@ def main():
             "parallelism": "single GPU",
         },
         "energy_drift_rel": {"max_abs": float(np.max(np.abs(drift))), "median_abs": float(np.median(np.abs(drift))),
-                             "steps": K},
+                             "steps": K, "normalised_by": "max(|E0|, 1)",
+                             "by_abs_phi": {"max_abs": float(np.max(np.abs(drift_phi))),
+                                            "median_abs": float(np.median(np.abs(drift_phi)))}},
         "interactions_per_particle": b["cnt"]["interactions"] / (n * K),
         "gravity_forced_accepts": int(b["cnt"]["forced_accepts"]),
         "neighbour_candidates_per_particle": b["cnt2"]["nbr_entries"] / n,
@@ -292,6 +472,8 @@ def main():
         "gravity_interactions_per_s": inter_per_launch / (grav_avg_ms * 1e-3),
         "tuning": b["tune"] and {"chosen": b["tune"]["chosen"], "steps": 8,
                                  "ms_per_step": {k: b["tune"][k] for k in ("pass1_ms", "pass2_ms", "drift_ms")}},
+        "step_roofline": step_roofline(stages, n, n1, n2, inter_pp, 1e3 * b["wall"] / K),
+        "traffic_ratio": kernel_traffic_ratios(),
         "roofline": {
             "bound": "valu",
             "kernel": "kGravity",
@@ -322,8 +504,7 @@ def main():
             A, args.drift_steps, S.sweep_scenarios(8, n_min=1500, n_max=3000))
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle)")
-        out["cpu_baseline"] = cpu_baseline(states, counts, S.plain_params(precision=A.PREC_FP64_ACCUM),
-                                           args.cpu_particles)
+        out["cpu_baseline"] = cpu_baseline(A)
     print(json.dumps(out), flush=True)
 
 
