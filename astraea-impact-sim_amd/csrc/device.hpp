@@ -126,10 +126,15 @@ __device__ __forceinline__ void waveSync() {
 
 // Walks the neighbours of slot i.  Every lane of the wavefront must call it together (inactive
 // slots have no entries).  Bank by bank, the wave's union is staged through `buf` (S records per
-// round): the round's slot ids are loaded together, then stage(slot, rec) fills the records with
-// coalesced loads; each lane consumes its entries that fall in the round, pair(rec) in table
-// order, reading the next entry's record from LDS before it runs pair on the current one.
-// Past the end of a bank a round stages copies of its last slot (never read).
+// round): the round's slot ids are loaded together, then load(slot, rec) fills the records with
+// coalesced loads and finish(rec) derives what a record needs from them; each lane consumes its
+// entries that fall in the round, pair(rec) in table order, reading the next entry's record from
+// LDS before it runs pair on the current one.  Past the end of a bank a round stages copies of
+// its last slot (never read).
+//
+// (Keeping the next round's records in flight while a round is evaluated does not pay here: the
+// wait for an entry chunk inside the pair loop is a wait for every older load, and the records
+// held across the round cost a wavefront per SIMD.)
 //
 // The records sit in LDS quad-major: 16-byte quad q of record k at quad q * S + k of the buffer
 // (the same S * sizeof(Rec) bytes).  Lanes read records at scattered positions; with records
@@ -146,9 +151,29 @@ __device__ __forceinline__ Rec ldsRecord(const float4* q, uint32_t k) {
     return r;
 }
 
-template <int S, class Rec, class Stage, class Pair>
-__device__ __forceinline__ void forEachNeighbour(const Neighbours& nb, uint32_t i, Rec* buf, Stage&& stage,
-                                                 Pair&& pair) {
+// Native 4- and 2-float vectors for the records a neighbour pass stages: the HIP vector types
+// copy through private memory when they are assigned, which keeps a record that lives across a
+// loop iteration out of registers; these are plain register values.
+typedef float Vec4 __attribute__((ext_vector_type(4)));
+typedef float Vec2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ Vec4 loadVec(const float4* a, uint32_t j) { return reinterpret_cast<const Vec4*>(a)[j]; }
+__device__ __forceinline__ Vec2 loadVec(const float2* a, uint32_t j) { return reinterpret_cast<const Vec2*>(a)[j]; }
+__device__ __forceinline__ float4 toFloat4(Vec4 v) { return make_float4(v.x, v.y, v.z, v.w); }
+__device__ __forceinline__ float2 toFloat2(Vec2 v) { return make_float2(v.x, v.y); }
+
+// A pointer read from device memory (a WaveTable's) is generic to the compiler, which then
+// issues flat loads: they count against both the vector-memory and the LDS wait counters, so
+// every wait for one also waits for LDS.  The table lives in global memory; this says so.
+template <class T>
+using GlobalPtr = const T __attribute__((address_space(1)))*;
+template <class T>
+__device__ __forceinline__ GlobalPtr<T> globalView(const T* p) {
+    return (GlobalPtr<T>)p;
+}
+
+template <int S, class Rec, class Load, class Finish, class Pair>
+__device__ __forceinline__ void forEachNeighbour(const Neighbours& nb, uint32_t i, Rec* buf, Load&& load,
+                                                 Finish&& finish, Pair&& pair) {
     static_assert(S % 64 == 0, "a round is whole wavefront loads");
     static_assert(sizeof(Rec) % 16 == 0 && alignof(Rec) >= 16, "records are whole 16-byte quads");
     constexpr int NQ = (int)(sizeof(Rec) / 16);
@@ -162,17 +187,21 @@ __device__ __forceinline__ void forEachNeighbour(const Neighbours& nb, uint32_t 
     const uint2 counts = nb.count[i];
     const uint32_t perBank = t.stride / (8u * kBanks);
     const uint32_t lastChunk = perBank - 1u;
+    // entry chunks as native vectors: a HIP uint4 read through a global pointer becomes a copy
+    // through private memory
+    typedef uint32_t Chunk __attribute__((ext_vector_type(4)));
+    const GlobalPtr<Chunk> chunks = globalView(reinterpret_cast<const Chunk*>(t.list)) + lane;
 #pragma unroll 1
     for (uint32_t bank = 0; bank < kBanks; ++bank) {
         const uint32_t len = (total + (kBanks - 1u) - bank) / kBanks;
         if (len == 0u) break;  // banks fill in turn: the later ones are empty too
-        const uint32_t* U = t.uni + bank * bankCap;
+        const GlobalPtr<uint32_t> U = globalView(t.uni) + bank * bankCap;
         const uint32_t cnt = bankCount(counts, bank);
         // chunks are loaded one ahead, unconditionally (clamped to the bank's last chunk: entries
         // past cnt are never used), so that no load result has to be merged with an older value
-        const uint4* ch = reinterpret_cast<const uint4*>(t.list) + lane + 64ull * (bank * perBank);
-        uint4 cur = ch[0];
-        uint4 nxt = ch[64ull * min(1u, lastChunk)];
+        const GlobalPtr<Chunk> ch = chunks + 64ull * (bank * perBank);
+        Chunk cur = ch[0];
+        Chunk nxt = ch[64ull * min(1u, lastChunk)];
         uint32_t k = 0;
         uint32_t e = cnt > 0u ? (cur.x & 0xffffu) : kPast;  // position of entry k; entries ascend, so e >= s0
         // moves to entry k + 1 and returns its position (kPast after the last)
@@ -197,11 +226,12 @@ __device__ __forceinline__ void forEachNeighbour(const Neighbours& nb, uint32_t 
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 Rec rec;
-                stage(id[r], rec);
-                float4 t[NQ];
-                __builtin_memcpy(t, &rec, sizeof(Rec));
+                load(id[r], rec);
+                finish(rec);
+                float4 q[NQ];
+                __builtin_memcpy(q, &rec, sizeof(Rec));
 #pragma unroll
-                for (int x = 0; x < NQ; ++x) quads[x * S + 64 * r + lane] = t[x];
+                for (int x = 0; x < NQ; ++x) quads[x * S + 64 * r + lane] = q[x];
             }
             waveSync();
             // two records alternate (a holds entry k when the loop body starts), so that a record

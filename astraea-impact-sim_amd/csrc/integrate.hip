@@ -171,7 +171,7 @@ __global__ __launch_bounds__(kBlock) void kDtCandidates(DeviceCtx c) {
 template <bool FP64, int S>
 __global__ __launch_bounds__(kBlock) void kDrift(DeviceCtx c) {
     struct Rec {
-        float4 p, v;
+        Vec4 p, v;
         float H, invH, wNorm;  // H_j = 2 max(h_j, 1e-6) and its kernel constants, once per record
     };
     __shared__ float sb[kWavesPerBlock][6];
@@ -193,15 +193,18 @@ __global__ __launch_bounds__(kBlock) void kDrift(DeviceCtx c) {
     forEachNeighbour<S>(
         c.nb, i, stageBuf[wave],
         [&](uint32_t j, Rec& r) {
-            r.p = c.cur.pos[j];
-            r.v = c.cur.vel[j];
-            r.H = 2.0f * maxPos(c.d.nb[j].x, 1.0e-6f);
+            r.p = loadVec(c.cur.pos, j);
+            r.v = loadVec(c.cur.vel, j);
+            r.H = c.d.nb[j].x;
+        },
+        [&](Rec& r) {
+            r.H = 2.0f * maxPos(r.H, 1.0e-6f);
             const Support sj = supportOf(r.H);
             r.invH = sj.invH;
             r.wNorm = sj.wNorm;
         },
         [&](const Rec& n) {
-            const float4 pj = n.p, vj = n.v;
+            const float4 pj = toFloat4(n.p), vj = toFloat4(n.v);
             const float Hj = n.H;
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float r2 = dist2f(dx, dy, dz);

@@ -47,7 +47,7 @@ __device__ __forceinline__ float maxPrincipalStress(float p, float sxx, float sy
 template <bool STRENGTH, int S>
 __global__ __launch_bounds__(kBlock) void kPass1(DeviceCtx c) {
     struct Rec {
-        float4 p, v;
+        Vec4 p, v;
     };
     __shared__ Rec stageBuf[kWavesPerBlock][S];
     if (stepHeld(c)) return;
@@ -81,12 +81,12 @@ __global__ __launch_bounds__(kBlock) void kPass1(DeviceCtx c) {
     forEachNeighbour<S>(
         c.nb, i, stageBuf[threadIdx.x >> 6],
         [&](uint32_t j, Rec& r) {
-            r.p = c.cur.pos[j];
-            r.v = c.cur.vel[j];
-            r.v.w = __fdiv_rn(r.p.w, fmaxf(r.v.w, 1.0e-9f));  // V_j = m_j / rho_j, once per record
+            r.p = loadVec(c.cur.pos, j);
+            r.v = loadVec(c.cur.vel, j);
         },
+        [&](Rec& r) { r.v.w = __fdiv_rn(r.p.w, fmaxf(r.v.w, 1.0e-9f)); },  // V_j = m_j / rho_j, once per record
         [&](const Rec& n) {
-            const float4 pj = n.p, vj = n.v;
+            const float4 pj = toFloat4(n.p), vj = toFloat4(n.v);
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float r2 = dist2f(dx, dy, dz);
             if (!(r2 < Hi2)) return;
@@ -185,14 +185,14 @@ __global__ __launch_bounds__(kBlock) void kEos(DeviceCtx c) {
 // smaller record lets five blocks share a CU instead of four).
 template <bool STRENGTH>
 struct Pass2Rec {
-    float4 p, n, v;  // n.x holds max(h_j, 1e-6)
+    Vec4 p, n, v;  // n.x holds max(h_j, 1e-6)
     uint32_t mat;
     float invH, gNorm;  // the kernel gradient's constants of H_j, once per staged record
 };
 template <>
 struct Pass2Rec<true> {
-    float4 p, n, v, s0;
-    float2 s1;
+    Vec4 p, n, v, s0;
+    Vec2 s1;
     uint32_t mat;
 };
 
@@ -241,14 +241,18 @@ __global__ __launch_bounds__(kBlock) void kPass2(DeviceCtx c) {
     forEachNeighbour<S>(
         c.nb, i, stageBuf[threadIdx.x >> 6],
         [&](uint32_t j, Rec& r) {
-            r.p = c.cur.pos[j];
-            r.n = c.d.nb[j];
-            r.v = c.cur.vel[j];
-            r.mat = c.cur.meta[j] & 0xffu;
+            r.p = loadVec(c.cur.pos, j);
+            r.n = loadVec(c.d.nb, j);
+            r.v = loadVec(c.cur.vel, j);
+            r.mat = c.cur.meta[j];
             if constexpr (STRENGTH) {
-                r.s0 = c.d.sr0[j];
-                r.s1 = c.d.sr1[j];
-            } else {
+                r.s0 = loadVec(c.d.sr0, j);
+                r.s1 = loadVec(c.d.sr1, j);
+            }
+        },
+        [&](Rec& r) {
+            r.mat &= 0xffu;
+            if constexpr (!STRENGTH) {
                 r.n.x = maxPos(r.n.x, 1.0e-6f);
                 const Support sj = supportOf(2.0f * r.n.x);
                 r.invH = sj.invH;
@@ -256,7 +260,7 @@ __global__ __launch_bounds__(kBlock) void kPass2(DeviceCtx c) {
             }
         },
         [&](const Rec& n) {
-            const float4 pj = n.p, nbj = n.n, vj = n.v;
+            const float4 pj = toFloat4(n.p), nbj = toFloat4(n.n), vj = toFloat4(n.v);
             const float hj = STRENGTH ? maxPos(nbj.x, 1.0e-6f) : nbj.x, Hj = 2.0f * hj;
             const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             const float r2 = dist2f(dx, dy, dz);
@@ -306,8 +310,8 @@ __global__ __launch_bounds__(kBlock) void kPass2(DeviceCtx c) {
             duAv += 0.5f * mj * piIJ * vdg;
             duT += 0.5f * mj * tens * vdg;
             if constexpr (STRENGTH) {
-                const float4 sj0 = n.s0;
-                const float2 sj1 = n.s1;
+                const float4 sj0 = toFloat4(n.s0);
+                const float2 sj1 = toFloat2(n.s1);
                 const float sxx = srI0.x + sj0.x, syy = srI0.y + sj0.y, szz = srI0.z + sj0.z;
                 const float sxy = srI0.w + sj0.w, sxz = srI1.x + sj1.x, syz = srI1.y + sj1.y;
                 ax += mj * (sxx * gvx + sxy * gvy + sxz * gvz);
