@@ -33,7 +33,7 @@ constexpr int kSCap = 1024;                  // per-wave node stack
 constexpr int kCCap = 256;                   // per-wave candidate records
 constexpr int kFlushAt = kCCap - 2 * kWave;  // one round lists at most 128 leaves
 constexpr int kBucket = 16;                  // an open node with this many leaves or fewer is listed whole
-constexpr int kBCap = 256;                   // per-wave listed buckets
+constexpr int kBCap = 192;                   // per-wave listed buckets (a round adds at most 128)
 constexpr int kBucketFlushAt = kBCap - 2 * kWave;
 constexpr int kSlotBits = 28;                // a bucket: first leaf slot | (leaves - 1) << kSlotBits
 constexpr uint32_t kBucketSlotMask = (1u << kSlotBits) - 1u;
@@ -44,7 +44,31 @@ struct WaveLds {
     float4 cand[kCCap];  // candidate particle (x, y, z, squared support bound)
     int candId[kCCap];
     uint32_t bucket[kBCap];
+    int selfAt[kWave];   // where lane l's own leaf sits among the listed candidates (-1: not listed)
 };
+
+// Exclusive prefix parity of a 32-bit mask: bit u is the parity of the set bits of x below u.
+__device__ __forceinline__ uint32_t prefixParityBelow(uint32_t x) {
+    uint32_t p = x;
+    p ^= p << 1;
+    p ^= p << 2;
+    p ^= p << 4;
+    p ^= p << 8;
+    p ^= p << 16;
+    return p ^ x;
+}
+
+// Four 16-bit entries' worth of a lane's current chunk of one bank, filled by shifting: after
+// eight appends entry 0 is the low half of `a`, entry 7 the high half of `d` (the table's order).
+struct ChunkAcc {
+    uint32_t a, b, c, d;
+};
+__device__ __forceinline__ void appendEntry(ChunkAcc& k, uint32_t pos) {
+    k.a = __builtin_amdgcn_alignbit(k.b, k.a, 16u);
+    k.b = __builtin_amdgcn_alignbit(k.c, k.b, 16u);
+    k.c = __builtin_amdgcn_alignbit(k.d, k.c, 16u);
+    k.d = __builtin_amdgcn_alignbit(pos, k.d, 16u);
+}
 
 // Wide walk: the wave expands up to 64 stack nodes per round, one per lane.  A node must be opened
 // when some group particle's sphere of radius max(Hb_p, node's max Hb) meets its box.  The group is
@@ -57,10 +81,17 @@ struct WaveLds {
 // than its particles' spheres; nodes that only such a sub-group reaches are tested exactly, one
 // node at a time across the lanes (lane = particle), with the node's box broadcast from the lane
 // that loaded it.  Leaf children are loaded by their lanes in parallel, kept when within reach of
-// the group's box, and staged in LDS, where every lane tests its own particle against them:
+// a sub-group's box (box distance is a lower bound of every member's distance, in FP32 as well,
+// so no hit is lost), and staged in LDS, where every lane tests its own particle against them:
 // 32 candidates into a hit mask, then the lane appends its hits in candidate order, so the
 // bookkeeping of an entry runs once per hit rather than once per candidate.  The candidates hit by
 // any lane form the wave's union list; a lane's entry is the hit's position in it.
+//
+// Entries are collected per lane and bank in registers (ChunkAcc) and each chunk of eight is
+// written once, as one 16-byte store, when it fills (a partial last chunk when the walk ends).
+// Which bank a hit goes to follows from its rank among the 32 candidates' union members (rank
+// mod 4, from two prefix parities of the wave-uniform union mask), so the banks are handled one at
+// a time with their registers known at compile time.
 struct WaveBuild {
     uint64_t cnt4;    // entries per bank of this lane's particle: bank b in bits 16 b .. 16 b + 15
     uint32_t ulen;    // length of the wavefront's union (uncapped)
@@ -91,10 +122,6 @@ __device__ __forceinline__ WaveBuild buildWave(const DeviceCtx& c, WaveLds& L, u
             Hb = supportBound(hu.x, hu.y);
         }
         const float Hb2 = Hb * Hb;
-        const float lx = waveMinF(active ? pi.x : 3.0e38f), hx = waveMaxF(active ? pi.x : -3.0e38f);
-        const float ly = waveMinF(active ? pi.y : 3.0e38f), hy = waveMaxF(active ? pi.y : -3.0e38f);
-        const float lz = waveMinF(active ? pi.z : 3.0e38f), hz = waveMaxF(active ? pi.z : -3.0e38f);
-        const float HbMax = waveMaxF(active ? Hb : 0.0f);
         // sub-group boxes (lanes 16q .. 16q+15); a sub-group without particles gets an empty box
         float sl[4][3], sh[4][3], sH[4];
         bool compact[4];
@@ -120,19 +147,27 @@ __device__ __forceinline__ WaveBuild buildWave(const DeviceCtx& c, WaveLds& L, u
         }
         const uint64_t lt = (1ull << lane) - 1ull;
         int sp = 1, nl = 0, nr = 0;
+        ChunkAcc acc[kBanks] = {};
         L.stack[0] = (int)off;
+        L.selfAt[lane] = -1;
         __builtin_amdgcn_wave_barrier();
-        // a leaf j as a candidate record, kept when within reach of the group's box
+        // a leaf j as a candidate record, kept when within reach of a sub-group's box (an empty
+        // sub-group's inverted box is out of reach of everything)
         const auto leafCandidate = [&](uint32_t j, float4& q) -> bool {
             const float4 pj = c.cur.pos[j];
             const float4 hj = c.cur.hu[j];
             const float hbj = supportBound(hj.x, hj.y);
             q = make_float4(pj.x, pj.y, pj.z, hbj * hbj);  // max(a, b)^2 = max(a^2, b^2) in FP32 too
-            const float reach = fmaxf(HbMax, hbj);
-            const float gx = fmaxf(0.0f, fmaxf(lx - q.x, q.x - hx));
-            const float gy = fmaxf(0.0f, fmaxf(ly - q.y, q.y - hy));
-            const float gz = fmaxf(0.0f, fmaxf(lz - q.z, q.z - hz));
-            return dist2f(gx, gy, gz) < reach * reach;
+            bool keep = false;
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const float reach = fmaxf(sH[g], hbj);
+                const float gx = fmaxf(0.0f, fmaxf(sl[g][0] - q.x, q.x - sh[g][0]));
+                const float gy = fmaxf(0.0f, fmaxf(sl[g][1] - q.y, q.y - sh[g][1]));
+                const float gz = fmaxf(0.0f, fmaxf(sl[g][2] - q.z, q.z - sh[g][2]));
+                keep = keep || dist2f(gx, gy, gz) < reach * reach;
+            }
+            return keep;
         };
         const auto listLeaves = [&](bool keep, const float4& q, uint32_t j) {
             const uint64_t mList = ballot(keep);
@@ -140,6 +175,8 @@ __device__ __forceinline__ WaveBuild buildWave(const DeviceCtx& c, WaveLds& L, u
                 const int at = nl + __popcll(mList & lt);
                 L.cand[at] = q;
                 L.candId[at] = (int)j;
+                const uint32_t rel = j - gw * kWave;  // the wavefront's own particle: lane rel
+                if (rel < (uint32_t)kWave) L.selfAt[rel] = at;
             }
             nl += __popcll(mList);
         };
@@ -148,6 +185,7 @@ __device__ __forceinline__ WaveBuild buildWave(const DeviceCtx& c, WaveLds& L, u
         // lane (a wave-uniform mask from the ballots) join the union, in candidate order
         const auto testCandidates = [&]() {
             __builtin_amdgcn_wave_barrier();
+            const int selfAt = L.selfAt[lane];  // a leaf is listed once per walk
             for (int c0 = 0; c0 < nl; c0 += 32) {
                 uint32_t m = 0u, any = 0u;
 #pragma unroll
@@ -164,27 +202,42 @@ __device__ __forceinline__ WaveBuild buildWave(const DeviceCtx& c, WaveLds& L, u
                     any &= (1u << left) - 1u;
                 }
                 if (!active) m = 0u;
+                if (selfAt >= c0 && selfAt < c0 + 32) m &= ~(1u << (selfAt - c0));  // j != i
                 if (lane < 32 && ((any >> lane) & 1u)) {
                     const uint32_t q = ulen + (uint32_t)__popc(any & ((1u << lane) - 1u));
                     if (q < UC) U[(q % kBanks) * UCb + q / kBanks] = (uint32_t)L.candId[c0 + lane];
                 }
-                while (m != 0u) {
-                    const int u = __builtin_ctz(m);
-                    m &= m - 1u;
-                    const uint32_t jj = (uint32_t)L.candId[c0 + u];
-                    if (jj == i) continue;
-                    const uint32_t q = ulen + (uint32_t)__popc(any & ((1u << u) - 1u));
-                    const uint32_t bk = q % kBanks, sh = bk << 4;
-                    const uint32_t cb = (uint32_t)(cnt4 >> sh) & 0xffffu;
-                    // past the union's capacity the entry is only counted: the wavefront is built
-                    // again in the spill pool with the width the counts ask for
-                    if (cb < Kb && q < UC)
-                        reinterpret_cast<uint16_t*>(out + 64ull * (bk * perBank + (cb >> 3)))[cb & 7u] =
-                            (uint16_t)(q / kBanks);
-                    cnt4 += 1ull << sh;
+                // union member u has rank r(u) among the members; it goes to bank (ulen + r) % 4 at
+                // position (ulen + r) / 4.  Label r % 4 = 2 (parity of the odd-ranked members below
+                // u) + (parity of the members below u).
+                const uint32_t par1 = prefixParityBelow(any);
+                const uint32_t par2 = prefixParityBelow(any & par1);
+                const uint32_t lab[4] = {any & ~par1 & ~par2, any & par1 & ~par2, any & ~par1 & par2, any & par1 & par2};
+#pragma unroll
+                for (uint32_t bk = 0; bk < kBanks; ++bk) {
+                    const uint32_t l = (bk - ulen) & 3u;
+                    const uint32_t mb = lab[l];
+                    const uint32_t first = (ulen + l) / kBanks;  // position of mb's first member
+                    uint32_t hits = m & mb;
+                    const uint32_t sh = bk << 4;
+                    uint32_t cb = (uint32_t)(cnt4 >> sh) & 0xffffu;
+                    cnt4 += (uint64_t)__popc(hits) << sh;
+                    while (hits != 0u) {
+                        const int u = __builtin_ctz(hits);
+                        hits &= hits - 1u;
+                        appendEntry(acc[bk], first + (uint32_t)__popc(mb & ((1u << u) - 1u)));
+                        ++cb;
+                        // a full chunk inside the bank's capacity is written; past the capacity (or
+                        // past the union's) the entries are only counted: the wavefront is built
+                        // again in the spill pool with the width the counts ask for
+                        if ((cb & 7u) == 0u && cb <= Kb)
+                            out[64ull * (bk * perBank + (cb >> 3) - 1u)] =
+                                make_uint4(acc[bk].a, acc[bk].b, acc[bk].c, acc[bk].d);
+                    }
                 }
                 ulen += (uint32_t)__popc(any);
             }
+            L.selfAt[lane] = -1;
             nl = 0;
             __builtin_amdgcn_wave_barrier();
         };
@@ -268,6 +321,16 @@ __device__ __forceinline__ WaveBuild buildWave(const DeviceCtx& c, WaveLds& L, u
             nr = 0;
             if (nl > kFlushAt || sp == 0) testCandidates();
             if (sp == 0) break;
+        }
+        // the partial last chunk of every bank, its entries shifted down to half-words 0, 1, ...
+        // (chunk cb / 8 lies inside the bank's capacity when cb < Kb)
+#pragma unroll
+        for (uint32_t bk = 0; bk < kBanks; ++bk) {
+            const uint32_t cb = (uint32_t)(cnt4 >> (bk << 4)) & 0xffffu;
+            if ((cb & 7u) != 0u && cb < Kb) {
+                for (uint32_t t = cb & 7u; t < 8u; ++t) appendEntry(acc[bk], 0u);
+                out[64ull * (bk * perBank + (cb >> 3))] = make_uint4(acc[bk].a, acc[bk].b, acc[bk].c, acc[bk].d);
+            }
         }
     }
     return WaveBuild{cnt4, ulen, stackFull};
