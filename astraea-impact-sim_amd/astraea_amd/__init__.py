@@ -125,8 +125,10 @@ class PlanetInfo(C.Structure):
 
 
 class Tuning(C.Structure):
-    """astraea_tuning: union records staged per round by pass 1, pass 2 and the drift."""
-    _fields_ = [("pass1_round", C.c_uint32), ("pass2_round", C.c_uint32), ("drift_round", C.c_uint32)]
+    """astraea_tuning: union records staged per round by pass 1, pass 2 and the drift, and the
+    radix sort's digit width."""
+    _fields_ = [("pass1_round", C.c_uint32), ("pass2_round", C.c_uint32), ("drift_round", C.c_uint32),
+                ("sort_bits", C.c_uint32)]
 
 
 class BodyOut(C.Structure):
@@ -430,19 +432,19 @@ class Context:
         _check(_lib.astraea_get_tuning(self._h, C.byref(t)))
         return {k: int(getattr(t, k)) for k, _ in Tuning._fields_}
 
-    def set_tuning(self, pass1_round: int, pass2_round: int, drift_round: int):
-        t = Tuning(pass1_round, pass2_round, drift_round)
+    def set_tuning(self, pass1_round: int, pass2_round: int, drift_round: int, sort_bits: int = 8):
+        t = Tuning(pass1_round, pass2_round, drift_round, sort_bits)
         _check(_lib.astraea_set_tuning(self._h, C.byref(t)))
 
     def autotune(self, max_dt: float, steps_per_variant: int = 1) -> dict:
         """Runs 8 * steps_per_variant steps timing every launch variant (astraea_autotune) and
         keeps the fastest; returns the choice and the measured mean milliseconds per step."""
         t = Tuning()
-        ms = (C.c_double * 8)()
+        ms = (C.c_double * 10)()
         _check(_lib.astraea_autotune(self._h, float(max_dt), int(steps_per_variant), C.byref(t), ms))
         chosen = {k: int(getattr(t, k)) for k, _ in Tuning._fields_}
         return dict(chosen=chosen, pass1_ms={128: ms[0], 256: ms[1]}, pass2_ms={64: ms[2], 128: ms[3]},
-                    drift_ms={64: ms[4], 128: ms[5], 192: ms[6], 256: ms[7]})
+                    drift_ms={64: ms[4], 128: ms[5], 192: ms[6], 256: ms[7]}, sort_ms={8: ms[8], 9: ms[9]})
 
 
 def tuning_key(particles: int, segments: int, strength: bool, fp64: bool) -> str:
@@ -461,7 +463,7 @@ def tune_cached(ctx: "Context", max_dt: float, path, key: str, steps_per_variant
     table = json.loads(p.read_text()) if p.exists() else {}
     if key in table:
         t = table[key]
-        ctx.set_tuning(t["pass1_round"], t["pass2_round"], t["drift_round"])
+        ctx.set_tuning(t["pass1_round"], t["pass2_round"], t["drift_round"], t.get("sort_bits", 8))
         return dict(chosen=dict(t), cached=True)
     r = ctx.autotune(max_dt, steps_per_variant)
     table[key] = r["chosen"]

@@ -465,7 +465,7 @@ void enqueueTree(astraea_ctx* c, bool timed) {
     hipStream_t s = c->stream;
     if (timed) stageMark(c, ASTRAEA_STAGE_MORTON);
     launchBoundsReduce(c->dc, s);
-    launchMorton(c->dc, s);
+    launchMorton(sortInput(c->dc), s);
     if (timed) { stageClose(c, ASTRAEA_STAGE_MORTON); stageMark(c, ASTRAEA_STAGE_SORT); }
     sortPairs(c->dc, s);
     if (timed) { stageClose(c, ASTRAEA_STAGE_SORT); stageMark(c, ASTRAEA_STAGE_PERMUTE_TREE); }
@@ -736,15 +736,20 @@ int astraea_create(astraea_ctx** out, int device, uint32_t capacity, uint32_t ma
     A(devAlloc(c, d.ss.simTime, maxSegments));
     A(devAlloc(c, d.ss.living, maxSegments));
     A(devAlloc(c, d.cnt.v, 8));
-    const uint32_t tiles = (C + kSortTile - 1) / kSortTile;
+    // tiles are cut along segment boundaries: at most one short tile per segment beyond C / tile
+    const uint32_t tiles = C / kSortTile + maxSegments + 1u;
     A(devAlloc(c, d.sb.keyA, C));
     A(devAlloc(c, d.sb.keyB, C));
     A(devAlloc(c, d.sb.valA, C));
     A(devAlloc(c, d.sb.valB, C));
-    A(devAlloc(c, d.sb.counts, 256 * (size_t)tiles));
-    A(devAlloc(c, d.sb.tilePrefix, 256 * (size_t)tiles));
-    A(devAlloc(c, d.sb.digitTotal, 256));
+    A(devAlloc(c, d.sb.counts, kSortMaxDigits * (size_t)tiles));
+    A(devAlloc(c, d.sb.tilePrefix, kSortMaxDigits * (size_t)tiles));
+    A(devAlloc(c, d.sb.digitTotal, kSortMaxDigits * (size_t)maxSegments));
+    A(devAlloc(c, d.sb.tileStart, (size_t)tiles + 1u));
+    A(devAlloc(c, d.sb.tileSeg, tiles));
+    A(devAlloc(c, d.sb.segTile0, (size_t)maxSegments + 1u));
     d.sb.tilesCap = tiles;
+    d.sb.numTiles = 0;
     A(devAlloc(c, c->dEnergy, (size_t)ASTRAEA_ENERGY_STRIDE * maxSegments));
     A(devAlloc(c, c->dClass, (size_t)ASTRAEA_CLASS_STRIDE * maxSegments));
     A(devAlloc(c, c->dRem, (size_t)24 * maxSegments));
@@ -844,6 +849,25 @@ int astraea_set_layout(astraea_ctx* c, uint32_t ns, const uint32_t* counts) {
     HIP_TRY(hipMemcpy(d.seg.off, offs.data(), sizeof(uint32_t) * (ns + 1), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.seg.n, counts, sizeof(uint32_t) * ns, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.seg.slotSeg, slotSeg.data(), sizeof(uint32_t) * c->slots, hipMemcpyHostToDevice));
+    // the sort's tiles: each segment's slot range cut into tiles of kSortTile (the last may be short)
+    {
+        std::vector<uint32_t> tileStart, tileSeg, segTile0(ns + 1, 0);
+        for (uint32_t s = 0; s < ns; ++s) {
+            segTile0[s] = (uint32_t)tileSeg.size();
+            for (uint32_t p = offs[s]; p < offs[s + 1]; p += kSortTile) {
+                tileStart.push_back(p);
+                tileSeg.push_back(s);
+            }
+        }
+        segTile0[ns] = (uint32_t)tileSeg.size();
+        tileStart.push_back(offs[ns]);
+        if (tileSeg.size() > d.sb.tilesCap)
+            return fail(ASTRAEA_E_CAPACITY, "%zu sort tiles, capacity %u", tileSeg.size(), d.sb.tilesCap);
+        HIP_TRY(hipMemcpy(d.sb.tileStart, tileStart.data(), sizeof(uint32_t) * tileStart.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d.sb.tileSeg, tileSeg.data(), sizeof(uint32_t) * tileSeg.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d.sb.segTile0, segTile0.data(), sizeof(uint32_t) * segTile0.size(), hipMemcpyHostToDevice));
+        d.sb.numTiles = (uint32_t)tileSeg.size();
+    }
     // state as uploadHostStateToDevice leaves it (simulation.cpp:469-520): zeros, eps_fail = 1,
     // original id = index in the concatenation
     for (Persist* p : {&d.cur, &d.alt}) {
@@ -1472,7 +1496,8 @@ int astraea_download_tree(astraea_ctx* c, uint32_t seg, uint64_t* keys, uint32_t
 
 static bool validTuning(const astraea_tuning& t) {
     return (t.pass1_round == 128u || t.pass1_round == 256u) && (t.pass2_round == 64u || t.pass2_round == 128u) &&
-           (t.drift_round == 64u || t.drift_round == 128u || t.drift_round == 192u || t.drift_round == 256u);
+           (t.drift_round == 64u || t.drift_round == 128u || t.drift_round == 192u || t.drift_round == 256u) &&
+           (t.sort_bits == 8u || t.sort_bits == 9u);
 }
 
 int astraea_get_tuning(astraea_ctx* c, astraea_tuning* out) {
@@ -1480,17 +1505,21 @@ int astraea_get_tuning(astraea_ctx* c, astraea_tuning* out) {
     out->pass1_round = c->dc.tune.pass1Round;
     out->pass2_round = c->dc.tune.pass2Round;
     out->drift_round = c->dc.tune.driftRound;
+    out->sort_bits = c->dc.tune.sortBits;
     return ASTRAEA_OK;
 }
 
 int astraea_set_tuning(astraea_ctx* c, const astraea_tuning* t) {
     if (!c || !t) return fail(ASTRAEA_E_INVALID, "null argument");
     if (!validTuning(*t))
-        return fail(ASTRAEA_E_INVALID, "tuning (%u, %u, %u) outside pass 1 {128, 256}, pass 2 {64, 128}, drift {64, 128, 192, 256}",
-                    t->pass1_round, t->pass2_round, t->drift_round);
+        return fail(ASTRAEA_E_INVALID,
+                    "tuning (%u, %u, %u, %u) outside pass 1 {128, 256}, pass 2 {64, 128}, drift {64, 128, 192, 256}, "
+                    "sort bits {8, 9}",
+                    t->pass1_round, t->pass2_round, t->drift_round, t->sort_bits);
     c->dc.tune.pass1Round = t->pass1_round;
     c->dc.tune.pass2Round = t->pass2_round;
     c->dc.tune.driftRound = t->drift_round;
+    c->dc.tune.sortBits = t->sort_bits;
     freeGraph(c);  // a captured step holds the old kernels
     return ASTRAEA_OK;
 }
@@ -1498,16 +1527,18 @@ int astraea_set_tuning(astraea_ctx* c, const astraea_tuning* t) {
 int astraea_autotune(astraea_ctx* c, float maxDt, uint32_t stepsPerVariant, astraea_tuning* chosen, double* ms) {
     if (!c) return fail(ASTRAEA_E_INVALID, "null context");
     if (stepsPerVariant == 0) return fail(ASTRAEA_E_INVALID, "steps per variant must be > 0");
-    static constexpr uint32_t p1[2] = {128u, 256u}, p2[2] = {64u, 128u}, dr[4] = {64u, 128u, 192u, 256u};
+    static constexpr uint32_t p1[2] = {128u, 256u}, p2[2] = {64u, 128u}, dr[4] = {64u, 128u, 192u, 256u},
+                              sb[2] = {8u, 9u};
     // eight runs in palindromic orders: every variant's runs sit on average at the middle of the
     // sequence, so a steady drift of the step's cost as the state evolves cancels out
     static constexpr int order2[8] = {0, 1, 1, 0, 0, 1, 1, 0}, order4[8] = {0, 1, 2, 3, 3, 2, 1, 0};
     const bool wasTiming = c->timing;
-    const astraea_tuning before = {c->dc.tune.pass1Round, c->dc.tune.pass2Round, c->dc.tune.driftRound};
-    double t1[2] = {0.0, 0.0}, t2[2] = {0.0, 0.0}, td[4] = {0.0, 0.0, 0.0, 0.0};
+    const astraea_tuning before = {c->dc.tune.pass1Round, c->dc.tune.pass2Round, c->dc.tune.driftRound,
+                                   c->dc.tune.sortBits};
+    double t1[2] = {0.0, 0.0}, t2[2] = {0.0, 0.0}, td[4] = {0.0, 0.0, 0.0, 0.0}, ts[2] = {0.0, 0.0};
     int rc = astraea_enable_stage_timing(c, 1);
     for (int r = 0; r < 8 && rc == ASTRAEA_OK; ++r) {
-        const astraea_tuning t = {p1[order2[r]], p2[order2[r]], dr[order4[r]]};
+        const astraea_tuning t = {p1[order2[r]], p2[order2[r]], dr[order4[r]], sb[order2[r]]};
         rc = astraea_set_tuning(c, &t);
         if (rc == ASTRAEA_OK) rc = astraea_step(c, maxDt, stepsPerVariant, 0);
         if (rc == ASTRAEA_OK) rc = settle(c);
@@ -1519,6 +1550,7 @@ int astraea_autotune(astraea_ctx* c, float maxDt, uint32_t stepsPerVariant, astr
         t1[order2[r]] += 0.25 * mean(ASTRAEA_STAGE_PASS1);
         t2[order2[r]] += 0.25 * mean(ASTRAEA_STAGE_PASS2);
         td[order4[r]] += 0.5 * mean(ASTRAEA_STAGE_DRIFT);
+        ts[order2[r]] += 0.25 * mean(ASTRAEA_STAGE_SORT);
     }
     const int rt = astraea_enable_stage_timing(c, wasTiming ? 1 : 0);
     if (rc != ASTRAEA_OK) {
@@ -1529,12 +1561,13 @@ int astraea_autotune(astraea_ctx* c, float maxDt, uint32_t stepsPerVariant, astr
     int bd = 0;
     for (int k = 1; k < 4; ++k)
         if (td[k] < td[bd]) bd = k;
-    const astraea_tuning best = {t1[0] < t1[1] ? p1[0] : p1[1], t2[0] < t2[1] ? p2[0] : p2[1], dr[bd]};
+    const astraea_tuning best = {t1[0] < t1[1] ? p1[0] : p1[1], t2[0] < t2[1] ? p2[0] : p2[1], dr[bd],
+                                 ts[0] < ts[1] ? sb[0] : sb[1]};
     TRY(astraea_set_tuning(c, &best));
     if (chosen) *chosen = best;
     if (ms) {
-        const double all[8] = {t1[0], t1[1], t2[0], t2[1], td[0], td[1], td[2], td[3]};
-        for (int k = 0; k < 8; ++k) ms[k] = all[k];
+        const double all[10] = {t1[0], t1[1], t2[0], t2[1], td[0], td[1], td[2], td[3], ts[0], ts[1]};
+        for (int k = 0; k < 10; ++k) ms[k] = all[k];
     }
     return ASTRAEA_OK;
 }

@@ -10,16 +10,23 @@ namespace astraea_hip {
 
 // Elements per tile of the radix sort (one 256-thread block, eight per thread).
 constexpr uint32_t kSortTile = 2048;
+constexpr uint32_t kSortMaxDigits = 512;  // 9-bit digits
+// Passes of `bits`-bit digits over the 63 key bits.
+constexpr int sortPassesOf(uint32_t bits) { return (int)((63u + bits - 1u) / bits); }
 
 struct SortBuffers {
     uint64_t* keyA;
     uint64_t* keyB;
     uint32_t* valA;
     uint32_t* valB;
-    uint32_t* counts;      // [256 * tiles] digit-major tile histograms
-    uint32_t* tilePrefix;  // [256 * tiles] exclusive prefix over tiles per digit
-    uint32_t* digitTotal;  // [256]
-    uint32_t tilesCap;     // tiles of kSortTile the counts hold
+    uint32_t* counts;      // [kSortMaxDigits * tilesCap] digit-major tile histograms
+    uint32_t* tilePrefix;  // [kSortMaxDigits * tilesCap] exclusive prefix over a segment's tiles per digit
+    uint32_t* digitTotal;  // [segments * digits] per segment and digit
+    uint32_t* tileStart;   // [numTiles + 1] first slot of each tile (tiles never straddle segments)
+    uint32_t* tileSeg;     // [numTiles] segment of each tile
+    uint32_t* segTile0;    // [segments + 1] first tile of each segment
+    uint32_t numTiles;
+    uint32_t tilesCap;
 };
 
 // Everything a kernel may need, passed by value.
@@ -29,6 +36,7 @@ struct Tuning {
     uint32_t pass1Round = 256;  // 128 or 256
     uint32_t pass2Round = 128;  // 64 or 128
     uint32_t driftRound = 128;  // 64, 128, 192 or 256
+    uint32_t sortBits = 8;      // radix sort digit: 8 (8 passes) or 9 (7 passes)
 };
 
 struct DeviceCtx {
@@ -64,6 +72,9 @@ __device__ __forceinline__ bool stepHeld(const DeviceCtx& c) {
 // sort.hip: stable LSD radix sort of (key, value) over `bits` key bits, then (segments > 1)
 // one stable pass on the segment of each value.  Result in keyA/valA.
 void sortPairs(const DeviceCtx& c, hipStream_t s);
+// The context the Morton kernel writes its keys through: the buffers an odd number of sort
+// passes starts from, so that the sorted keys and slots end in keyA / valA.
+DeviceCtx sortInput(const DeviceCtx& c);
 
 // tree.hip
 void launchPrepareSort(const DeviceCtx& c, hipStream_t s);    // posNew = pos, bounds

@@ -1,17 +1,22 @@
 // sort.hip — stable LSD radix sort of (uint64 Morton key, uint32 slot) pairs for gfx950.
 //
 // Replaces cub::DeviceRadixSort::SortPairs over bits [0, 63) (engine/sph_gravity.cu:2028-2046).
-// The result is exactly std::stable_sort by key: every pass is stable.
+// The result is exactly std::stable_sort by key within every segment: every pass is stable.
 //
-// One pass (8-bit digit) = three kernels:
-//   hist    per 2048-element tile, digit counts (digit-major: counts[d * tiles + t]);
-//   scan    one block per digit: exclusive prefix over tiles + digit totals;
+// Each segment (scenario) is sorted inside its own slot range, which its particles never leave:
+// the tiles are cut along segment boundaries (a segment's last tile may be short), a tile's
+// elements go to its own segment's range, and no pass has to group the segments afterwards.
+// One pass (a BITS-bit digit, 8 or 9: 8 or 7 passes over the 63 key bits) = three kernels:
+//   hist    per tile of up to 2048 elements, digit counts (digit-major: counts[d * tilesCap + t]);
+//   scan    one wavefront per (digit, segment): exclusive prefix over the segment's tiles + the
+//           segment's digit total;
 //   scatter per tile: stable local ranks from wavefront ballots (each lane's digit is
-//           matched against the wave with 8 ballots), per-wave running counts in LDS,
+//           matched against the wave with BITS ballots), per-wave running counts in LDS,
 //           an LDS-staged reorder, then coalesced runs to HBM.
 // Tiles are wave-contiguous: wave w of a tile owns kIPT consecutive 64-element rows, so the
-// element order inside a tile is (wave, row, lane) and ranks stay stable.
-// With several segments a final stable pass on the segment id groups the scenarios.
+// element order inside a tile is (wave, row, lane) and ranks stay stable.  The passes alternate
+// between the two buffers and end in keyA/valA: with an odd number of passes the Morton kernel
+// writes its keys to keyB/valB (sortInput).
 #include "launch.hpp"
 
 namespace astraea_hip {
@@ -21,15 +26,12 @@ constexpr int kTile = (int)kSortTile;
 constexpr int kIPT = kTile / kBlock;
 static_assert(kTile % kBlock == 0, "a tile is whole rows of the block");
 
-__device__ __forceinline__ uint32_t digitOf(uint64_t key, uint32_t val, int shift, const uint32_t* slotSeg, bool seg) {
-    return seg ? slotSeg[val] : (uint32_t)(key >> shift) & 0xffu;
-}
-
-// Lanes holding the same 8-bit digit: the match mask, from 8 ballots.
+// Lanes holding the same BITS-bit digit: the match mask, from BITS ballots.
+template <int BITS>
 __device__ __forceinline__ uint64_t matchDigit(uint32_t d, uint64_t valid) {
     uint64_t m = valid;
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < BITS; ++b) {
         const bool bit = (d >> b) & 1u;
         const uint64_t bb = ballot(bit);
         m &= bit ? bb : ~bb;
@@ -60,87 +62,129 @@ __device__ __forceinline__ uint32_t blockExclusiveScan(uint32_t v, uint32_t* scr
     return pre + inc - v;
 }
 
-__global__ __launch_bounds__(kBlock) void kHist(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                                                uint32_t n, int shift, const uint32_t* __restrict__ slotSeg, int segPass,
-                                                uint32_t* __restrict__ counts, uint32_t tiles) {
-    __shared__ uint32_t h[kWavesPerBlock][256];
+// Exclusive scan over the DIG digits of `in` (LDS or global), thread t owning digits
+// t * DIG / 256 .. (t + 1) * DIG / 256 - 1; writes out[d] (LDS).
+template <int DIG>
+__device__ __forceinline__ void digitExclusiveScan(const uint32_t* in, uint32_t* out, uint32_t* scratch) {
+    constexpr int PER = DIG / kBlock;
+    static_assert(DIG % kBlock == 0, "digits are whole multiples of the block");
+    const int tid = threadIdx.x;
+    uint32_t v[PER], sum = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        v[k] = in[tid * PER + k];
+        sum += v[k];
+    }
+    uint32_t run = blockExclusiveScan(sum, scratch, nullptr);
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        out[tid * PER + k] = run;
+        run += v[k];
+    }
+}
+
+template <int BITS>
+__global__ __launch_bounds__(kBlock) void kHist(const uint64_t* __restrict__ keys, uint32_t* __restrict__ counts,
+                                                const uint32_t* __restrict__ tileStart, uint32_t tilesCap, int shift) {
+    constexpr int DIG = 1 << BITS;
+    __shared__ uint32_t h[kWavesPerBlock][DIG];
     const int tid = threadIdx.x, wave = tid >> 6, lane = laneId();
-    for (int k = tid; k < kWavesPerBlock * 256; k += kBlock) (&h[0][0])[k] = 0u;
+    for (int k = tid; k < kWavesPerBlock * DIG; k += kBlock) (&h[0][0])[k] = 0u;
     __syncthreads();
     const uint32_t tile = blockIdx.x;
-    const uint32_t base = tile * kTile + wave * kWave * kIPT;
+    const uint32_t end = tileStart[tile + 1];
+    const uint32_t base = tileStart[tile] + wave * kWave * kIPT;
     for (int k = 0; k < kIPT; ++k) {
         const uint32_t idx = base + k * kWave + lane;
-        const bool v = idx < n;
+        const bool v = idx < end;
         uint32_t d = 0;
-        if (v) d = digitOf(keys[idx], vals[idx], shift, slotSeg, segPass != 0);
-        const uint64_t m = matchDigit(d, ballot(v));
+        if (v) d = (uint32_t)(keys[idx] >> shift) & (uint32_t)(DIG - 1);
+        const uint64_t m = matchDigit<BITS>(d, ballot(v));
         const uint64_t lt = (1ull << lane) - 1ull;
         if (v && (m & lt) == 0ull) h[wave][d] += (uint32_t)__popcll(m);
     }
     __syncthreads();
-    const uint32_t c = h[0][tid] + h[1][tid] + h[2][tid] + h[3][tid];
-    counts[(size_t)tid * tiles + tile] = c;
-}
-
-// One block per digit: exclusive prefix of counts over tiles.
-__global__ __launch_bounds__(kBlock) void kScan(const uint32_t* __restrict__ counts, uint32_t* __restrict__ prefix,
-                                                uint32_t* __restrict__ totals, uint32_t tiles) {
-    __shared__ uint32_t scratch[kWavesPerBlock];
-    const uint32_t d = blockIdx.x;
-    const uint32_t* src = counts + (size_t)d * tiles;
-    uint32_t* dst = prefix + (size_t)d * tiles;
-    uint32_t carry = 0;
-    for (uint32_t b = 0; b < tiles; b += kBlock) {
-        const uint32_t t = b + threadIdx.x;
-        const uint32_t v = t < tiles ? src[t] : 0u;
-        uint32_t tot;
-        const uint32_t ex = blockExclusiveScan(v, scratch, &tot);
-        if (t < tiles) dst[t] = carry + ex;
-        carry += tot;
+    for (int d = tid; d < DIG; d += kBlock) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int w = 0; w < kWavesPerBlock; ++w) c += h[w][d];
+        counts[(size_t)d * tilesCap + tile] = c;
     }
-    if (threadIdx.x == 0) totals[d] = carry;
 }
 
+// One wavefront per (digit, segment): the exclusive prefix of the digit's counts over the
+// segment's tiles, 64 at a time, and the segment's total of the digit.
+__global__ __launch_bounds__(kBlock) void kScan(const uint32_t* __restrict__ counts, uint32_t* __restrict__ prefix,
+                                                uint32_t* __restrict__ totals, const uint32_t* __restrict__ segTile0,
+                                                uint32_t segments, uint32_t tilesCap, uint32_t digits) {
+    const int lane = laneId();
+    const uint32_t pair = uniformU32(blockIdx.x * (uint32_t)kWavesPerBlock + (threadIdx.x >> 6));
+    if (pair >= digits * segments) return;
+    const uint32_t d = pair / segments, s = pair % segments;
+    const uint32_t* src = counts + (size_t)d * tilesCap;
+    uint32_t* dst = prefix + (size_t)d * tilesCap;
+    const uint32_t t0 = uniformU32(segTile0[s]), t1 = uniformU32(segTile0[s + 1]);
+    uint32_t carry = 0;
+    for (uint32_t b = t0; b < t1; b += kWave) {
+        const uint32_t t = b + (uint32_t)lane;
+        const uint32_t v = t < t1 ? src[t] : 0u;
+        uint32_t inc = v;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t u = (uint32_t)__shfl_up((int)inc, o, 64);
+            if (lane >= o) inc += u;
+        }
+        if (t < t1) dst[t] = carry + inc - v;
+        carry += (uint32_t)__builtin_amdgcn_readlane((int)inc, kWave - 1);
+    }
+    if (lane == 0) totals[(size_t)s * digits + d] = carry;
+}
+
+template <int BITS>
 __global__ __launch_bounds__(kBlock) void kScatter(const uint64_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn,
                                                    uint64_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut,
-                                                   uint32_t n, int shift, const uint32_t* __restrict__ slotSeg, int segPass,
-                                                   const uint32_t* __restrict__ prefix,
-                                                   const uint32_t* __restrict__ totals, uint32_t tiles) {
+                                                   const uint32_t* __restrict__ tileStart,
+                                                   const uint32_t* __restrict__ tileSeg,
+                                                   const uint32_t* __restrict__ segOff, const uint32_t* __restrict__ prefix,
+                                                   const uint32_t* __restrict__ totals, uint32_t tilesCap, int shift) {
+    constexpr int DIG = 1 << BITS;
     __shared__ uint64_t sKey[kTile];
     __shared__ uint32_t sVal[kTile];
-    __shared__ uint32_t sWave[kWavesPerBlock][256];
-    __shared__ uint32_t sLocal[256];
-    __shared__ uint32_t sGlobal[256];
+    __shared__ uint32_t sWave[kWavesPerBlock][DIG];
+    __shared__ uint32_t sLocal[DIG];
+    __shared__ uint32_t sGlobal[DIG];
     __shared__ uint32_t scratch[kWavesPerBlock];
     const int tid = threadIdx.x, wave = tid >> 6, lane = laneId();
-    const bool seg = segPass != 0;
     const uint32_t tile = blockIdx.x;
-    for (int k = tid; k < kWavesPerBlock * 256; k += kBlock) (&sWave[0][0])[k] = 0u;
-    // global start of every digit for this tile
+    const uint32_t seg = tileSeg[tile];
+    const uint32_t t0 = tileStart[tile], end = tileStart[tile + 1];
+    for (int k = tid; k < kWavesPerBlock * DIG; k += kBlock) (&sWave[0][0])[k] = 0u;
+    // start of every digit for this tile: the segment's offset, the digits below in the segment,
+    // the same digit in the segment's earlier tiles
+    digitExclusiveScan<DIG>(totals + (size_t)seg * DIG, sGlobal, scratch);
+    __syncthreads();
     {
-        const uint32_t tot = totals[tid];
-        const uint32_t start = blockExclusiveScan(tot, scratch, nullptr);
-        sGlobal[tid] = start + prefix[(size_t)tid * tiles + tile];
+        const uint32_t so = segOff[seg];
+        for (int d = tid; d < DIG; d += kBlock) sGlobal[d] += so + prefix[(size_t)d * tilesCap + tile];
     }
     __syncthreads();
 
     uint64_t key[kIPT];
     uint32_t val[kIPT], rank[kIPT], dig[kIPT];
-    const uint32_t base = tile * kTile + wave * kWave * kIPT;
+    const uint32_t base = t0 + wave * kWave * kIPT;
     const uint64_t lt = (1ull << lane) - 1ull;
 #pragma unroll
     for (int k = 0; k < kIPT; ++k) {
         const uint32_t idx = base + k * kWave + lane;
-        const bool v = idx < n;
+        const bool v = idx < end;
         key[k] = v ? keysIn[idx] : 0ull;
         val[k] = v ? valsIn[idx] : 0u;
-        dig[k] = v ? digitOf(key[k], val[k], shift, slotSeg, seg) : 256u;
+        dig[k] = v ? (uint32_t)(key[k] >> shift) & (uint32_t)(DIG - 1) : (uint32_t)DIG;
     }
 #pragma unroll
     for (int k = 0; k < kIPT; ++k) {
-        const bool v = dig[k] < 256u;
-        const uint64_t m = matchDigit(dig[k] & 0xffu, ballot(v));
+        const bool v = dig[k] < (uint32_t)DIG;
+        const uint64_t m = matchDigit<BITS>(dig[k] & (uint32_t)(DIG - 1), ballot(v));
         uint32_t r = 0;
         if (v) {
             const uint32_t before = sWave[wave][dig[k]];
@@ -150,35 +194,38 @@ __global__ __launch_bounds__(kBlock) void kScatter(const uint64_t* __restrict__ 
         rank[k] = r;
     }
     __syncthreads();
-    {
-        // per digit: exclusive prefix over waves, tile total, local start
+    // per digit: exclusive prefix over waves (in place) and the tile's count (sLocal, then its
+    // exclusive prefix over digits: the digit's start inside the tile)
+    for (int d = tid; d < DIG; d += kBlock) {
         uint32_t acc = 0;
 #pragma unroll
         for (int w = 0; w < kWavesPerBlock; ++w) {
-            const uint32_t c = sWave[w][tid];
-            sWave[w][tid] = acc;
+            const uint32_t c = sWave[w][d];
+            sWave[w][d] = acc;
             acc += c;
         }
-        sLocal[tid] = blockExclusiveScan(acc, scratch, nullptr);
+        sLocal[d] = acc;
     }
+    __syncthreads();
+    digitExclusiveScan<DIG>(sLocal, sLocal, scratch);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kIPT; ++k) {
-        if (dig[k] < 256u) {
+        if (dig[k] < (uint32_t)DIG) {
             const uint32_t p = sLocal[dig[k]] + sWave[wave][dig[k]] + rank[k];
             sKey[p] = key[k];
             sVal[p] = val[k];
         }
     }
     __syncthreads();
-    const uint32_t inTile = min((uint32_t)kTile, n - tile * kTile);
+    const uint32_t inTile = end - t0;
 #pragma unroll
     for (int e = 0; e < kIPT; ++e) {
         const uint32_t p = e * kBlock + tid;
         if (p < inTile) {
             const uint64_t kk = sKey[p];
             const uint32_t vv = sVal[p];
-            const uint32_t d = digitOf(kk, vv, shift, slotSeg, seg);
+            const uint32_t d = (uint32_t)(kk >> shift) & (uint32_t)(DIG - 1);
             const uint32_t g = sGlobal[d] + (p - sLocal[d]);
             keysOut[g] = kk;
             valsOut[g] = vv;
@@ -186,31 +233,41 @@ __global__ __launch_bounds__(kBlock) void kScatter(const uint64_t* __restrict__ 
     }
 }
 
-}  // namespace
-
-void sortPairs(const DeviceCtx& c, hipStream_t s) {
-    const uint32_t n = c.slots;
-    const uint32_t tiles = (n + kTile - 1) / kTile;
-    uint64_t* kin = c.sb.keyA;
-    uint64_t* kout = c.sb.keyB;
-    uint32_t* vin = c.sb.valA;
-    uint32_t* vout = c.sb.valB;
-    const int passes = 8 + (c.seg.count > 1 ? 1 : 0);
+template <int BITS>
+void sortPasses(const DeviceCtx& c, hipStream_t s) {
+    constexpr int DIG = 1 << BITS;
+    constexpr int passes = (63 + BITS - 1) / BITS;
+    const uint32_t tiles = c.sb.numTiles;
+    uint64_t* kin = passes % 2 ? c.sb.keyB : c.sb.keyA;
+    uint64_t* kout = passes % 2 ? c.sb.keyA : c.sb.keyB;
+    uint32_t* vin = passes % 2 ? c.sb.valB : c.sb.valA;
+    uint32_t* vout = passes % 2 ? c.sb.valA : c.sb.valB;
     for (int p = 0; p < passes; ++p) {
-        const int segPass = p == 8 ? 1 : 0;
-        const int shift = segPass ? 0 : 8 * p;
-        kHist<<<tiles, kBlock, 0, s>>>(kin, vin, n, shift, c.seg.slotSeg, segPass, c.sb.counts, tiles);
-        kScan<<<256, kBlock, 0, s>>>(c.sb.counts, c.sb.tilePrefix, c.sb.digitTotal, tiles);
-        kScatter<<<tiles, kBlock, 0, s>>>(kin, vin, kout, vout, n, shift, c.seg.slotSeg, segPass, c.sb.tilePrefix,
-                                          c.sb.digitTotal, tiles);
+        const int shift = BITS * p;
+        kHist<BITS><<<tiles, kBlock, 0, s>>>(kin, c.sb.counts, c.sb.tileStart, c.sb.tilesCap, shift);
+        kScan<<<(DIG * c.seg.count + kWavesPerBlock - 1) / kWavesPerBlock, kBlock, 0, s>>>(
+            c.sb.counts, c.sb.tilePrefix, c.sb.digitTotal, c.sb.segTile0, c.seg.count, c.sb.tilesCap, (uint32_t)DIG);
+        kScatter<BITS><<<tiles, kBlock, 0, s>>>(kin, vin, kout, vout, c.sb.tileStart, c.sb.tileSeg, c.seg.off,
+                                                c.sb.tilePrefix, c.sb.digitTotal, c.sb.tilesCap, shift);
         uint64_t* tk = kin; kin = kout; kout = tk;
         uint32_t* tv = vin; vin = vout; vout = tv;
     }
-    // an even number of passes leaves the result in keyA/valA; after an odd number copy it back
-    if (passes & 1) {
-        (void)hipMemcpyAsync(c.sb.keyA, kin, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, s);
-        (void)hipMemcpyAsync(c.sb.valA, vin, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s);
+}
+
+}  // namespace
+
+DeviceCtx sortInput(const DeviceCtx& c) {
+    DeviceCtx m = c;
+    if (sortPassesOf(c.tune.sortBits) % 2) {
+        m.sb.keyA = c.sb.keyB;
+        m.sb.valA = c.sb.valB;
     }
+    return m;
+}
+
+void sortPairs(const DeviceCtx& c, hipStream_t s) {
+    if (c.tune.sortBits == 9u) sortPasses<9>(c, s);
+    else sortPasses<8>(c, s);
 }
 
 }  // namespace astraea_hip

@@ -356,23 +356,26 @@ int astraea_download_tree(astraea_ctx* ctx, uint32_t seg,
 int astraea_enable_stage_timing(astraea_ctx* ctx, int enable);
 int astraea_stage_times(astraea_ctx* ctx, double* out_ms, uint64_t* out_samples);
 
-/* Launch variants (no counterpart in the reference, whose block sizes and stack depth are
- * compile-time constants, sph_gravity.cu:19-22): the union records each neighbour pass stages
- * in LDS per round.  Every variant gives bit-identical results (a particle's pairs run in the
- * same order); they differ in LDS use, and so in how many wavefronts share a SIMD. */
+/* Launch variants (no counterpart in the reference, whose block sizes, stack depth and sort are
+ * compile-time constants or a library's, sph_gravity.cu:19-22, 2028-2046): the union records each
+ * neighbour pass stages in LDS per round, and the radix sort's digit width.  Every variant gives
+ * bit-identical results (a particle's pairs run in the same order; every sort is the stable sort
+ * by key); they differ in LDS use, and so in how many wavefronts share a SIMD, and in passes. */
 typedef struct astraea_tuning {
     uint32_t pass1_round; /* 128 or 256 (default 256) */
     uint32_t pass2_round; /* 64 or 128 (default 128) */
     uint32_t drift_round; /* 64, 128, 192 or 256 (default 128) */
+    uint32_t sort_bits;   /* 8 (8 passes) or 9 (7 passes) (default 8) */
 } astraea_tuning;
 int astraea_get_tuning(astraea_ctx* ctx, astraea_tuning* out);
 int astraea_set_tuning(astraea_ctx* ctx, const astraea_tuning* tuning);
 /* The autotuner: advances the loaded state by 8 * steps_per_variant eager steps, timing every
- * variant of each pass with the stage events (in palindromic orders, so that the state's slow
- * change of cost cancels), and keeps the fastest for each pass.  The simulation does not depend
+ * variant of each stage with the stage events (in palindromic orders, so that the state's slow
+ * change of cost cancels), and keeps the fastest for each.  The simulation does not depend
  * on the variant, so these are ordinary steps.  `chosen` and `ms`
  * (mean milliseconds per step: pass 1 at 128 and 256, pass 2 at 64 and 128, drift at 64, 128,
- * 192 and 256; 8 values) may be NULL.  On an error the previous tuning is restored. */
+ * 192 and 256, the sort at 8 and 9 bits; 10 values) may be NULL.  On an error the previous
+ * tuning is restored. */
 int astraea_autotune(astraea_ctx* ctx, float max_dt_psu, uint32_t steps_per_variant, astraea_tuning* chosen,
                      double* ms);
 

@@ -142,6 +142,37 @@ def test_tree_bit_exact_segments(lib, orc):
         assert_tree_equal(ctx.tree(s), o.tree(s))
 
 
+@pytest.mark.parametrize("sort_bits", [8, 9])
+def test_tree_bit_exact_sort_radix(lib, orc, sort_bits):
+    """The sort at either digit width (8 passes of 8 bits, 7 of 9) over segments whose slot ranges
+    cut the 2048-element tiles at every kind of boundary (one slot short of a tile, exactly one,
+    one past, two plus one, and small ones), with duplicate positions: each segment's keys,
+    permutation and tree equal the oracle's stable sort."""
+    from astraea_amd import scenario as S
+
+    counts = [1, 2047, 2048, 2049, 4097, 300]
+    rng = np.random.default_rng(sort_bits)
+    n = sum(counts)
+    st = {"pos_x": rng.normal(0, 1, n).astype(np.float32), "pos_y": rng.normal(0, 1, n).astype(np.float32),
+          "pos_z": rng.normal(0, 1, n).astype(np.float32), "mass": rng.uniform(0.5, 1.5, n).astype(np.float32),
+          "h": np.full(n, 0.1, np.float32), "h0": np.full(n, 0.1, np.float32)}
+    for k in ("pos_x", "pos_y", "pos_z"):
+        st[k][10:200] = st[k][10]  # duplicate keys inside one segment (ties broken by slot)
+    ctx = lib.Context(capacity=n, max_segments=len(counts))
+    ctx.set_params(S.plain_params())
+    ctx.set_layout(counts)
+    ctx.load_state(st)
+    ctx.set_tuning(256, 128, 128, sort_bits)
+    ctx.sort_and_build()
+    ctx.synchronize()
+    o = orc.Oracle(S.plain_params(), group_size=lib.GROUP_SIZE)
+    o.set_layout(counts)
+    o.load_state(st)
+    o.sort_and_build()
+    for s in range(len(counts)):
+        assert_tree_equal(ctx.tree(s), o.tree(s))
+
+
 def force_parity(lib, orc, st, counts, params, tol):
     ctx, o = run_pair(lib, orc, st, counts, params, [("initial_forces",)])
     return check_forces(ctx, o, tol)

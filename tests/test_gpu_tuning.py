@@ -38,7 +38,7 @@ def _state(ctx):
 def test_variants_are_bit_identical(lib, strength):
     st = perturbed_small_state(strength=strength)
     want = None
-    for t in [(256, 128, 128), (128, 64, 64), (256, 64, 192), (128, 128, 256)]:
+    for t in [(256, 128, 128, 8), (128, 64, 64, 9), (256, 64, 192, 9), (128, 128, 256, 8)]:
         ctx = _context(lib, st, strength)
         ctx.set_tuning(*t)
         assert tuple(ctx.tuning().values()) == t
@@ -56,9 +56,9 @@ def test_autotune_steps_are_ordinary_steps(lib):
     tuned = _context(lib, st, False)
     r = tuned.autotune(MAX_DT, steps_per_variant=1)
     assert r["chosen"]["pass1_round"] in (128, 256) and r["chosen"]["pass2_round"] in (64, 128)
-    assert r["chosen"]["drift_round"] in (64, 128, 192, 256)
+    assert r["chosen"]["drift_round"] in (64, 128, 192, 256) and r["chosen"]["sort_bits"] in (8, 9)
     assert tuned.tuning() == r["chosen"]
-    assert all(v > 0.0 for d in ("pass1_ms", "pass2_ms", "drift_ms") for v in r[d].values())
+    assert all(v > 0.0 for d in ("pass1_ms", "pass2_ms", "drift_ms", "sort_ms") for v in r[d].values())
     plain = _context(lib, st, False)
     plain.step(MAX_DT, 8, graph=False)
     a, b = _state(tuned), _state(plain)
@@ -70,4 +70,6 @@ def test_invalid_tuning_is_rejected(lib):
     ctx = _context(lib, perturbed_small_state(), False)
     with pytest.raises(lib.AstraeaError):
         ctx.set_tuning(512, 128, 128)
-    assert ctx.tuning() == {"pass1_round": 256, "pass2_round": 128, "drift_round": 128}
+    with pytest.raises(lib.AstraeaError):
+        ctx.set_tuning(256, 128, 128, 11)
+    assert ctx.tuning() == {"pass1_round": 256, "pass2_round": 128, "drift_round": 128, "sort_bits": 8}
