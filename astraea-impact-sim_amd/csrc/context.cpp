@@ -725,7 +725,7 @@ int astraea_create(astraea_ctx** out, int device, uint32_t capacity, uint32_t ma
     A(devAlloc(c, d.seg.off, maxSegments + 1));
     A(devAlloc(c, d.seg.n, maxSegments));
     A(devAlloc(c, d.seg.slotSeg, C));
-    A(devAlloc(c, d.ss.bounds, 6 * maxSegments));
+    A(devAlloc(c, d.ss.bounds, 6 * (size_t)maxSegments));
     A(devAlloc(c, d.d.blockBounds, 6 * (size_t)(C / kBlock)));
     A(devAlloc(c, d.ss.dtRawBits, maxSegments));
     A(devAlloc(c, d.ss.dt, maxSegments));

@@ -46,7 +46,8 @@ struct Persist {
 
 struct Derived {
     float4* posNew;   // drift output (x', y', z', m) in the pre-sort order
-    float* blockBounds; // [6 per block] min xyz / max xyz of the block's positions
+    double* blockBounds; // [6 per block] min xyz / max xyz of the block's positions (FP32 values,
+                         // or the FP64 positions' with FP64 accumulators)
     float4* grav;     // (gx, gy, gz, phi)
     float4* rate;     // (ax, ay, az, drho/dt): total acceleration after pass 2
     float4* rate2;    // (du/dt, dD^(1/3)/dt, radPowerW, 0)
@@ -262,7 +263,8 @@ struct Segments {
 
 // Per-segment device scalars.
 struct SegScalars {
-    float* bounds;        // [6 per segment] min xyz / max xyz of the positions being sorted
+    double* bounds;       // [6 per segment] min xyz / max xyz of the positions being sorted (the Morton
+                          // box: FP32 values, or FP64 with FP64 accumulators)
     uint32_t* dtRawBits;  // float bits of min dt_i (atomicMin target)
     float* dt;            // dt of the current step
     float* dtPrev;        // previous dt (0 before the first step)
@@ -428,6 +430,10 @@ __device__ __forceinline__ float waveMaxF(float v) {
 }
 __device__ __forceinline__ double waveSumD(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double waveMinD(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
     return v;
 }
 __device__ __forceinline__ double waveMaxD(double v) {

@@ -174,7 +174,7 @@ __global__ __launch_bounds__(kBlock) void kDrift(DeviceCtx c) {
         Vec4 p, v;
         float H, invH, wNorm;  // H_j = 2 max(h_j, 1e-6) and its kernel constants, once per record
     };
-    __shared__ float sb[kWavesPerBlock][6];
+    __shared__ double sb[kWavesPerBlock][6];
     __shared__ Rec stageBuf[kWavesPerBlock][S];
     if (stepHeld(c)) return;
     const uint32_t b = xcdBlock(blockIdx.x, gridDim.x);
@@ -218,34 +218,39 @@ __global__ __launch_bounds__(kBlock) void kDrift(DeviceCtx c) {
             ys += co * (vj.y - vi.y);
             zs += co * (vj.z - vi.z);
         });
+    double px = 0.0, py = 0.0, pz = 0.0;
     if (active) {
         const float dt = c.ss.dt[sg];
         if (FP64) {
             const size_t cap = (size_t)c.stride64;
             const double d64 = (double)dt;
-            const double x = c.cur.p64[i] + d64 * (c.cur.v64[i] + 0.25 * (double)xs);
-            const double y = c.cur.p64[cap + i] + d64 * (c.cur.v64[cap + i] + 0.25 * (double)ys);
-            const double z = c.cur.p64[2 * cap + i] + d64 * (c.cur.v64[2 * cap + i] + 0.25 * (double)zs);
-            c.cur.p64[i] = x; c.cur.p64[cap + i] = y; c.cur.p64[2 * cap + i] = z;
-            pn = make_float4((float)x, (float)y, (float)z, pi.w);
+            px = c.cur.p64[i] + d64 * (c.cur.v64[i] + 0.25 * (double)xs);
+            py = c.cur.p64[cap + i] + d64 * (c.cur.v64[cap + i] + 0.25 * (double)ys);
+            pz = c.cur.p64[2 * cap + i] + d64 * (c.cur.v64[2 * cap + i] + 0.25 * (double)zs);
+            c.cur.p64[i] = px; c.cur.p64[cap + i] = py; c.cur.p64[2 * cap + i] = pz;
+            pn = make_float4((float)px, (float)py, (float)pz, pi.w);
         } else {
             pn = make_float4(pi.x + dt * (vi.x + 0.25f * xs), pi.y + dt * (vi.y + 0.25f * ys),
                              pi.z + dt * (vi.z + 0.25f * zs), pi.w);
+            px = pn.x;
+            py = pn.y;
+            pz = pn.z;
         }
         c.d.posNew[i] = pn;
     }
-    // per-block scene bounds of the drifted positions (reduced per segment before the sort)
-    float v[6] = {active ? pn.x : 1.0e30f, active ? pn.y : 1.0e30f, active ? pn.z : 1.0e30f,
-                  active ? pn.x : -1.0e30f, active ? pn.y : -1.0e30f, active ? pn.z : -1.0e30f};
-    for (int a = 0; a < 3; ++a) v[a] = waveMinF(v[a]);
-    for (int a = 3; a < 6; ++a) v[a] = waveMaxF(v[a]);
+    // per-block scene bounds of the drifted positions (reduced per segment before the sort): of
+    // the FP32 positions, or of the FP64 ones with FP64 accumulators (the Morton box)
+    double v[6] = {active ? px : 1.0e30, active ? py : 1.0e30, active ? pz : 1.0e30,
+                   active ? px : -1.0e30, active ? py : -1.0e30, active ? pz : -1.0e30};
+    for (int a = 0; a < 3; ++a) v[a] = waveMinD(v[a]);
+    for (int a = 3; a < 6; ++a) v[a] = waveMaxD(v[a]);
     if (lane == 0)
         for (int a = 0; a < 6; ++a) sb[wave][a] = v[a];
     __syncthreads();
     if (threadIdx.x < 6) {
         const int a = threadIdx.x;
-        float r = sb[0][a];
-        for (int w = 1; w < kWavesPerBlock; ++w) r = a < 3 ? fminf(r, sb[w][a]) : fmaxf(r, sb[w][a]);
+        double r = sb[0][a];
+        for (int w = 1; w < kWavesPerBlock; ++w) r = a < 3 ? fmin(r, sb[w][a]) : fmax(r, sb[w][a]);
         c.d.blockBounds[6 * b + a] = r;
     }
 }

@@ -38,6 +38,20 @@ __device__ __forceinline__ float invSpan(float mn, float mx) {
     return span > 1.0e-6f ? __fdiv_rn(2097151.0f, span) : 0.0f;
 }
 
+// The same quantisation in FP64, from the FP64 positions and an FP64 box (the mixed form with
+// FP64 leapfrog accumulators): 21 bits per axis whatever the box's distance from the origin.
+__device__ __forceinline__ uint32_t quantise64(double x, double mn, double inv) {
+#pragma clang fp contract(off)
+    const double f = fmin(fmax((x - mn) * inv, 0.0), 2097151.0);
+    return (uint32_t)f;
+}
+
+__device__ __forceinline__ double invSpan64(double mn, double mx) {
+#pragma clang fp contract(off)
+    const double span = mx - mn;
+    return span > 1.0e-6 ? 2097151.0 / span : 0.0;
+}
+
 __device__ __forceinline__ int karrasDelta(const uint64_t* __restrict__ k, int n, int i, int j) {
     if (j < 0 || j >= n) return -1;
     const uint64_t a = k[i], b = k[j];
@@ -46,20 +60,20 @@ __device__ __forceinline__ int karrasDelta(const uint64_t* __restrict__ k, int n
 }
 
 // Block min/max of the positions into the block's partial-bounds record.
-__device__ __forceinline__ void blockBounds(float x, float y, float z, bool active, float* out6) {
-    __shared__ float sb[kWavesPerBlock][6];
+__device__ __forceinline__ void blockBounds(double x, double y, double z, bool active, double* out6) {
+    __shared__ double sb[kWavesPerBlock][6];
     const int wave = threadIdx.x >> 6, lane = laneId();
-    float v[6] = {active ? x : 1.0e30f, active ? y : 1.0e30f, active ? z : 1.0e30f,
-                  active ? x : -1.0e30f, active ? y : -1.0e30f, active ? z : -1.0e30f};
-    for (int a = 0; a < 3; ++a) v[a] = waveMinF(v[a]);
-    for (int a = 3; a < 6; ++a) v[a] = waveMaxF(v[a]);
+    double v[6] = {active ? x : 1.0e30, active ? y : 1.0e30, active ? z : 1.0e30,
+                   active ? x : -1.0e30, active ? y : -1.0e30, active ? z : -1.0e30};
+    for (int a = 0; a < 3; ++a) v[a] = waveMinD(v[a]);
+    for (int a = 3; a < 6; ++a) v[a] = waveMaxD(v[a]);
     if (lane == 0)
         for (int a = 0; a < 6; ++a) sb[wave][a] = v[a];
     __syncthreads();
     if (threadIdx.x < 6) {
         const int a = threadIdx.x;
-        float r = sb[0][a];
-        for (int w = 1; w < kWavesPerBlock; ++w) r = a < 3 ? fminf(r, sb[w][a]) : fmaxf(r, sb[w][a]);
+        double r = sb[0][a];
+        for (int w = 1; w < kWavesPerBlock; ++w) r = a < 3 ? fmin(r, sb[w][a]) : fmax(r, sb[w][a]);
         out6[a] = r;
     }
 }
@@ -71,7 +85,12 @@ __global__ __launch_bounds__(kBlock) void kPrepareSort(DeviceCtx c) {
     const bool active = i - c.seg.off[sg] < c.seg.n[sg];
     const float4 p = c.cur.pos[i];  // in the FP64 mode pos always holds the rounded FP64 position
     if (active) c.d.posNew[i] = p;
-    blockBounds(p.x, p.y, p.z, active, c.d.blockBounds + 6 * blockIdx.x);
+    if (c.fp64) {
+        const size_t cap = (size_t)c.stride64;
+        blockBounds(c.cur.p64[i], c.cur.p64[cap + i], c.cur.p64[2 * cap + i], active, c.d.blockBounds + 6 * blockIdx.x);
+    } else {
+        blockBounds(p.x, p.y, p.z, active, c.d.blockBounds + 6 * blockIdx.x);
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void kMorton(DeviceCtx c) {
@@ -80,12 +99,21 @@ __global__ __launch_bounds__(kBlock) void kMorton(DeviceCtx c) {
     const bool active = i - c.seg.off[sg] < c.seg.n[sg];
     uint64_t key = kPadKey;
     if (active) {
-        const float* bo = c.ss.bounds + 6 * sg;
-        const float mnx = bo[0], mny = bo[1], mnz = bo[2];
-        const float mxx = bo[3], mxy = bo[4], mxz = bo[5];
-        const float4 p = c.d.posNew[i];
-        key = (spread21(quantise(p.x, mnx, invSpan(mnx, mxx))) << 2) |
-              (spread21(quantise(p.y, mny, invSpan(mny, mxy))) << 1) | spread21(quantise(p.z, mnz, invSpan(mnz, mxz)));
+        const double* bo = c.ss.bounds + 6 * sg;
+        if (c.fp64) {
+            // the mixed form: FP64 box and FP64 positions (the leapfrog's masters)
+            const size_t cap = (size_t)c.stride64;
+            const double x = c.cur.p64[i], y = c.cur.p64[cap + i], z = c.cur.p64[2 * cap + i];
+            key = (spread21(quantise64(x, bo[0], invSpan64(bo[0], bo[3]))) << 2) |
+                  (spread21(quantise64(y, bo[1], invSpan64(bo[1], bo[4]))) << 1) |
+                  spread21(quantise64(z, bo[2], invSpan64(bo[2], bo[5])));
+        } else {
+            const float mnx = (float)bo[0], mny = (float)bo[1], mnz = (float)bo[2];  // FP32 values, exactly
+            const float mxx = (float)bo[3], mxy = (float)bo[4], mxz = (float)bo[5];
+            const float4 p = c.d.posNew[i];
+            key = (spread21(quantise(p.x, mnx, invSpan(mnx, mxx))) << 2) |
+                  (spread21(quantise(p.y, mny, invSpan(mny, mxy))) << 1) | spread21(quantise(p.z, mnz, invSpan(mnz, mxz)));
+        }
     }
     c.sb.keyA[i] = key;
     c.sb.valA[i] = i;
@@ -293,27 +321,27 @@ __global__ __launch_bounds__(kChunk) void kMoments(DeviceCtx c) {
 
 // Scene bounds of each segment from its blocks' partial records (min/max is exact in any order).
 __global__ __launch_bounds__(kBlock) void kBoundsReduce(DeviceCtx c) {
-    __shared__ float sb[kWavesPerBlock][6];
+    __shared__ double sb[kWavesPerBlock][6];
     const uint32_t sg = blockIdx.x;
     const uint32_t b0 = c.seg.off[sg] / kBlock, b1 = c.seg.off[sg + 1] / kBlock;
-    float v[6] = {1.0e30f, 1.0e30f, 1.0e30f, -1.0e30f, -1.0e30f, -1.0e30f};
+    double v[6] = {1.0e30, 1.0e30, 1.0e30, -1.0e30, -1.0e30, -1.0e30};
     for (uint32_t b = b0 + threadIdx.x; b < b1; b += kBlock) {
-        const float* p = c.d.blockBounds + 6 * b;
+        const double* p = c.d.blockBounds + 6 * b;
         for (int a = 0; a < 3; ++a) {
-            v[a] = fminf(v[a], p[a]);
-            v[3 + a] = fmaxf(v[3 + a], p[3 + a]);
+            v[a] = fmin(v[a], p[a]);
+            v[3 + a] = fmax(v[3 + a], p[3 + a]);
         }
     }
-    for (int a = 0; a < 3; ++a) v[a] = waveMinF(v[a]);
-    for (int a = 3; a < 6; ++a) v[a] = waveMaxF(v[a]);
+    for (int a = 0; a < 3; ++a) v[a] = waveMinD(v[a]);
+    for (int a = 3; a < 6; ++a) v[a] = waveMaxD(v[a]);
     const int wave = threadIdx.x >> 6;
     if (laneId() == 0)
         for (int a = 0; a < 6; ++a) sb[wave][a] = v[a];
     __syncthreads();
     if (threadIdx.x < 6) {
         const int a = threadIdx.x;
-        float r = sb[0][a];
-        for (int w = 1; w < kWavesPerBlock; ++w) r = a < 3 ? fminf(r, sb[w][a]) : fmaxf(r, sb[w][a]);
+        double r = sb[0][a];
+        for (int w = 1; w < kWavesPerBlock; ++w) r = a < 3 ? fmin(r, sb[w][a]) : fmax(r, sb[w][a]);
         c.ss.bounds[6 * sg + a] = r;
     }
 }

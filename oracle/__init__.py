@@ -91,6 +91,7 @@ _sig = {
     "or_invert_density_si": (C.c_double, [C.c_int, C.c_double, C.c_double, C.c_double]),
     "or_material_param": (C.c_double, [C.c_int, C.c_int]),
     "or_morton_key": (C.c_uint64, [C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float)]),
+    "or_morton_key64": (C.c_uint64, [C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]),
 }
 for _name, (_res, _args) in _sig.items():
     _fn = getattr(_lib, _name)
@@ -312,3 +313,9 @@ def material(mat) -> dict:
 def morton_key(x, y, z, bounds6) -> int:
     b = (C.c_float * 6)(*[float(v) for v in bounds6])
     return int(_lib.or_morton_key(x, y, z, b))
+
+
+def morton_key64(x, y, z, bounds6) -> int:
+    """The Morton key of the mixed form (FP64 positions and box)."""
+    b = (C.c_double * 6)(*[float(v) for v in bounds6])
+    return int(_lib.or_morton_key64(float(x), float(y), float(z), b))

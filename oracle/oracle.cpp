@@ -325,6 +325,20 @@ inline uint64_t mortonKey(float x, float y, float z, const float* b, const float
     return (spread21(quantise(x, b[0], inv[0])) << 2) | (spread21(quantise(y, b[1], inv[1])) << 1) |
            spread21(quantise(z, b[2], inv[2]));
 }
+// The mixed form's Morton box (FP64 accumulators): the same quantisation in FP64, from the FP64
+// positions and an FP64 box (the kernels' quantise64 / invSpan64).
+inline uint32_t quantise64(double x, double mn, double inv) {
+    const double f = std::fmin(std::fmax((x - mn) * inv, 0.0), 2097151.0);
+    return static_cast<uint32_t>(f);
+}
+inline double invSpan64Of(double mn, double mx) {
+    const double span = mx - mn;
+    return span > 1.0e-6 ? 2097151.0 / span : 0.0;
+}
+inline uint64_t mortonKey64(double x, double y, double z, const double* b, const double* inv) {
+    return (spread21(quantise64(x, b[0], inv[0])) << 2) | (spread21(quantise64(y, b[1], inv[1])) << 1) |
+           spread21(quantise64(z, b[2], inv[2]));
+}
 
 // Squared distance with the kernels' operation order: ((dx*dx) + dy*dy) + dz*dz as two FMAs.
 inline float dist2f(float dx, float dy, float dz) {
@@ -459,17 +473,30 @@ void permuteVec(std::vector<T>& v, const std::vector<uint32_t>& perm) {
 void sortAndBuild(or_ctx* c, Seg& s) {
     const uint32_t n = s.n;
     if (n == 0) return;
-    float b[6] = {1.0e30f, 1.0e30f, 1.0e30f, -1.0e30f, -1.0e30f, -1.0e30f};
-    for (uint32_t i = 0; i < n; ++i) {
-        const float p[3] = {(float)s.x[i], (float)s.y[i], (float)s.z[i]};
-        for (int a = 0; a < 3; ++a) {
-            b[a] = std::fmin(b[a], p[a]);
-            b[3 + a] = std::fmax(b[3 + a], p[a]);
-        }
-    }
-    const float inv[3] = {invSpanOf(b[0], b[3]), invSpanOf(b[1], b[4]), invSpanOf(b[2], b[5])};
     std::vector<uint64_t> key(n);
-    for (uint32_t i = 0; i < n; ++i) key[i] = mortonKey((float)s.x[i], (float)s.y[i], (float)s.z[i], b, inv);
+    if (c->prm.precision == ASTRAEA_PREC_FP64_ACCUM) {
+        double b[6] = {1.0e30, 1.0e30, 1.0e30, -1.0e30, -1.0e30, -1.0e30};
+        for (uint32_t i = 0; i < n; ++i) {
+            const double p[3] = {s.x[i], s.y[i], s.z[i]};
+            for (int a = 0; a < 3; ++a) {
+                b[a] = std::fmin(b[a], p[a]);
+                b[3 + a] = std::fmax(b[3 + a], p[a]);
+            }
+        }
+        const double inv[3] = {invSpan64Of(b[0], b[3]), invSpan64Of(b[1], b[4]), invSpan64Of(b[2], b[5])};
+        for (uint32_t i = 0; i < n; ++i) key[i] = mortonKey64(s.x[i], s.y[i], s.z[i], b, inv);
+    } else {
+        float b[6] = {1.0e30f, 1.0e30f, 1.0e30f, -1.0e30f, -1.0e30f, -1.0e30f};
+        for (uint32_t i = 0; i < n; ++i) {
+            const float p[3] = {(float)s.x[i], (float)s.y[i], (float)s.z[i]};
+            for (int a = 0; a < 3; ++a) {
+                b[a] = std::fmin(b[a], p[a]);
+                b[3 + a] = std::fmax(b[3 + a], p[a]);
+            }
+        }
+        const float inv[3] = {invSpanOf(b[0], b[3]), invSpanOf(b[1], b[4]), invSpanOf(b[2], b[5])};
+        for (uint32_t i = 0; i < n; ++i) key[i] = mortonKey((float)s.x[i], (float)s.y[i], (float)s.z[i], b, inv);
+    }
     s.perm.resize(n);
     std::iota(s.perm.begin(), s.perm.end(), 0u);
     // Stable LSD radix sort over bits [0,63) == stable sort on the key (bit 63 is always 0).
@@ -1590,6 +1617,11 @@ void or_classify_arrays(uint32_t n, const double* x, const double* y, const doub
 uint64_t or_morton_key(float x, float y, float z, const float* b) {
     const float inv[3] = {invSpanOf(b[0], b[3]), invSpanOf(b[1], b[4]), invSpanOf(b[2], b[5])};
     return mortonKey(x, y, z, b, inv);
+}
+
+uint64_t or_morton_key64(double x, double y, double z, const double* b) {
+    const double inv[3] = {invSpan64Of(b[0], b[3]), invSpan64Of(b[1], b[4]), invSpan64Of(b[2], b[5])};
+    return mortonKey64(x, y, z, b, inv);
 }
 
 }  // extern "C"

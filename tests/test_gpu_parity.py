@@ -173,6 +173,45 @@ def test_tree_bit_exact_sort_radix(lib, orc, sort_bits):
         assert_tree_equal(ctx.tree(s), o.tree(s))
 
 
+def test_tree_bit_exact_fp64_morton_box(lib, orc):
+    """With FP64 accumulators the Morton box and the keys come from the FP64 positions: bodies
+    1e3 PSU from the origin (FP32 resolves 6e-5 there, a quantisation step is 1e-5), positions
+    given to FP64 resolution.  Keys, permutation and tree equal the oracle's FP64-box
+    construction, and differ from the FP32-box keys of the same bodies."""
+    from astraea_amd import scenario as S
+
+    st = S.config_small()
+    n = len(st["mass"])
+    rng = np.random.default_rng(5)
+    p64 = {k: st[k].astype(np.float64) + 1.0e3 + rng.uniform(-1e-6, 1e-6, n) for k in ("pos_x", "pos_y", "pos_z")}
+    st32 = dict(st)
+    for k in p64:
+        st32[k] = p64[k].astype(np.float32)
+
+    def build(params):
+        ctx = lib.Context(n)
+        ctx.set_params(params)
+        ctx.set_layout([n])
+        ctx.load_state(st32)
+        if params.precision == lib.PREC_FP64_ACCUM:
+            for a in ("x", "y", "z"):
+                ctx.upload("pos64_" + a, p64["pos_" + a])
+        ctx.sort_and_build()
+        ctx.synchronize()
+        return ctx
+
+    mixed = build(S.plain_params(precision=lib.PREC_FP64_ACCUM))
+    o = orc.Oracle(S.plain_params(precision=lib.PREC_FP64_ACCUM), group_size=lib.GROUP_SIZE)
+    o.set_layout([n])
+    o.load_state(st32)
+    for a in ("x", "y", "z"):
+        o.upload("pos64_" + a, p64["pos_" + a])
+    o.sort_and_build()
+    assert_tree_equal(mixed.tree(0), o.tree(0))
+    plain = build(S.plain_params())
+    assert not np.array_equal(plain.tree(0)["keys"], mixed.tree(0)["keys"])
+
+
 def force_parity(lib, orc, st, counts, params, tol):
     ctx, o = run_pair(lib, orc, st, counts, params, [("initial_forces",)])
     return check_forces(ctx, o, tol)

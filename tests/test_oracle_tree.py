@@ -189,3 +189,21 @@ def test_moments_invariants(orc):
             if ch >= 0:
                 assert np.all(t["node_min"][ch, :3] >= t["node_min"][p, :3])
                 assert np.all(t["node_max"][ch, :3] <= t["node_max"][p, :3])
+
+
+def test_morton_fp64_box(orc):
+    """The mixed form's keys (FP64 positions, FP64 box), pinned by arithmetic: a box 1e4 from the
+    origin with unit span, where FP32 loses ten of the 21 bits.  x = 1e4 + 1e-3 quantises to
+    floor(1e-3 * 2097151) = 2097 in FP64; FP32 gives 2047 (1e4 + 1e-3 rounds to 1e4 + 2^-10),
+    so the FP32 key must differ.  Axes, clamping and the far corner as in the FP32 key."""
+    lo = 1.0e4
+    b = (lo, lo, lo, lo + 1.0, lo + 1.0, lo + 1.0)
+    assert orc.morton_key64(lo + 1e-3, lo, lo, b) == interleave(2097, 0, 0)
+    assert orc.morton_key64(lo, lo + 1e-3, lo, b) == interleave(0, 2097, 0)
+    assert orc.morton_key64(lo, lo, lo + 0.5, b) == interleave(0, 0, 1048575)  # floor(2097151 / 2)
+    assert orc.morton_key64(lo + 1.0, lo + 1.0, lo + 1.0, b) == 0x7FFFFFFFFFFFFFFF
+    assert orc.morton_key64(lo - 5.0, lo + 7.0, lo, b) == interleave(0, 2097151, 0)
+    assert orc.morton_key(lo + 1e-3, lo, lo, b) == interleave(2047, 0, 0)
+    # a zero-span axis quantises to 0
+    flat = (lo, 0.0, lo, lo + 1.0, 0.0, lo + 1.0)
+    assert orc.morton_key64(lo + 0.25, 0.0, lo, flat) == interleave(524287, 0, 0)
