@@ -245,13 +245,17 @@ __device__ __forceinline__ WaveBuild buildWave(const DeviceCtx& c, WaveLds& L, u
             while (sp > 0 && nl <= kFlushAt && nr <= kBucketFlushAt) {
                 const int k = sp < kWave ? sp : kWave;
                 sp -= k;
-                const bool has = lane < k;
+                // up to 32 nodes: a lane per child (lanes 2t, 2t + 1 take node t's two children) in
+                // one pass; more: a lane per node, its two children in two passes
+                const bool paired = k <= kWave / 2;
+                const int at = paired ? lane >> 1 : lane;
+                const bool has = at < k;
                 int2 ch = make_int2(0, 0);
-                if (has) ch = c.t.child[L.stack[sp + lane]];
+                if (has) ch = c.t.child[L.stack[sp + at]];
                 __builtin_amdgcn_wave_barrier();
-#pragma unroll
-                for (int side = 0; side < 2; ++side) {
-                    const int cc = side == 0 ? ch.x : ch.y;
+                const int passes = paired ? 1 : 2;
+                for (int side = 0; side < passes; ++side) {
+                    const int cc = (paired ? (lane & 1) : side) == 0 ? ch.x : ch.y;
                     const bool isInt = has && cc >= 0;
                     bool isLeaf = has && cc < 0;
                     float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f), mn = q, mx = q;
