@@ -9,12 +9,13 @@
 // The group is the 64-lane wavefront (the reference's is the 32-lane warp).
 //
 // The reference pops one node per iteration, so each wave waits on two dependent loads per node.
-// Here the wave expands up to 64 stack nodes of its LDS stack at once, one per lane: each lane
-// loads its node's children and decides the group opening test for them.  The per-particle FP32
-// test is rewritten as r^2 >= T with T the exact FP32 acceptance threshold of the node, and most
-// decisions come from the squared distances between the node's centre of mass and the group's
-// bounding box (nearest point and farthest corner): with a relative margin far above FP32
-// rounding the test is then known to pass (or fail) for all particles.  Of the remaining nodes,
+// Here the wave expands up to 64 stack nodes of its LDS stack at once, one per lane (one child per
+// lane when there are 32 or fewer): each lane loads its node's children and decides the group
+// opening test for them.  The per-particle FP32 test is rewritten as r^2 >= T with T the exact
+// FP32 acceptance threshold of the node, and most decisions come from the squared distances
+// between the node's centre of mass and the group's bounding box (nearest point and farthest
+// corner): with a relative margin far above FP32 rounding the test is then known to pass (or
+// fail) for all particles.  Of the remaining nodes,
 // those that one representative particle fails (the nearest sub-group's extreme particle toward
 // the node, tested with the same FP32 expression) are opened at once; the rest are tested
 // exactly, one node at a time across the lanes (lane = particle), read back from LDS.
