@@ -6,9 +6,10 @@
 //            key = X << 2 | Y << 1 | Z (sph_gravity.cu:151-171, 335-361);
 //   permute  gather of the persistent state by the sorted slot (sph_gravity.cu:363-430), fused
 //            with the Karras build (sph_gravity.cu:432-529), which needs only the sorted keys;
-//   moments  leaf-to-root climb with an arrival counter per node (sph_gravity.cu:531-620): the
-//            counter is an agent-scope acq_rel atomic, so the second arriver sees the first
-//            one's node record across the 8 XCD-private L2 caches.
+//   moments  leaf-to-root climb with an arrival counter per node (sph_gravity.cu:531-620), in
+//            LDS per chunk of leaves, then over the nodes spanning chunks in a second kernel
+//            with agent-scope counters, so the second arriver sees the first one's node record
+//            across the 8 XCD-private L2 caches.
 // All FP32 arithmetic that decides integer results (quantisation, combine of moments) is
 // written with explicit operation order; the oracle repeats it bit for bit.
 #include "launch.hpp"
@@ -184,7 +185,7 @@ __global__ __launch_bounds__(kBlock) void kPermuteKarras(DeviceCtx c) {
     }
 }
 
-constexpr int kChunk = 1024;  // leaf slots per block of the moments climb
+constexpr int kChunk = 512;  // leaf slots per block of the moments climb
 
 struct NodeRec {
     float4 cm, mn, mx;
@@ -250,72 +251,118 @@ __device__ __forceinline__ NodeRec combine(const NodeRec& L, const NodeRec& R) {
     return o;
 }
 
-__device__ __forceinline__ NodeRec leafRec(const DeviceCtx& c, int j) {
-    const float4 p = c.cur.pos[j];
-    const float4 hu = c.cur.hu[j];
-    return NodeRec{p, make_float4(p.x, p.y, p.z, 0.0f), make_float4(p.x, p.y, p.z, supportBound(hu.x, hu.y))};
-}
-
-__device__ __forceinline__ bool inChunk(const DeviceCtx& c, int p, int base) {
+__device__ __forceinline__ bool spansChunks(const DeviceCtx& c, int p) {
     const int o = c.t.other[p];
     const int lo = p < o ? p : o, hi = p < o ? o : p;
-    return lo >= base && hi < base + kChunk;
+    return lo / kChunk != hi / kChunk;
 }
 
-// Leaf-to-root climb over the permuted state (the host has swapped the sets: `cur` is sorted).
-// The first thread to reach a node stops; the second combines both children.
+// Leaf-to-root climb over the permuted state (the host has swapped the sets: `cur` is sorted), in
+// two kernels.  The first thread to reach a node stops; the second combines both children.
 //
-// A block takes a chunk of kChunk consecutive leaf slots.  A node whose leaves all lie in the
-// chunk (its index is one end of its range, so it is in the chunk too) is reached only by the
-// block's own threads: its arrival counter and record live in LDS.  Only the nodes whose range
-// spans two chunks (a few percent) use the global protocol: records written and read with
+// kMoments: a block takes a chunk of kChunk consecutive leaf slots.  A node whose leaves all lie in
+// the chunk (its index is one end of its range, so it is in the chunk too) is reached only by the
+// block's own threads.  The block first loads what the climb reads into LDS with coalesced loads:
+// each slot's leaf (position, support bound) and internal node (children, parent, far end of its
+// range), so that a level of the climb costs LDS accesses only, not three dependent global loads;
+// arrival counters and node records live in LDS too.  A thread that reaches a node spanning two
+// chunks (a few percent of the nodes, all near the top of the tree) counts its arrival there and
+// stops, so no block waits for a climb through the top of the tree.
+//
+// kMomentsTop: the nodes spanning chunks, climbed by one thread per node whose children were both
+// finished by kMoments; its arrivals add to the counts kMoments left, so a node's second arrival
+// (of two children, however they were finished) continues.  Records are written and read with
 // agent-scope relaxed atomics (global stores/loads with the sc1 coherence bit, visible across the
 // XCD-private L2 caches without a whole-L2 write-back), and a thread's record stores completed
 // (s_waitcnt) before its arrival increment.
 __global__ __launch_bounds__(kChunk) void kMoments(DeviceCtx c) {
     __shared__ uint32_t arrive[kChunk];
     __shared__ NodeRec rec[kChunk];
+    __shared__ int2 kid[kChunk];
+    __shared__ int up[kChunk];
+    __shared__ int far[kChunk];
+    __shared__ float4 leafP[kChunk];
+    __shared__ float leafH[kChunk];
     if (stepHeld(c)) return;
     const int base = (int)(blockIdx.x * kChunk);
-    const uint32_t i = (uint32_t)base + threadIdx.x;
-    arrive[threadIdx.x] = 0u;
+    const int t = (int)threadIdx.x;
+    const uint32_t i = (uint32_t)base + (uint32_t)t;
+    arrive[t] = 0u;
+    int p = -1;
+    if (i < c.slots) {
+        const uint32_t sg = c.seg.slotSeg[i];
+        const uint32_t n = c.seg.n[sg], local = i - c.seg.off[sg];
+        if (n >= 2u && local < n) {
+            leafP[t] = c.cur.pos[i];
+            const float4 hu = c.cur.hu[i];
+            leafH[t] = supportBound(hu.x, hu.y);
+            p = c.t.parentL[i];
+            if (local + 1u < n) {  // slot i also holds internal node i
+                kid[t] = c.t.child[i];
+                up[t] = c.t.parentI[i];
+                far[t] = c.t.other[i];
+            }
+        }
+    }
     __syncthreads();
-    if (i >= c.slots) return;
+    const auto leafRecAt = [&](int u) {
+        const float4 q = leafP[u];
+        return NodeRec{q, make_float4(q.x, q.y, q.z, 0.0f), make_float4(q.x, q.y, q.z, leafH[u])};
+    };
+    while (p >= 0) {
+        // node p lies in the chunk when both ends of its range do (p is one end, far[] the other)
+        const int q = p - base;
+        bool in = (uint32_t)q < (uint32_t)kChunk;
+        if (in) in = (uint32_t)(far[q] - base) < (uint32_t)kChunk;
+        if (!in) {
+            __hip_atomic_fetch_add(&c.t.visited[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+        const uint32_t prior = __hip_atomic_fetch_add(&arrive[q], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (prior == 0u) return;
+        const int2 ch = kid[q];
+        const NodeRec L = ch.x < 0 ? leafRecAt(~ch.x - base) : rec[ch.x - base];
+        const NodeRec R = ch.y < 0 ? leafRecAt(~ch.y - base) : rec[ch.y - base];
+        const NodeRec out = combine(L, R);
+        rec[q] = out;
+        c.t.cm[p] = out.cm;
+        c.t.bmin[p] = out.mn;
+        c.t.bmax[p] = out.mx;
+        p = up[q];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void kMomentsTop(DeviceCtx c) {
+    if (stepHeld(c)) return;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t sg = c.seg.slotSeg[i];
     const uint32_t n = c.seg.n[sg];
-    if (i - c.seg.off[sg] >= n || n < 2u) return;
-    int p = c.t.parentL[i];
-    bool local = true;
-    while (p >= 0) {
-        local = local && inChunk(c, p, base);
-        const int2 ch = c.t.child[p];
-        NodeRec out;
-        if (local) {
-            const uint32_t prior =
-                __hip_atomic_fetch_add(&arrive[p - base], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (prior == 0u) return;
-            const NodeRec L = ch.x < 0 ? leafRec(c, ~ch.x) : rec[ch.x - base];
-            const NodeRec R = ch.y < 0 ? leafRec(c, ~ch.y) : rec[ch.y - base];
-            out = combine(L, R);
-            rec[p - base] = out;
-        } else {
-            __builtin_amdgcn_s_waitcnt(0);
-            const uint32_t prior = __hip_atomic_fetch_add(&c.t.visited[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (prior == 0u) return;
-            out = combine(loadChild(c, ch.x), loadChild(c, ch.y));
-        }
-        const int pp = c.t.parentI[p];
-        if (pp >= 0 && !(local && inChunk(c, pp, base))) {
-            // the parent spans chunks: another block may combine it
-            storeAgent(&c.t.cm[p], out.cm);
-            storeAgent(&c.t.bmin[p], out.mn);
-            storeAgent(&c.t.bmax[p], out.mx);
-        } else {
-            c.t.cm[p] = out.cm;
-            c.t.bmin[p] = out.mn;
-            c.t.bmax[p] = out.mx;
-        }
+    if (n < 2u || i - c.seg.off[sg] >= n - 1u) return;  // internal nodes only
+    int p = (int)i;
+    if (!spansChunks(c, p)) return;
+    int2 ch = c.t.child[p];
+    // a child spanning chunks is finished here, by the climb that reaches p through it
+    if ((ch.x >= 0 && spansChunks(c, ch.x)) || (ch.y >= 0 && spansChunks(c, ch.y))) return;
+    NodeRec out = combine(loadChild(c, ch.x), loadChild(c, ch.y));
+    int pp = c.t.parentI[p];
+    // a level costs three round trips to memory: the record's stores (with the parent's topology
+    // loaded meanwhile), the arrival, and the sibling's record (the node's own is in registers)
+    for (;;) {
+        storeAgent(&c.t.cm[p], out.cm);
+        storeAgent(&c.t.bmin[p], out.mn);
+        storeAgent(&c.t.bmax[p], out.mx);
+        if (pp < 0) return;
+        const int2 up = c.t.child[pp];
+        const int upp = c.t.parentI[pp];
+        __builtin_amdgcn_s_waitcnt(0);
+        const uint32_t prior = __hip_atomic_fetch_add(&c.t.visited[pp], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (prior == 0u) return;  // the other child is not finished yet: its climb continues
+        // p is an internal node (>= 0): a leaf child is stored as ~slot (< 0)
+        const bool left = up.x == p;
+        const NodeRec sib = loadChild(c, left ? up.y : up.x);
+        out = left ? combine(out, sib) : combine(sib, out);
         p = pp;
+        pp = upp;
     }
 }
 
@@ -355,6 +402,7 @@ void launchMorton(const DeviceCtx& c, hipStream_t s) { kMorton<<<c.blocks, kBloc
 void launchPermuteKarras(const DeviceCtx& c, hipStream_t s) { kPermuteKarras<<<c.blocks, kBlock, 0, s>>>(c); }
 void launchMoments(const DeviceCtx& c, hipStream_t s) {
     kMoments<<<(c.slots + kChunk - 1) / kChunk, kChunk, 0, s>>>(c);
+    kMomentsTop<<<c.blocks, kBlock, 0, s>>>(c);
 }
 
 }  // namespace astraea_hip
