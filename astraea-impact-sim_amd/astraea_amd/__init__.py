@@ -125,10 +125,16 @@ class PlanetInfo(C.Structure):
 
 
 class Tuning(C.Structure):
-    """astraea_tuning: union records staged per round by pass 1, pass 2 and the drift, and the
-    radix sort's digit width."""
+    """astraea_tuning: union records staged per round by pass 1, pass 2 and the drift, the radix
+    sort's digit width, and the wavefronts per block and stack entries of the gravity walk and of
+    the neighbour build."""
     _fields_ = [("pass1_round", C.c_uint32), ("pass2_round", C.c_uint32), ("drift_round", C.c_uint32),
-                ("sort_bits", C.c_uint32)]
+                ("sort_bits", C.c_uint32), ("gravity_waves", C.c_uint32), ("gravity_stack", C.c_uint32),
+                ("build_waves", C.c_uint32), ("build_stack", C.c_uint32)]
+
+
+# the (wavefronts per block, stack entries) variants of the gravity walk and the neighbour build
+WALK_VARIANTS = ((2, 1024), (4, 1024), (4, 512), (8, 512))
 
 
 class BodyOut(C.Structure):
@@ -432,19 +438,26 @@ class Context:
         _check(_lib.astraea_get_tuning(self._h, C.byref(t)))
         return {k: int(getattr(t, k)) for k, _ in Tuning._fields_}
 
-    def set_tuning(self, pass1_round: int, pass2_round: int, drift_round: int, sort_bits: int = 8):
-        t = Tuning(pass1_round, pass2_round, drift_round, sort_bits)
+    def set_tuning(self, pass1_round: int, pass2_round: int, drift_round: int, sort_bits: int = 8,
+                   gravity_waves: int = 4, gravity_stack: int = 1024, build_waves: int = 4, build_stack: int = 1024):
+        t = Tuning(pass1_round, pass2_round, drift_round, sort_bits, gravity_waves, gravity_stack, build_waves,
+                   build_stack)
         _check(_lib.astraea_set_tuning(self._h, C.byref(t)))
 
     def autotune(self, max_dt: float, steps_per_variant: int = 1) -> dict:
-        """Runs 8 * steps_per_variant steps timing every launch variant (astraea_autotune) and
-        keeps the fastest; returns the choice and the measured mean milliseconds per step."""
+        """Runs 8 * steps_per_variant steps timing every launch variant of the passes, the drift
+        and the sort, then times the gravity and neighbour-build variants on the settled state
+        (astraea_autotune), and keeps the fastest; returns the choice and the measured mean
+        milliseconds (-1 for a walk variant whose stack filled)."""
         t = Tuning()
-        ms = (C.c_double * 10)()
+        ms = (C.c_double * 18)()
         _check(_lib.astraea_autotune(self._h, float(max_dt), int(steps_per_variant), C.byref(t), ms))
         chosen = {k: int(getattr(t, k)) for k, _ in Tuning._fields_}
+        walk = [f"{w}x{s}" for w, s in WALK_VARIANTS]
         return dict(chosen=chosen, pass1_ms={128: ms[0], 256: ms[1]}, pass2_ms={64: ms[2], 128: ms[3]},
-                    drift_ms={64: ms[4], 128: ms[5], 192: ms[6], 256: ms[7]}, sort_ms={8: ms[8], 9: ms[9]})
+                    drift_ms={64: ms[4], 128: ms[5], 192: ms[6], 256: ms[7]}, sort_ms={8: ms[8], 9: ms[9]},
+                    gravity_ms={v: ms[10 + k] for k, v in enumerate(walk)},
+                    build_ms={v: ms[14 + k] for k, v in enumerate(walk)})
 
 
 def tuning_key(particles: int, segments: int, strength: bool, fp64: bool) -> str:
@@ -463,7 +476,9 @@ def tune_cached(ctx: "Context", max_dt: float, path, key: str, steps_per_variant
     table = json.loads(p.read_text()) if p.exists() else {}
     if key in table:
         t = table[key]
-        ctx.set_tuning(t["pass1_round"], t["pass2_round"], t["drift_round"], t.get("sort_bits", 8))
+        ctx.set_tuning(t["pass1_round"], t["pass2_round"], t["drift_round"], t.get("sort_bits", 8),
+                       t.get("gravity_waves", 4), t.get("gravity_stack", 1024), t.get("build_waves", 4),
+                       t.get("build_stack", 1024))
         return dict(chosen=dict(t), cached=True)
     r = ctx.autotune(max_dt, steps_per_variant)
     table[key] = r["chosen"]

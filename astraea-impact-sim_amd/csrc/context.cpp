@@ -570,6 +570,14 @@ int fitNeighbourTable(astraea_ctx* c) {
         HIP_TRY(hipMemset(c->dc.cnt.v + kCntHold, 0, 2 * sizeof(unsigned long long)));
         HIP_TRY(hipMemset(c->dc.cnt.v + 5, 0, 3 * sizeof(unsigned long long)));
         if (v[6] != 0ull) {
+            if (c->dc.tune.buildStack < 1024u) {
+                // the shorter stack of a tuned variant filled: the full one from now on
+                c->dc.tune.buildWaves = 4u;
+                c->dc.tune.buildStack = 1024u;
+                freeGraph(c);
+                launchNeighbours(c->dc, c->stream);
+                continue;
+            }
             c->listValid = false;
             return fail(ASTRAEA_E_OVERFLOW, "neighbour traversal stack overflow");
         }
@@ -1494,10 +1502,20 @@ int astraea_download_tree(astraea_ctx* c, uint32_t seg, uint64_t* keys, uint32_t
     return ASTRAEA_OK;
 }
 
+// The (wavefronts per block, stack entries) pairs of the gravity walk and the neighbour build.
+static constexpr uint32_t kWalkWaves[4] = {2u, 4u, 4u, 8u}, kWalkStack[4] = {1024u, 1024u, 512u, 512u};
+
+static bool validWalk(uint32_t waves, uint32_t stack) {
+    for (int k = 0; k < 4; ++k)
+        if (waves == kWalkWaves[k] && stack == kWalkStack[k]) return true;
+    return false;
+}
+
 static bool validTuning(const astraea_tuning& t) {
     return (t.pass1_round == 128u || t.pass1_round == 256u) && (t.pass2_round == 64u || t.pass2_round == 128u) &&
            (t.drift_round == 64u || t.drift_round == 128u || t.drift_round == 192u || t.drift_round == 256u) &&
-           (t.sort_bits == 8u || t.sort_bits == 9u);
+           (t.sort_bits == 8u || t.sort_bits == 9u) && validWalk(t.gravity_waves, t.gravity_stack) &&
+           validWalk(t.build_waves, t.build_stack);
 }
 
 int astraea_get_tuning(astraea_ctx* c, astraea_tuning* out) {
@@ -1506,6 +1524,10 @@ int astraea_get_tuning(astraea_ctx* c, astraea_tuning* out) {
     out->pass2_round = c->dc.tune.pass2Round;
     out->drift_round = c->dc.tune.driftRound;
     out->sort_bits = c->dc.tune.sortBits;
+    out->gravity_waves = c->dc.tune.gravityWaves;
+    out->gravity_stack = c->dc.tune.gravityStack;
+    out->build_waves = c->dc.tune.buildWaves;
+    out->build_stack = c->dc.tune.buildStack;
     return ASTRAEA_OK;
 }
 
@@ -1513,16 +1535,113 @@ int astraea_set_tuning(astraea_ctx* c, const astraea_tuning* t) {
     if (!c || !t) return fail(ASTRAEA_E_INVALID, "null argument");
     if (!validTuning(*t))
         return fail(ASTRAEA_E_INVALID,
-                    "tuning (%u, %u, %u, %u) outside pass 1 {128, 256}, pass 2 {64, 128}, drift {64, 128, 192, 256}, "
-                    "sort bits {8, 9}",
-                    t->pass1_round, t->pass2_round, t->drift_round, t->sort_bits);
+                    "tuning (%u, %u, %u, %u, gravity %u x %u, build %u x %u) outside pass 1 {128, 256}, pass 2 "
+                    "{64, 128}, drift {64, 128, 192, 256}, sort bits {8, 9}, (waves, stack) {(2, 1024), (4, 1024), "
+                    "(4, 512), (8, 512)}",
+                    t->pass1_round, t->pass2_round, t->drift_round, t->sort_bits, t->gravity_waves, t->gravity_stack,
+                    t->build_waves, t->build_stack);
     c->dc.tune.pass1Round = t->pass1_round;
     c->dc.tune.pass2Round = t->pass2_round;
     c->dc.tune.driftRound = t->drift_round;
     c->dc.tune.sortBits = t->sort_bits;
+    c->dc.tune.gravityWaves = t->gravity_waves;
+    c->dc.tune.gravityStack = t->gravity_stack;
+    c->dc.tune.buildWaves = t->build_waves;
+    c->dc.tune.buildStack = t->build_stack;
     freeGraph(c);  // a captured step holds the old kernels
     return ASTRAEA_OK;
 }
+
+extern "C++" {
+// Milliseconds of the work `launch` enqueues on the context stream, timed with events.
+template <class F>
+static int timeLaunch(astraea_ctx* c, F&& launch, double* ms) {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    int rc = ASTRAEA_OK;
+    for (auto& x : e)
+        if (rc == ASTRAEA_OK && hipEventCreate(&x) != hipSuccess) rc = fail(ASTRAEA_E_HIP, "hipEventCreate failed");
+    if (rc == ASTRAEA_OK && hipEventRecord(e[0], c->stream) != hipSuccess) rc = fail(ASTRAEA_E_HIP, "hipEventRecord failed");
+    if (rc == ASTRAEA_OK) {
+        launch();
+        rc = checkLaunch(c);
+    }
+    if (rc == ASTRAEA_OK && hipEventRecord(e[1], c->stream) != hipSuccess) rc = fail(ASTRAEA_E_HIP, "hipEventRecord failed");
+    if (rc == ASTRAEA_OK && hipEventSynchronize(e[1]) != hipSuccess) rc = fail(ASTRAEA_E_HIP, "hipEventSynchronize failed");
+    float t = 0.0f;
+    if (rc == ASTRAEA_OK && hipEventElapsedTime(&t, e[0], e[1]) != hipSuccess) rc = fail(ASTRAEA_E_HIP, "hipEventElapsedTime failed");
+    for (auto& x : e)
+        if (x) (void)hipEventDestroy(x);
+    *ms = t;
+    return rc;
+}
+
+// The (waves, stack) variants of the gravity walk and the neighbour build, timed by launching them
+// again on the settled state (they rewrite what the last step computed, bit for bit): eight runs in
+// palindromic order.  A gravity variant that force-accepts a node or a build variant whose stack
+// fills is ruled out (time -1).  The table is then built again with the build variant chosen.
+static int tuneWalks(astraea_ctx* c, double gms[4], double bms[4], int* gBest, int* bBest) {
+    static constexpr int order4[8] = {0, 1, 2, 3, 3, 2, 1, 0};
+    bool gOk[4] = {true, true, true, true}, bOk[4] = {true, true, true, true};
+    for (int k = 0; k < 4; ++k) gms[k] = bms[k] = 0.0;
+    unsigned long long saved[2];  // the interaction and forced-acceptance counts of the steps
+    HIP_TRY(hipMemcpy(saved, c->dc.cnt.v, sizeof(saved), hipMemcpyDeviceToHost));
+    const Tuning keep = c->dc.tune;
+    int rc = ASTRAEA_OK;
+    for (int r = 0; r < 8 && rc == ASTRAEA_OK; ++r) {
+        const int k = order4[r];
+        c->dc.tune.gravityWaves = kWalkWaves[k];
+        c->dc.tune.gravityStack = kWalkStack[k];
+        double t = 0.0;
+        rc = timeLaunch(c, [&] { launchGravity(c->dc, c->stream); }, &t);
+        unsigned long long v[8];
+        if (rc == ASTRAEA_OK && hipMemcpy(v, c->dc.cnt.v, sizeof(v), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(ASTRAEA_E_HIP, "hipMemcpy failed");
+        if (rc != ASTRAEA_OK) break;
+        if (v[1] != saved[1]) gOk[k] = false;
+        gms[k] += 0.5 * t;
+    }
+    for (int r = 0; r < 8 && rc == ASTRAEA_OK; ++r) {
+        const int k = order4[r];
+        c->dc.tune.buildWaves = kWalkWaves[k];
+        c->dc.tune.buildStack = kWalkStack[k];
+        double t = 0.0;
+        rc = timeLaunch(c, [&] { launchNeighbours(c->dc, c->stream); }, &t);
+        unsigned long long v[8];
+        if (rc == ASTRAEA_OK && hipMemcpy(v, c->dc.cnt.v, sizeof(v), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(ASTRAEA_E_HIP, "hipMemcpy failed");
+        if (rc != ASTRAEA_OK) break;
+        if (v[6] != 0ull) bOk[k] = false;
+        bms[k] += 0.5 * t;
+        // a held build (a full stack, or a spill pool the variant found too small) leaves the
+        // table incomplete: clear the flags; the chosen variant builds it again below
+        if (v[kCntHold] != 0ull) {
+            HIP_TRY(hipMemset(c->dc.cnt.v + kCntHold, 0, 2 * sizeof(unsigned long long)));
+            HIP_TRY(hipMemset(c->dc.cnt.v + 5, 0, 3 * sizeof(unsigned long long)));
+        }
+    }
+    // restore the step's counts (the gravity launches above added interactions)
+    if (rc == ASTRAEA_OK && hipMemcpy(c->dc.cnt.v, saved, sizeof(saved), hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(ASTRAEA_E_HIP, "hipMemcpy failed");
+    c->dc.tune = keep;
+    if (rc != ASTRAEA_OK) return rc;
+    int g = -1, b = -1;
+    for (int k = 0; k < 4; ++k) {
+        if (gOk[k] && (g < 0 || gms[k] < gms[g])) g = k;
+        if (bOk[k] && (b < 0 || bms[k] < bms[b])) b = k;
+        if (!gOk[k]) gms[k] = -1.0;
+        if (!bOk[k]) bms[k] = -1.0;
+    }
+    *gBest = g < 0 ? 1 : g;  // (4, 1024) when every variant filled its stack
+    *bBest = b < 0 ? 1 : b;
+    c->dc.tune.buildWaves = kWalkWaves[*bBest];
+    c->dc.tune.buildStack = kWalkStack[*bBest];
+    launchNeighbours(c->dc, c->stream);
+    TRY(checkLaunch(c));
+    rc = fitNeighbourTable(c);
+    c->dc.tune = keep;
+    return rc;
+}
+}  // extern "C++"
 
 int astraea_autotune(astraea_ctx* c, float maxDt, uint32_t stepsPerVariant, astraea_tuning* chosen, double* ms) {
     if (!c) return fail(ASTRAEA_E_INVALID, "null context");
@@ -1533,12 +1652,16 @@ int astraea_autotune(astraea_ctx* c, float maxDt, uint32_t stepsPerVariant, astr
     // sequence, so a steady drift of the step's cost as the state evolves cancels out
     static constexpr int order2[8] = {0, 1, 1, 0, 0, 1, 1, 0}, order4[8] = {0, 1, 2, 3, 3, 2, 1, 0};
     const bool wasTiming = c->timing;
-    const astraea_tuning before = {c->dc.tune.pass1Round, c->dc.tune.pass2Round, c->dc.tune.driftRound,
-                                   c->dc.tune.sortBits};
+    astraea_tuning before;
+    TRY(astraea_get_tuning(c, &before));
     double t1[2] = {0.0, 0.0}, t2[2] = {0.0, 0.0}, td[4] = {0.0, 0.0, 0.0, 0.0}, ts[2] = {0.0, 0.0};
     int rc = astraea_enable_stage_timing(c, 1);
     for (int r = 0; r < 8 && rc == ASTRAEA_OK; ++r) {
-        const astraea_tuning t = {p1[order2[r]], p2[order2[r]], dr[order4[r]], sb[order2[r]]};
+        astraea_tuning t = before;
+        t.pass1_round = p1[order2[r]];
+        t.pass2_round = p2[order2[r]];
+        t.drift_round = dr[order4[r]];
+        t.sort_bits = sb[order2[r]];
         rc = astraea_set_tuning(c, &t);
         if (rc == ASTRAEA_OK) rc = astraea_step(c, maxDt, stepsPerVariant, 0);
         if (rc == ASTRAEA_OK) rc = settle(c);
@@ -1553,6 +1676,9 @@ int astraea_autotune(astraea_ctx* c, float maxDt, uint32_t stepsPerVariant, astr
         ts[order2[r]] += 0.25 * mean(ASTRAEA_STAGE_SORT);
     }
     const int rt = astraea_enable_stage_timing(c, wasTiming ? 1 : 0);
+    double gms[4] = {0.0, 0.0, 0.0, 0.0}, bms[4] = {0.0, 0.0, 0.0, 0.0};
+    int gBest = 1, bBest = 1;
+    if (rc == ASTRAEA_OK) rc = tuneWalks(c, gms, bms, &gBest, &bBest);
     if (rc != ASTRAEA_OK) {
         (void)astraea_set_tuning(c, &before);
         return rc;
@@ -1562,12 +1688,14 @@ int astraea_autotune(astraea_ctx* c, float maxDt, uint32_t stepsPerVariant, astr
     for (int k = 1; k < 4; ++k)
         if (td[k] < td[bd]) bd = k;
     const astraea_tuning best = {t1[0] < t1[1] ? p1[0] : p1[1], t2[0] < t2[1] ? p2[0] : p2[1], dr[bd],
-                                 ts[0] < ts[1] ? sb[0] : sb[1]};
+                                 ts[0] < ts[1] ? sb[0] : sb[1], kWalkWaves[gBest], kWalkStack[gBest],
+                                 kWalkWaves[bBest], kWalkStack[bBest]};
     TRY(astraea_set_tuning(c, &best));
     if (chosen) *chosen = best;
     if (ms) {
-        const double all[10] = {t1[0], t1[1], t2[0], t2[1], td[0], td[1], td[2], td[3], ts[0], ts[1]};
-        for (int k = 0; k < 10; ++k) ms[k] = all[k];
+        const double all[18] = {t1[0], t1[1], t2[0], t2[1], td[0], td[1], td[2], td[3], ts[0], ts[1],
+                                gms[0], gms[1], gms[2], gms[3], bms[0], bms[1], bms[2], bms[3]};
+        for (int k = 0; k < 18; ++k) ms[k] = all[k];
     }
     return ASTRAEA_OK;
 }

@@ -33,15 +33,15 @@
 namespace astraea_hip {
 namespace {
 
-constexpr int kSCap = 1024;                 // per-wave node stack
 constexpr int kSub = 4;                     // sub-groups of a wavefront with boxes of their own
 constexpr int kSubLanes = kWave / kSub;
 constexpr int kFlushAt = 32;                // evaluate once either list holds more than this
 constexpr int kRCap = kFlushAt + 2 * kWave; // one round lists at most 128 records
 constexpr int kPend = kRCap - kWave;        // undecided nodes of one side, staged past the records
 
+template <int SCAP>  // per-wave node stack: 1024 or 512 entries (a launch variant)
 struct WaveLds {
-    int stack[kSCap];
+    int stack[SCAP];
     float4 node[kRCap];     // accepted node: (com x, y, z, G m)
     float4 leaf[kRCap];     // leaf: (x, y, z, G m)
     float leafEps[kRCap];   // eps_j / 2
@@ -72,20 +72,23 @@ __device__ __forceinline__ float acceptThreshold(float e2, float theta2, float i
 
 // GROUP particles share the opening decisions of one wavefront: 64 (all lanes), or 32, the
 // reference's warp groups, decided by lanes 0-31 while lanes 32-63 help with their partners'
-// exact tests and interactions.  A block covers kWavesPerBlock * GROUP slots, which never
-// straddle a segment (segments start at multiples of 256).
-template <bool CHECK_INSIDE, int GROUP>
-__global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
+// exact tests and interactions.  A block is W wavefronts over W * GROUP slots, and a wavefront's
+// GROUP slots never straddle a segment (segments start at multiples of 256).  W and the stack
+// depth SCAP are launch variants: they set how many wavefronts share a block's LDS and a CU's L1
+// (neighbouring groups walk the same nodes), never the results while the stack does not fill.
+template <bool CHECK_INSIDE, int GROUP, int W, int SCAP>
+__global__ __launch_bounds__(W * kWave) void kGravity(DeviceCtx c) {
     static_assert(GROUP == kWave || GROUP == kWave / 2, "a group is a wavefront or half of one");
     constexpr int kSubG = kSub * GROUP / kWave;  // the sub-groups that hold the group's lanes
-    constexpr uint32_t kSlots = (uint32_t)(kWavesPerBlock * GROUP);
-    __shared__ WaveLds lds[kWavesPerBlock];
+    constexpr uint32_t kSlots = (uint32_t)(W * GROUP);
+    __shared__ WaveLds<SCAP> lds[W];
     if (stepHeld(c)) return;
     const uint32_t b = xcdBlock(blockIdx.x, gridDim.x);
     const int wave = threadIdx.x >> 6, lane = laneId();
     const uint32_t waveBase = b * kSlots + (uint32_t)wave * GROUP;
+    if (waveBase >= c.slots) return;  // the last block of a grid over a multiple of 256 slots
     const uint32_t i = waveBase + (uint32_t)lane;
-    const uint32_t sg = uniformU32(c.seg.slotSeg[b * kSlots]);
+    const uint32_t sg = uniformU32(c.seg.slotSeg[waveBase]);
     const uint32_t off = uniformU32(c.seg.off[sg]), n = uniformU32(c.seg.n[sg]);
     const bool active = lane < GROUP && i - off < n;
     const uint64_t am = ballot(active);
@@ -94,7 +97,7 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
         if (active) c.d.grav[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         return;
     }
-    WaveLds& L = lds[wave];
+    WaveLds<SCAP>& L = lds[wave];
     const float theta2 = c.prm.theta2;
     const float invTheta2 = theta2 > 0.0f ? 1.0f / theta2 : 0.0f;
     const bool pow2 = c.prm.theta2Pow2 != 0u;
@@ -330,7 +333,7 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
                 const uint64_t mPush = ballot(toPush);
                 const int nPush = __popcll(mPush);
                 const int pushAt = sp + __popcll(mPush & lt);
-                if (toPush && pushAt >= kSCap) toPush = false;  // forced acceptance (reference: full stack)
+                if (toPush && pushAt >= SCAP) toPush = false;  // forced acceptance (reference: full stack)
                 const bool forcedHere = isInt && !accept && !toPush;
                 const bool asNode = isInt && (accept || forcedHere);
                 const uint64_t mNode = ballot(asNode), mLeaf = ballot(isLeaf);
@@ -343,7 +346,7 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
                 if (toPush) L.stack[pushAt] = cc;
                 nn += __popcll(mNode);
                 nf += __popcll(mLeaf);
-                sp = sp + nPush < kSCap ? sp + nPush : kSCap;
+                sp = sp + nPush < SCAP ? sp + nPush : SCAP;
                 forced += (uint32_t)__popcll(ballot(forcedHere));
             }
             __builtin_amdgcn_wave_barrier();
@@ -403,20 +406,29 @@ __global__ __launch_bounds__(kBlock) void kGravity(DeviceCtx c) {
     }
 }
 
+// The launch variants (waves per block, stack depth): (2, 1024), (4, 1024), (4, 512), (8, 512);
+// a block's LDS is W wavefront records of 10 KB (8 KB with the shorter stack).
+template <int GROUP, int W, int SCAP>
+void gravityAs(const DeviceCtx& c, hipStream_t s) {
+    const uint32_t grid = (c.slots + (uint32_t)(W * GROUP) - 1u) / (uint32_t)(W * GROUP);
+    if (c.prm.theta2 > 0.333333f) kGravity<true, GROUP, W, SCAP><<<grid, W * kWave, 0, s>>>(c);
+    else kGravity<false, GROUP, W, SCAP><<<grid, W * kWave, 0, s>>>(c);
+}
+
+template <int GROUP>
+void gravityGroup(const DeviceCtx& c, hipStream_t s) {
+    const uint32_t w = c.tune.gravityWaves, st = c.tune.gravityStack;
+    if (w == 2u) gravityAs<GROUP, 2, 1024>(c, s);
+    else if (w == 8u) gravityAs<GROUP, 8, 512>(c, s);
+    else if (st == 512u) gravityAs<GROUP, 4, 512>(c, s);
+    else gravityAs<GROUP, 4, 1024>(c, s);
+}
 
 }  // namespace
 
 void launchGravity(const DeviceCtx& c, hipStream_t s) {
-    const bool inside = c.prm.theta2 > 0.333333f;
-    if (c.prm.group == 32u) {
-        // 128 slots per block: twice the blocks over the same slots
-        if (inside) kGravity<true, 32><<<2 * c.blocks, kBlock, 0, s>>>(c);
-        else kGravity<false, 32><<<2 * c.blocks, kBlock, 0, s>>>(c);
-    } else if (inside) {
-        kGravity<true, kWave><<<c.blocks, kBlock, 0, s>>>(c);
-    } else {
-        kGravity<false, kWave><<<c.blocks, kBlock, 0, s>>>(c);
-    }
+    if (c.prm.group == 32u) gravityGroup<32>(c, s);
+    else gravityGroup<kWave>(c, s);
 }
 
 }  // namespace astraea_hip

@@ -29,7 +29,6 @@
 namespace astraea_hip {
 namespace {
 
-constexpr int kSCap = 1024;                  // per-wave node stack
 constexpr int kCCap = 256;                   // per-wave candidate records
 constexpr int kFlushAt = kCCap - 2 * kWave;  // one round lists at most 128 leaves
 constexpr int kBucket = 16;                  // an open node with this many leaves or fewer is listed whole
@@ -39,8 +38,9 @@ constexpr int kSlotBits = 28;                // a bucket: first leaf slot | (lea
 constexpr uint32_t kBucketSlotMask = (1u << kSlotBits) - 1u;
 static_assert(kBucket <= (1 << (32 - kSlotBits)) && kWave % kBucket == 0, "a bucket's leaf count fits the top bits");
 
+template <int SCAP>  // per-wave node stack: 1024 or 512 entries (a launch variant)
 struct WaveLds {
-    int stack[kSCap];
+    int stack[SCAP];
     float4 cand[kCCap];  // candidate particle (x, y, z, squared support bound)
     int candId[kCCap];
     uint32_t bucket[kBCap];
@@ -100,7 +100,8 @@ struct WaveBuild {
 
 // Builds the table of wavefront gw (slots 64 gw .. 64 gw + 63) into `list` (64 lanes x K entries in
 // chunks, WaveTable) and `U` (UC union ids).  Every lane of the wavefront calls it.
-__device__ __forceinline__ WaveBuild buildWave(const DeviceCtx& c, WaveLds& L, uint32_t gw, int lane, uint16_t* list,
+template <int SCAP>
+__device__ __forceinline__ WaveBuild buildWave(const DeviceCtx& c, WaveLds<SCAP>& L, uint32_t gw, int lane, uint16_t* list,
                                                uint32_t K, uint32_t* U, uint32_t UC) {
     const uint32_t i = gw * kWave + (uint32_t)lane;
     const uint32_t sg = uniformU32(c.seg.slotSeg[gw * kWave]);
@@ -300,13 +301,13 @@ __device__ __forceinline__ WaveBuild buildWave(const DeviceCtx& c, WaveLds& L, u
                     nr += __popcll(mBucket);
                     const uint64_t mPush = ballot(open);
                     const int pushAt = sp + __popcll(mPush & lt);
-                    if (open && pushAt >= kSCap) {
+                    if (open && pushAt >= SCAP) {
                         open = false;
                         stackFull = true;
                     }
                     if (open) L.stack[pushAt] = cc;
                     const int nPush = __popcll(mPush);
-                    sp = sp + nPush < kSCap ? sp + nPush : kSCap;
+                    sp = sp + nPush < SCAP ? sp + nPush : SCAP;
                     listLeaves(isLeaf, q, (uint32_t)~cc);
                 }
                 __builtin_amdgcn_wave_barrier();
@@ -379,13 +380,17 @@ __device__ __forceinline__ bool finishWave(const DeviceCtx& c, uint32_t gw, int 
     return want <= K && w.ulen <= UC;
 }
 
-// First build: every wavefront into the regular region; the ones that need more are listed.
-__global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
-    __shared__ WaveLds lds[kWavesPerBlock];
+// First build: every wavefront into the regular region; the ones that need more are listed.  A
+// block is W wavefronts with SCAP-entry stacks (launch variants: how many neighbouring walks share a
+// CU's L1 and a block's LDS; a full stack makes the build incomplete, and the host builds again).
+template <int W, int SCAP>
+__global__ __launch_bounds__(W * kWave) void kNeighbours(DeviceCtx c) {
+    __shared__ WaveLds<SCAP> lds[W];
     if (stepHeld(c)) return;
     const uint32_t b = xcdBlock(blockIdx.x, gridDim.x);
     const int wave = threadIdx.x >> 6, lane = laneId();
-    const uint32_t gw = (b * kBlock) / kWave + (uint32_t)wave;
+    const uint32_t gw = b * (uint32_t)W + (uint32_t)wave;
+    if (gw * (uint32_t)kWave >= c.slots) return;  // the last block of a grid over a multiple of 256 slots
     const uint32_t K = c.nb.stride, UC = c.nb.unionCap;
     uint16_t* list = c.nb.list + (uint64_t)gw * (64ull * K);
     uint32_t* U = c.nb.unionIds + (uint64_t)gw * UC;
@@ -410,7 +415,7 @@ __global__ __launch_bounds__(kBlock) void kNeighbours(DeviceCtx c) {
 // does the atomics, then every lane reads its result), so no lane can be left behind while the
 // others read lane 0.
 __global__ __launch_bounds__(kBlock) void kNeighboursSpill(DeviceCtx c) {
-    __shared__ WaveLds lds[kWavesPerBlock];
+    __shared__ WaveLds<1024> lds[kWavesPerBlock];
     if (stepHeld(c)) return;
     const int wave = threadIdx.x >> 6, lane = laneId();
     const uint32_t k = uniformU32(blockIdx.x * (uint32_t)kWavesPerBlock + (uint32_t)wave);
@@ -449,11 +454,22 @@ __global__ void kNeighboursReset(DeviceCtx c) {
     if (threadIdx.x < 2) c.nb.q[threadIdx.x] = 0ull;
 }
 
+// The launch variants (waves per block, stack depth): (2, 1024), (4, 1024), (4, 512), (8, 512).
+template <int W, int SCAP>
+void buildAs(const DeviceCtx& c, hipStream_t s) {
+    const uint32_t grid = (c.slots + (uint32_t)(W * kWave) - 1u) / (uint32_t)(W * kWave);
+    kNeighbours<W, SCAP><<<grid, W * kWave, 0, s>>>(c);
+}
+
 }  // namespace
 
 void launchNeighbours(const DeviceCtx& c, hipStream_t s) {
     kNeighboursReset<<<1, 64, 0, s>>>(c);
-    kNeighbours<<<c.blocks, kBlock, 0, s>>>(c);
+    const uint32_t w = c.tune.buildWaves, st = c.tune.buildStack;
+    if (w == 2u) buildAs<2, 1024>(c, s);
+    else if (w == 8u) buildAs<8, 512>(c, s);
+    else if (st == 512u) buildAs<4, 512>(c, s);
+    else buildAs<4, 1024>(c, s);
     kNeighboursSpill<<<c.blocks, kBlock, 0, s>>>(c);
 }
 

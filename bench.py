@@ -471,7 +471,8 @@ def main():
                     "bytes_per_particle": ALG_BYTES[k]} for k in ALG_BYTES},
         "gravity_interactions_per_s": inter_per_launch / (grav_avg_ms * 1e-3),
         "tuning": b["tune"] and {"chosen": b["tune"]["chosen"], "steps": 8,
-                                 "ms_per_step": {k: b["tune"][k] for k in ("pass1_ms", "pass2_ms", "drift_ms", "sort_ms")}},
+                                 "ms_per_step": {k: b["tune"][k] for k in ("pass1_ms", "pass2_ms", "drift_ms", "sort_ms")},
+                                 "ms_per_launch": {k: b["tune"][k] for k in ("gravity_ms", "build_ms")}},
         "step_roofline": step_roofline(stages, n, n1, n2, inter_pp, 1e3 * b["wall"] / K),
         "traffic_ratio": kernel_traffic_ratios(),
         "roofline": {

@@ -358,24 +358,36 @@ int astraea_stage_times(astraea_ctx* ctx, double* out_ms, uint64_t* out_samples)
 
 /* Launch variants (no counterpart in the reference, whose block sizes, stack depth and sort are
  * compile-time constants or a library's, sph_gravity.cu:19-22, 2028-2046): the union records each
- * neighbour pass stages in LDS per round, and the radix sort's digit width.  Every variant gives
- * bit-identical results (a particle's pairs run in the same order; every sort is the stable sort
- * by key); they differ in LDS use, and so in how many wavefronts share a SIMD, and in passes. */
+ * neighbour pass stages in LDS per round, the radix sort's digit width, and the wavefronts per
+ * block and per-wave traversal stack entries of the gravity walk and of the neighbour build.
+ * Every variant gives bit-identical results (a particle's pairs and records run in the same order;
+ * every sort is the stable sort by key) as long as no stack fills; they differ in LDS use, and so
+ * in how many wavefronts share a SIMD or a CU's cache, and in passes.  The (waves, stack) pairs
+ * are (2, 1024), (4, 1024), (4, 512) and (8, 512). */
 typedef struct astraea_tuning {
-    uint32_t pass1_round; /* 128 or 256 (default 256) */
-    uint32_t pass2_round; /* 64 or 128 (default 128) */
-    uint32_t drift_round; /* 64, 128, 192 or 256 (default 128) */
-    uint32_t sort_bits;   /* 8 (8 passes) or 9 (7 passes) (default 8) */
+    uint32_t pass1_round;   /* 128 or 256 (default 256) */
+    uint32_t pass2_round;   /* 64 or 128 (default 128) */
+    uint32_t drift_round;   /* 64, 128, 192 or 256 (default 128) */
+    uint32_t sort_bits;     /* 8 (8 passes) or 9 (7 passes) (default 8) */
+    uint32_t gravity_waves; /* wavefronts per block of the gravity walk (default 4) */
+    uint32_t gravity_stack; /* its per-wave stack entries (default 1024) */
+    uint32_t build_waves;   /* wavefronts per block of the neighbour build (default 4) */
+    uint32_t build_stack;   /* its per-wave stack entries (default 1024) */
 } astraea_tuning;
 int astraea_get_tuning(astraea_ctx* ctx, astraea_tuning* out);
 int astraea_set_tuning(astraea_ctx* ctx, const astraea_tuning* tuning);
 /* The autotuner: advances the loaded state by 8 * steps_per_variant eager steps, timing every
  * variant of each stage with the stage events (in palindromic orders, so that the state's slow
  * change of cost cancels), and keeps the fastest for each.  The simulation does not depend
- * on the variant, so these are ordinary steps.  `chosen` and `ms`
- * (mean milliseconds per step: pass 1 at 128 and 256, pass 2 at 64 and 128, drift at 64, 128,
- * 192 and 256, the sort at 8 and 9 bits; 10 values) may be NULL.  On an error the previous
- * tuning is restored. */
+ * on the variant, so these are ordinary steps.  Then it launches the gravity walk and the
+ * neighbour build of the resulting state again in each (waves, stack) variant, twice each in
+ * palindromic order, timed with events; those launches rewrite what the step computed, bit for
+ * bit, and a variant whose stack filled (a forced acceptance, an incomplete table) is not chosen
+ * (the table is then built again with the chosen variant).  `chosen` and `ms` (mean milliseconds:
+ * pass 1 at 128 and 256, pass 2 at 64 and 128, drift at 64, 128, 192 and 256, the sort at 8 and 9
+ * bits, then gravity and the build at (2, 1024), (4, 1024), (4, 512) and (8, 512); 18 values, a
+ * variant whose stack filled reads -1) may be NULL.  On an error the previous tuning is
+ * restored. */
 int astraea_autotune(astraea_ctx* ctx, float max_dt_psu, uint32_t steps_per_variant, astraea_tuning* chosen,
                      double* ms);
 

@@ -1,8 +1,11 @@
 """Launch variants and the autotuner (astraea_set_tuning, astraea_autotune).
 
-A variant changes how many union records a neighbour pass stages in LDS per round, never the
-order in which a particle's pairs are summed, so every variant must give bit-identical states;
-the autotuner's steps are ordinary steps and must equal the same number of untuned steps.
+A variant changes how many union records a neighbour pass stages in LDS per round, the sort's digit
+width, or how many wavefronts share a block and how deep their stacks are in the gravity walk and
+the neighbour build, never the order in which a particle's pairs or records are summed, so every
+variant must give bit-identical states; the autotuner's steps are ordinary steps, and its extra
+gravity and build launches rewrite what the last step computed, so the tuned run must equal the
+same number of untuned steps.
 """
 import numpy as np
 import pytest
@@ -38,7 +41,8 @@ def _state(ctx):
 def test_variants_are_bit_identical(lib, strength):
     st = perturbed_small_state(strength=strength)
     want = None
-    for t in [(256, 128, 128, 8), (128, 64, 64, 9), (256, 64, 192, 9), (128, 128, 256, 8)]:
+    for t in [(256, 128, 128, 8, 4, 1024, 4, 1024), (128, 64, 64, 9, 2, 1024, 8, 512),
+              (256, 64, 192, 9, 8, 512, 2, 1024), (128, 128, 256, 8, 4, 512, 4, 512)]:
         ctx = _context(lib, st, strength)
         ctx.set_tuning(*t)
         assert tuple(ctx.tuning().values()) == t
@@ -57,8 +61,15 @@ def test_autotune_steps_are_ordinary_steps(lib):
     r = tuned.autotune(MAX_DT, steps_per_variant=1)
     assert r["chosen"]["pass1_round"] in (128, 256) and r["chosen"]["pass2_round"] in (64, 128)
     assert r["chosen"]["drift_round"] in (64, 128, 192, 256) and r["chosen"]["sort_bits"] in (8, 9)
+    assert (r["chosen"]["gravity_waves"], r["chosen"]["gravity_stack"]) in lib.WALK_VARIANTS
+    assert (r["chosen"]["build_waves"], r["chosen"]["build_stack"]) in lib.WALK_VARIANTS
     assert tuned.tuning() == r["chosen"]
     assert all(v > 0.0 for d in ("pass1_ms", "pass2_ms", "drift_ms", "sort_ms") for v in r[d].values())
+    # a walk variant whose stack filled reads -1; the chosen ones were timed
+    assert all(v > 0.0 or v == -1.0 for d in ("gravity_ms", "build_ms") for v in r[d].values())
+    g = f"{r['chosen']['gravity_waves']}x{r['chosen']['gravity_stack']}"
+    b = f"{r['chosen']['build_waves']}x{r['chosen']['build_stack']}"
+    assert r["gravity_ms"][g] > 0.0 and r["build_ms"][b] > 0.0
     plain = _context(lib, st, False)
     plain.step(MAX_DT, 8, graph=False)
     a, b = _state(tuned), _state(plain)
@@ -72,4 +83,9 @@ def test_invalid_tuning_is_rejected(lib):
         ctx.set_tuning(512, 128, 128)
     with pytest.raises(lib.AstraeaError):
         ctx.set_tuning(256, 128, 128, 11)
-    assert ctx.tuning() == {"pass1_round": 256, "pass2_round": 128, "drift_round": 128, "sort_bits": 8}
+    with pytest.raises(lib.AstraeaError):
+        ctx.set_tuning(256, 128, 128, 8, gravity_waves=8, gravity_stack=1024)  # 80 KB of LDS per block
+    with pytest.raises(lib.AstraeaError):
+        ctx.set_tuning(256, 128, 128, 8, build_waves=3)
+    assert ctx.tuning() == {"pass1_round": 256, "pass2_round": 128, "drift_round": 128, "sort_bits": 8,
+                            "gravity_waves": 4, "gravity_stack": 1024, "build_waves": 4, "build_stack": 1024}
