@@ -182,19 +182,23 @@ __global__ __launch_bounds__(kBlock) void kEos(DeviceCtx c) {
 }
 
 // Pass-2 neighbour record staged in LDS: 64 bytes, 80 with the stress deviator S/rho^2 (a
-// smaller record lets five blocks share a CU instead of four).
+// smaller record lets five blocks share a CU instead of four).  v.w holds max(rho_j, 1e-9) and
+// volJ = m_j / rho_j (hardware reciprocal), once per staged record rather than per pair.
 template <bool STRENGTH>
 struct Pass2Rec {
     Vec4 p, n, v;  // n.x holds max(h_j, 1e-6)
     uint32_t mat;
     float invH, gNorm;  // the kernel gradient's constants of H_j, once per staged record
+    float volJ;
 };
 template <>
 struct Pass2Rec<true> {
     Vec4 p, n, v, s0;
     Vec2 s1;
     uint32_t mat;
+    float volJ;
 };
+static_assert(sizeof(Pass2Rec<false>) == 64 && sizeof(Pass2Rec<true>) == 80, "pass-2 records are 4 or 5 quads");
 
 template <bool STRENGTH, int S>
 __global__ __launch_bounds__(kBlock) void kPass2(DeviceCtx c) {
@@ -252,6 +256,8 @@ __global__ __launch_bounds__(kBlock) void kPass2(DeviceCtx c) {
         },
         [&](Rec& r) {
             r.mat &= 0xffu;
+            r.v.w = maxPos(r.v.w, 1.0e-9f);
+            r.volJ = r.p.w * fastRcp(r.v.w);
             if constexpr (!STRENGTH) {
                 r.n.x = maxPos(r.n.x, 1.0e-6f);
                 const Support sj = supportOf(2.0f * r.n.x);
@@ -267,7 +273,7 @@ __global__ __launch_bounds__(kBlock) void kPass2(DeviceCtx c) {
             const float Hm = fmaxf(Hi, Hj);
             if (!(r2 < Hm * Hm)) return;
             ++pairs;
-            const float mj = pj.w, rhoJ = maxPos(vj.w, 1.0e-9f);
+            const float mj = pj.w, rhoJ = vj.w;
             const float pr2J = nbj.y, csJ = nbj.z, fBalJ = nbj.w;
             const float dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
             const float vDotR = dvx * dx + dvy * dy + dvz * dz;
@@ -279,7 +285,7 @@ __global__ __launch_bounds__(kBlock) void kPass2(DeviceCtx c) {
             const float gb = 0.5f * (gI + gJ);
             const float gvx = gb * dx, gvy = gb * dy, gvz = gb * dz;
             const float vdg = dvx * gvx + dvy * gvy + dvz * gvz;
-            const float volJ = mj * fastRcp(rhoJ);
+            const float volJ = n.volJ;
             dCont += volJ * vdg;
             if (n.mat == matI) {
                 const float rdg = -(dx * gvx + dy * gvy + dz * gvz);
