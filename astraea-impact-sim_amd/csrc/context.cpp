@@ -97,6 +97,10 @@ struct astraea_ctx {
     int device = 0;
     uint32_t* scratchDev = nullptr;        // this device's address of the file-scope `scratch` table
     hipStream_t stream = nullptr;
+    // gravity runs on `side`, beside the neighbour build and pass 1 on `stream` (it reads only the
+    // tree and the positions; pass 2 waits for it): fork and join events
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     uint32_t capSlots = 0;
     uint32_t maxSeg = 0;
     astraea_params params{};
@@ -461,7 +465,7 @@ void swapSets(astraea_ctx* c) {
 }
 
 // Sort -> permute/Karras -> moments -> neighbour table.  posNew and the bounds must be ready.
-void enqueueTree(astraea_ctx* c, bool timed) {
+void enqueueTree(astraea_ctx* c, bool timed, bool gravitySide = false) {
     hipStream_t s = c->stream;
     if (timed) stageMark(c, ASTRAEA_STAGE_MORTON);
     launchBoundsReduce(c->dc, s);
@@ -473,7 +477,15 @@ void enqueueTree(astraea_ctx* c, bool timed) {
     swapSets(c);
     if (timed) { stageClose(c, ASTRAEA_STAGE_PERMUTE_TREE); stageMark(c, ASTRAEA_STAGE_MOMENTS); }
     launchMoments(c->dc, s);
-    if (timed) { stageClose(c, ASTRAEA_STAGE_MOMENTS); stageMark(c, ASTRAEA_STAGE_NEIGHBOURS); }
+    if (timed) stageClose(c, ASTRAEA_STAGE_MOMENTS);
+    if (gravitySide) {
+        // gravity needs the tree and the positions only: it runs on the side stream from here
+        (void)hipEventRecord(c->fork, s);
+        (void)hipStreamWaitEvent(c->side, c->fork, 0);
+        launchGravity(c->dc, c->side);
+        (void)hipEventRecord(c->join, c->side);
+    }
+    if (timed) stageMark(c, ASTRAEA_STAGE_NEIGHBOURS);
     launchNeighbours(c->dc, s);
     if (timed) stageClose(c, ASTRAEA_STAGE_NEIGHBOURS);
     c->treeValid = c->listValid = true;
@@ -490,11 +502,14 @@ void enqueueStep(astraea_ctx* c, float maxDt, bool timed) {
     if (timed) { stageClose(c, ASTRAEA_STAGE_KICK1); stageMark(c, ASTRAEA_STAGE_DRIFT); }
     launchDrift(c->dc, s);
     if (timed) stageClose(c, ASTRAEA_STAGE_DRIFT);
-    enqueueTree(c, timed);
+    // untimed, gravity overlaps the neighbour build and pass 1 (the two fill each other's idle
+    // CUs at the ends of their launches); timed, the stages run one after another
+    enqueueTree(c, timed, !timed);
     if (timed) stageMark(c, ASTRAEA_STAGE_PASS1);
     launchPass1(c->dc, s);
     if (timed) { stageClose(c, ASTRAEA_STAGE_PASS1); stageMark(c, ASTRAEA_STAGE_GRAVITY); }
-    launchGravity(c->dc, s);
+    if (timed) launchGravity(c->dc, s);
+    else (void)hipStreamWaitEvent(s, c->join, 0);
     if (timed) { stageClose(c, ASTRAEA_STAGE_GRAVITY); stageMark(c, ASTRAEA_STAGE_PASS2); }
     launchPass2(c->dc, s);
     if (timed) { stageClose(c, ASTRAEA_STAGE_PASS2); stageMark(c, ASTRAEA_STAGE_KICK2); }
@@ -689,8 +704,12 @@ int astraea_create(astraea_ctx** out, int device, uint32_t capacity, uint32_t ma
         astraea_destroy(c);
         return r;
     };
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(ASTRAEA_E_HIP, "hipStreamCreate failed"));
+    if (hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess)
+        return cleanup(fail(ASTRAEA_E_HIP, "hipEventCreate failed"));
     {
         const int rs = scratchAddress(device, &c->scratchDev);
         if (rs != ASTRAEA_OK) return cleanup(rs);
@@ -780,6 +799,9 @@ int astraea_destroy(astraea_ctx* c) {
     freeGraph(c);
     for (void* p : c->allocs) (void)hipFree(p);
     for (hipEvent_t e : c->evPool) (void)hipEventDestroy(e);
+    if (c->fork) (void)hipEventDestroy(c->fork);
+    if (c->join) (void)hipEventDestroy(c->join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return ASTRAEA_OK;
