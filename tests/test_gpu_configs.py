@@ -302,3 +302,61 @@ def test_strength_4m_full_size(lib, orc):
     assert np.all(np.isfinite(j2))
     # within the intact yield surface of granite (the largest Y0 of the strong materials)
     assert float(np.sqrt(3.0 * j2).max()) <= 3.5e-3 * (1.0 + 1e-5)
+
+
+# ---------------------------------------------------------------------------------------------
+# a few particles at the h clamp against a 1M body
+# ---------------------------------------------------------------------------------------------
+def _device_free_bytes():
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so.7")
+    free, total = C.c_size_t(), C.c_size_t()
+    assert hip.hipMemGetInfo(C.byref(free), C.byref(total)) == 0
+    return free.value
+
+
+def test_large_smoothing_lengths_against_a_1m_body(lib, orc):
+    """Eight particles of the 1M giant impact set at the upper clamp of the h update (h = 6 h0),
+    inside the target: each reaches thousands of neighbours, and every particle within 12 h0 of
+    one of them has it as a neighbour.  A single table width for all slots would have to hold the
+    widest particle in every one of the 1M slots; the listed wavefronts spill instead.  Initial
+    forces and five steps run without an exception, the table stays within a few GB of device
+    memory, the pass-1 neighbour counts of the large particles equal the oracle's, and every
+    force is finite."""
+    from astraea_amd import scenario as S
+
+    _say("large h against 1M: build")
+    st = S.config_giant()
+    n = len(st["mass"])
+    rng = np.random.default_rng(6)
+    r = np.sqrt(st["pos_x"] ** 2 + st["pos_y"] ** 2 + st["pos_z"] ** 2)
+    inner = np.flatnonzero((st["body"] == 0) & (r < 0.5 * np.max(r[st["body"] == 0])))
+    big = rng.choice(inner, 8, replace=False)
+    st["h"][big] = (6.0 * st["h0"][big]).astype(np.float32)
+    p = S.plain_params()
+    free0 = _device_free_bytes()
+    ctx = lib.Context(n)
+    ctx.set_params(p)
+    ctx.set_layout([n])
+    ctx.load_state(st)
+    ctx.initial_forces()
+    ctx.synchronize()
+    c = ctx.counters()
+    _say("after initial forces:", c)
+    assert c["spilled_waves"] > 0 and c["nbr_max"] > c["nbr_stride"]
+    # the pass-1 neighbour counts of the large particles against the oracle's, at load
+    o = orc.Oracle(p, group_size=p.group_size or lib.GROUP_SIZE)
+    o.set_layout([n])
+    o.load_state(st)
+    o.initial_forces()
+    ng, nr = by_id(ctx, o, "ngb_count")
+    assert np.array_equal(ng[big], nr[big]), (ng[big], nr[big])
+    assert np.min(nr[big]) > 1000
+    ctx.step(MAX_DT, 5)
+    ctx.synchronize()
+    used = free0 - _device_free_bytes()
+    _say(f"after 5 steps: {ctx.counters()}, device memory {used / 2**30:.2f} GiB")
+    assert used < 6 * 2**30
+    for k in ("acc_x", "acc_y", "acc_z", "du_dt", "pos_x", "h"):
+        assert np.all(np.isfinite(ctx.download(k))), k
