@@ -15,8 +15,8 @@
 //
 // Layout (WaveTable): each wavefront writes the union of its particles' neighbour sets (the hit
 // candidates, in the order the walk found them, dealt out to kBanks banks) and, for every
-// particle and bank, the ascending 16-bit positions of its neighbours in that bank, one 16-bit
-// store each (the table's lines fill in L2).  A wavefront first builds into the regular region,
+// particle and bank, the ascending 16-bit positions of its neighbours in that bank, in 16-byte
+// chunks of eight written whole.  A wavefront first builds into the regular region,
 // `stride` entries per slot and `unionCap` union ids.  One that needs more (a particle with many
 // neighbours, as an envelope particle of a gas giant with h ten times its core's) is listed, and a
 // second kernel builds it again into the spill pool with exactly the width it needs, so memory
