@@ -97,11 +97,9 @@ __global__ __launch_bounds__(kBlock) void kHist(const uint64_t* __restrict__ key
     for (int k = 0; k < kIPT; ++k) {
         const uint32_t idx = base + k * kWave + lane;
         const bool v = idx < end;
-        uint32_t d = 0;
-        if (v) d = (uint32_t)(keys[idx] >> shift) & (uint32_t)(DIG - 1);
-        const uint64_t m = matchDigit<BITS>(d, ballot(v));
-        const uint64_t lt = (1ull << lane) - 1ull;
-        if (v && (m & lt) == 0ull) h[wave][d] += (uint32_t)__popcll(m);
+        // counts only (no ranks): an LDS atomic per element, where matching the digit across the
+        // wavefront would take BITS ballots
+        if (v) atomicAdd(&h[wave][(uint32_t)(keys[idx] >> shift) & (uint32_t)(DIG - 1)], 1u);
     }
     __syncthreads();
     for (int d = tid; d < DIG; d += kBlock) {
