@@ -91,7 +91,7 @@ __global__ __launch_bounds__(kBlock) void kHist(const uint64_t* __restrict__ key
     const int tid = threadIdx.x, wave = tid >> 6, lane = laneId();
     for (int k = tid; k < kWavesPerBlock * DIG; k += kBlock) (&h[0][0])[k] = 0u;
     __syncthreads();
-    const uint32_t tile = blockIdx.x;
+    const uint32_t tile = xcdBlock(blockIdx.x, gridDim.x);  // the scatter's tile order
     const uint32_t end = tileStart[tile + 1];
     const uint32_t base = tileStart[tile] + wave * kWave * kIPT;
     for (int k = 0; k < kIPT; ++k) {
@@ -153,7 +153,9 @@ __global__ __launch_bounds__(kBlock) void kScatter(const uint64_t* __restrict__ 
     __shared__ uint32_t sGlobal[DIG];
     __shared__ uint32_t scratch[kWavesPerBlock];
     const int tid = threadIdx.x, wave = tid >> 6, lane = laneId();
-    const uint32_t tile = blockIdx.x;
+    // consecutive tiles on one XCD: tile t's run of a digit ends where tile t + 1's begins, and the
+    // cache line they share is completed in one L2 instead of reaching HBM as two partial writes
+    const uint32_t tile = xcdBlock(blockIdx.x, gridDim.x);
     const uint32_t seg = tileSeg[tile];
     const uint32_t t0 = tileStart[tile], end = tileStart[tile + 1];
     for (int k = tid; k < kWavesPerBlock * DIG; k += kBlock) (&sWave[0][0])[k] = 0u;
