@@ -29,6 +29,7 @@ _lib = C.CDLL(str(LIB_PATH))
 
 # ---- constants mirrored from include/astraea_hip.h ----
 OK = 0
+E_INVALID = -1
 PREC_FP32 = 0
 PREC_FP64_ACCUM = 1
 GROUP_SIZE = 64
@@ -445,7 +446,8 @@ class Context:
         _check(_lib.astraea_set_tuning(self._h, C.byref(t)))
 
     def autotune(self, max_dt: float, steps_per_variant: int = 1) -> dict:
-        """Runs 8 * steps_per_variant steps timing every launch variant of the passes, the drift
+        """Runs 4 + 8 * steps_per_variant steps (one untimed step per variant, then the timed ones)
+        timing every launch variant of the passes, the drift
         and the sort, then times the gravity and neighbour-build variants on the settled state
         (astraea_autotune), and keeps the fastest; returns the choice and the measured mean
         milliseconds (-1 for a walk variant whose stack filled)."""
@@ -455,7 +457,7 @@ class Context:
         chosen = {k: int(getattr(t, k)) for k, _ in Tuning._fields_}
         walk = [f"{w}x{s}" for w, s in WALK_VARIANTS]
         return dict(chosen=chosen, pass1_ms={128: ms[0], 256: ms[1]}, pass2_ms={64: ms[2], 128: ms[3]},
-                    drift_ms={64: ms[4], 128: ms[5], 192: ms[6], 256: ms[7]}, sort_ms={8: ms[8], 9: ms[9]},
+                    drift_ms={128: ms[4], 160: ms[5], 192: ms[6], 256: ms[7]}, sort_ms={8: ms[8], 9: ms[9]},
                     gravity_ms={v: ms[10 + k] for k, v in enumerate(walk)},
                     build_ms={v: ms[14 + k] for k, v in enumerate(walk)})
 
@@ -469,17 +471,22 @@ def tuning_key(particles: int, segments: int, strength: bool, fp64: bool) -> str
 
 def tune_cached(ctx: "Context", max_dt: float, path, key: str, steps_per_variant: int = 1) -> dict:
     """Applies the tuning stored under `key` in the JSON table at `path`, or runs the autotuner
-    (8 * steps_per_variant steps of the loaded state) and stores its choice there."""
+    (4 + 8 * steps_per_variant steps of the loaded state) and stores its choice there.  A stored
+    choice this library no longer offers (a launch variant since replaced) is tuned again."""
     import json
 
     p = Path(path)
     table = json.loads(p.read_text()) if p.exists() else {}
     if key in table:
         t = table[key]
-        ctx.set_tuning(t["pass1_round"], t["pass2_round"], t["drift_round"], t.get("sort_bits", 8),
-                       t.get("gravity_waves", 4), t.get("gravity_stack", 1024), t.get("build_waves", 4),
-                       t.get("build_stack", 1024))
-        return dict(chosen=dict(t), cached=True)
+        try:
+            ctx.set_tuning(t["pass1_round"], t["pass2_round"], t["drift_round"], t.get("sort_bits", 8),
+                           t.get("gravity_waves", 4), t.get("gravity_stack", 1024), t.get("build_waves", 4),
+                           t.get("build_stack", 1024))
+            return dict(chosen=dict(t), cached=True)
+        except AstraeaError as e:
+            if e.code != E_INVALID:
+                raise
     r = ctx.autotune(max_dt, steps_per_variant)
     table[key] = r["chosen"]
     p.parent.mkdir(parents=True, exist_ok=True)

@@ -1535,7 +1535,7 @@ static bool validWalk(uint32_t waves, uint32_t stack) {
 
 static bool validTuning(const astraea_tuning& t) {
     return (t.pass1_round == 128u || t.pass1_round == 256u) && (t.pass2_round == 64u || t.pass2_round == 128u) &&
-           (t.drift_round == 64u || t.drift_round == 128u || t.drift_round == 192u || t.drift_round == 256u) &&
+           (t.drift_round == 128u || t.drift_round == 160u || t.drift_round == 192u || t.drift_round == 256u) &&
            (t.sort_bits == 8u || t.sort_bits == 9u) && validWalk(t.gravity_waves, t.gravity_stack) &&
            validWalk(t.build_waves, t.build_stack);
 }
@@ -1558,7 +1558,7 @@ int astraea_set_tuning(astraea_ctx* c, const astraea_tuning* t) {
     if (!validTuning(*t))
         return fail(ASTRAEA_E_INVALID,
                     "tuning (%u, %u, %u, %u, gravity %u x %u, build %u x %u) outside pass 1 {128, 256}, pass 2 "
-                    "{64, 128}, drift {64, 128, 192, 256}, sort bits {8, 9}, (waves, stack) {(2, 1024), (4, 1024), "
+                    "{64, 128}, drift {128, 160, 192, 256}, sort bits {8, 9}, (waves, stack) {(2, 1024), (4, 1024), "
                     "(4, 512), (8, 512)}",
                     t->pass1_round, t->pass2_round, t->drift_round, t->sort_bits, t->gravity_waves, t->gravity_stack,
                     t->build_waves, t->build_stack);
@@ -1609,6 +1609,13 @@ static int tuneWalks(astraea_ctx* c, double gms[4], double bms[4], int* gBest, i
     HIP_TRY(hipMemcpy(saved, c->dc.cnt.v, sizeof(saved), hipMemcpyDeviceToHost));
     const Tuning keep = c->dc.tune;
     int rc = ASTRAEA_OK;
+    // every variant once, untimed, so that no timed run carries a kernel's first launch
+    for (int k = 0; k < 4 && rc == ASTRAEA_OK; ++k) {
+        c->dc.tune.gravityWaves = kWalkWaves[k];
+        c->dc.tune.gravityStack = kWalkStack[k];
+        double t = 0.0;
+        rc = timeLaunch(c, [&] { launchGravity(c->dc, c->stream); }, &t);
+    }
     for (int r = 0; r < 8 && rc == ASTRAEA_OK; ++r) {
         const int k = order4[r];
         c->dc.tune.gravityWaves = kWalkWaves[k];
@@ -1621,6 +1628,19 @@ static int tuneWalks(astraea_ctx* c, double gms[4], double bms[4], int* gBest, i
         if (rc != ASTRAEA_OK) break;
         if (v[1] != saved[1]) gOk[k] = false;
         gms[k] += 0.5 * t;
+    }
+    for (int k = 0; k < 4 && rc == ASTRAEA_OK; ++k) {
+        c->dc.tune.buildWaves = kWalkWaves[k];
+        c->dc.tune.buildStack = kWalkStack[k];
+        double t = 0.0;
+        rc = timeLaunch(c, [&] { launchNeighbours(c->dc, c->stream); }, &t);
+        unsigned long long v[8];
+        if (rc == ASTRAEA_OK && hipMemcpy(v, c->dc.cnt.v, sizeof(v), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(ASTRAEA_E_HIP, "hipMemcpy failed");
+        if (rc == ASTRAEA_OK && v[kCntHold] != 0ull) {
+            HIP_TRY(hipMemset(c->dc.cnt.v + kCntHold, 0, 2 * sizeof(unsigned long long)));
+            HIP_TRY(hipMemset(c->dc.cnt.v + 5, 0, 3 * sizeof(unsigned long long)));
+        }
     }
     for (int r = 0; r < 8 && rc == ASTRAEA_OK; ++r) {
         const int k = order4[r];
@@ -1668,7 +1688,7 @@ static int tuneWalks(astraea_ctx* c, double gms[4], double bms[4], int* gBest, i
 int astraea_autotune(astraea_ctx* c, float maxDt, uint32_t stepsPerVariant, astraea_tuning* chosen, double* ms) {
     if (!c) return fail(ASTRAEA_E_INVALID, "null context");
     if (stepsPerVariant == 0) return fail(ASTRAEA_E_INVALID, "steps per variant must be > 0");
-    static constexpr uint32_t p1[2] = {128u, 256u}, p2[2] = {64u, 128u}, dr[4] = {64u, 128u, 192u, 256u},
+    static constexpr uint32_t p1[2] = {128u, 256u}, p2[2] = {64u, 128u}, dr[4] = {128u, 160u, 192u, 256u},
                               sb[2] = {8u, 9u};
     // eight runs in palindromic orders: every variant's runs sit on average at the middle of the
     // sequence, so a steady drift of the step's cost as the state evolves cancels out
@@ -1677,7 +1697,20 @@ int astraea_autotune(astraea_ctx* c, float maxDt, uint32_t stepsPerVariant, astr
     astraea_tuning before;
     TRY(astraea_get_tuning(c, &before));
     double t1[2] = {0.0, 0.0}, t2[2] = {0.0, 0.0}, td[4] = {0.0, 0.0, 0.0, 0.0}, ts[2] = {0.0, 0.0};
-    int rc = astraea_enable_stage_timing(c, 1);
+    // one untimed step with each variant first, so that no timed step carries a kernel's first
+    // launch or the timing's set-up
+    int rc = ASTRAEA_OK;
+    for (int r = 0; r < 4 && rc == ASTRAEA_OK; ++r) {
+        astraea_tuning t = before;
+        t.pass1_round = p1[r % 2];
+        t.pass2_round = p2[r % 2];
+        t.drift_round = dr[r];
+        t.sort_bits = sb[r % 2];
+        rc = astraea_set_tuning(c, &t);
+        if (rc == ASTRAEA_OK) rc = astraea_step(c, maxDt, 1, 0);
+        if (rc == ASTRAEA_OK) rc = settle(c);
+    }
+    if (rc == ASTRAEA_OK) rc = astraea_enable_stage_timing(c, 1);
     for (int r = 0; r < 8 && rc == ASTRAEA_OK; ++r) {
         astraea_tuning t = before;
         t.pass1_round = p1[order2[r]];

@@ -175,11 +175,11 @@ __device__ __forceinline__ GlobalPtr<T> globalView(const T* p) {
 template <int S, class Rec, class Load, class Finish, class Pair>
 __device__ __forceinline__ void forEachNeighbour(const Neighbours& nb, uint32_t i, Rec* buf, Load&& load,
                                                  Finish&& finish, Pair&& pair) {
-    static_assert(S % 64 == 0, "a round is whole wavefront loads");
+    static_assert(S % 32 == 0, "a round is whole half-wavefront loads");
     static_assert(sizeof(Rec) % 16 == 0 && alignof(Rec) >= 16, "records are whole 16-byte quads");
     constexpr int NQ = (int)(sizeof(Rec) / 16);
     float4* quads = reinterpret_cast<float4*>(buf);
-    constexpr int R = S / 64;
+    constexpr int R = (S + 63) / 64;  // wavefront loads per round; the last may fill half the lanes
     constexpr uint32_t kPast = 0xffffffffu;  // an entry position beyond every round
     const uint32_t w = i >> 6, lane = i & 63u;
     const WaveTable t = nb.tab[w];
@@ -232,7 +232,8 @@ __device__ __forceinline__ void forEachNeighbour(const Neighbours& nb, uint32_t 
                 float4 q[NQ];
                 __builtin_memcpy(q, &rec, sizeof(Rec));
 #pragma unroll
-                for (int x = 0; x < NQ; ++x) quads[x * S + 64 * r + lane] = q[x];
+                for (int x = 0; x < NQ; ++x)
+                    if (64 * r + (int)lane < S) quads[x * S + 64 * r + lane] = q[x];
             }
             waveSync();
             // two records alternate (a holds entry k when the loop body starts), so that a record

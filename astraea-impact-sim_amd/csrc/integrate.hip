@@ -428,7 +428,7 @@ void driftAs(const DeviceCtx& c, hipStream_t s) {
 
 void launchDrift(const DeviceCtx& c, hipStream_t s) {
     switch (c.tune.driftRound) {
-        case 64u: driftAs<64>(c, s); break;
+        case 160u: driftAs<160>(c, s); break;
         case 192u: driftAs<192>(c, s); break;
         case 256u: driftAs<256>(c, s); break;
         default: driftAs<128>(c, s); break;

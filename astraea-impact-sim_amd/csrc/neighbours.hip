@@ -189,7 +189,10 @@ __device__ __forceinline__ WaveBuild buildWave(const DeviceCtx& c, WaveLds<SCAP>
             const int selfAt = L.selfAt[lane];  // a leaf is listed once per walk
             for (int c0 = 0; c0 < nl; c0 += 32) {
                 uint32_t m = 0u, any = 0u;
-#pragma unroll
+                // eight tests per unrolled body: fully unrolled, the 32 compare masks held at once
+                // pushed the sub-group boxes out of the scalar registers (50 spilled, reloaded in
+                // every walk round)
+#pragma unroll 8
                 for (int u = 0; u < 32; ++u) {
                     const float4 q = L.cand[c0 + u];
                     const float r2 = dist2f(pi.x - q.x, pi.y - q.y, pi.z - q.z);

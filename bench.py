@@ -161,7 +161,7 @@ def run_batched(A, state, counts, steps, warmup, autotune=True):
     tune = None
     if autotune:
         # untimed like the warm-up: four eager steps, one per launch variant of each pass
-        log("[bench] batched: autotune (8 steps)")
+        log("[bench] batched: autotune (12 steps)")
         tune = ctx.autotune(MAX_DT, steps_per_variant=1)
         log(f"[bench] batched: tuning {tune['chosen']}")
     ctx.step(MAX_DT, warmup, graph=True)
@@ -470,7 +470,7 @@ def main():
                     "frac": n * ALG_BYTES[k] / (stages[k][0] / max(stages[k][1], 1) * 1e-3) / 1e9 / PEAK_HBM_GBS,
                     "bytes_per_particle": ALG_BYTES[k]} for k in ALG_BYTES},
         "gravity_interactions_per_s": inter_per_launch / (grav_avg_ms * 1e-3),
-        "tuning": b["tune"] and {"chosen": b["tune"]["chosen"], "steps": 8,
+        "tuning": b["tune"] and {"chosen": b["tune"]["chosen"], "steps": 12,
                                  "ms_per_step": {k: b["tune"][k] for k in ("pass1_ms", "pass2_ms", "drift_ms", "sort_ms")},
                                  "ms_per_launch": {k: b["tune"][k] for k in ("gravity_ms", "build_ms")}},
         "step_roofline": step_roofline(stages, n, n1, n2, inter_pp, 1e3 * b["wall"] / K),

@@ -367,7 +367,7 @@ int astraea_stage_times(astraea_ctx* ctx, double* out_ms, uint64_t* out_samples)
 typedef struct astraea_tuning {
     uint32_t pass1_round;   /* 128 or 256 (default 256) */
     uint32_t pass2_round;   /* 64 or 128 (default 128) */
-    uint32_t drift_round;   /* 64, 128, 192 or 256 (default 128) */
+    uint32_t drift_round;   /* 128, 160, 192 or 256 (default 128) */
     uint32_t sort_bits;     /* 8 (8 passes) or 9 (7 passes) (default 8) */
     uint32_t gravity_waves; /* wavefronts per block of the gravity walk (default 4) */
     uint32_t gravity_stack; /* its per-wave stack entries (default 1024) */
@@ -376,15 +376,16 @@ typedef struct astraea_tuning {
 } astraea_tuning;
 int astraea_get_tuning(astraea_ctx* ctx, astraea_tuning* out);
 int astraea_set_tuning(astraea_ctx* ctx, const astraea_tuning* tuning);
-/* The autotuner: advances the loaded state by 8 * steps_per_variant eager steps, timing every
- * variant of each stage with the stage events (in palindromic orders, so that the state's slow
- * change of cost cancels), and keeps the fastest for each.  The simulation does not depend
+/* The autotuner: advances the loaded state by 4 + 8 * steps_per_variant eager steps (one untimed
+ * step with each variant first, then the timed ones), timing every variant of each stage with the
+ * stage events (in palindromic orders, so that the state's slow change of cost cancels), and keeps
+ * the fastest for each.  The simulation does not depend
  * on the variant, so these are ordinary steps.  Then it launches the gravity walk and the
- * neighbour build of the resulting state again in each (waves, stack) variant, twice each in
- * palindromic order, timed with events; those launches rewrite what the step computed, bit for
+ * neighbour build of the resulting state again in each (waves, stack) variant, once untimed and
+ * twice timed in palindromic order, timed with events; those launches rewrite what the step computed, bit for
  * bit, and a variant whose stack filled (a forced acceptance, an incomplete table) is not chosen
  * (the table is then built again with the chosen variant).  `chosen` and `ms` (mean milliseconds:
- * pass 1 at 128 and 256, pass 2 at 64 and 128, drift at 64, 128, 192 and 256, the sort at 8 and 9
+ * pass 1 at 128 and 256, pass 2 at 64 and 128, drift at 128, 160, 192 and 256, the sort at 8 and 9
  * bits, then gravity and the build at (2, 1024), (4, 1024), (4, 512) and (8, 512); 18 values, a
  * variant whose stack filled reads -1) may be NULL.  On an error the previous tuning is
  * restored. */

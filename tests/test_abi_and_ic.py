@@ -294,3 +294,16 @@ def test_tuning_table_round_trip(lib, tmp_path):
     import json
 
     assert set(json.loads(path.read_text())) == {key, "gfx950/n20/s1/plain/fp32"}
+
+    class StaleContext(FakeContext):
+        def set_tuning(self, p1, p2, d, *rest):
+            if d == 64:  # a drift round this library no longer offers
+                raise lib.AstraeaError(lib.E_INVALID, "invalid tuning")
+            super().set_tuning(p1, p2, d, *rest)
+
+    table = json.loads(path.read_text())
+    table[key]["drift_round"] = 64
+    path.write_text(json.dumps(table))
+    stale = StaleContext()
+    third = lib.tune_cached(stale, 5e-3, path, key)
+    assert not third["cached"] and stale.tuned == 1 and json.loads(path.read_text())[key]["drift_round"] == 192
