@@ -35,12 +35,13 @@ namespace {
 
 constexpr int kSub = 4;                     // sub-groups of a wavefront with boxes of their own
 constexpr int kSubLanes = kWave / kSub;
-// Evaluate once either list holds more than kFlushAt records.  Sixteen keeps a block of four
-// wavefronts with the 512-entry stack at 30 KB of LDS, five blocks per CU (at 32 the block took
-// 32,256 bytes and ran at four blocks per CU, 9.1 against 8.4 ms on the 64-scenario sweep).
+// The staged records are evaluated once either list holds more than kFlushAt of them, checked
+// after every pass of a round, so a list never exceeds kFlushAt + 64 (one pass lists at most a
+// record per lane).  The undecided nodes of a pass are staged past the kFlushAt records a list
+// can hold when the pass starts.  The LDS a wavefront holds sets how many share a CU.
 constexpr int kFlushAt = 16;
-constexpr int kRCap = kFlushAt + 2 * kWave; // one round lists at most 128 records
-constexpr int kPend = kRCap - kWave;        // undecided nodes of one side, staged past the records
+constexpr int kRCap = kFlushAt + kWave;
+constexpr int kPend = kFlushAt;
 
 template <int SCAP>  // per-wave node stack: 1024 or 512 entries (a launch variant)
 struct WaveLds {
@@ -165,6 +166,42 @@ __global__ __launch_bounds__(W * kWave) void kGravity(DeviceCtx c) {
     const int e0 = GROUP == kWave ? 0 : lane / GROUP;
 
     int sp = 1, nn = 0, nf = 0;
+    // evaluates the staged records and empties both lists
+    const auto evaluate = [&]() {
+        __builtin_amdgcn_wave_barrier();
+        listed += (uint32_t)(nn + nf);
+#pragma unroll 4
+        for (int e = e0; e < nn; e += kStride) {
+            const float4 q = L.node[e];
+            const float dx = pi.x - q.x, dy = pi.y - q.y, dz = pi.z - q.z;
+            const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, e2Node)));
+            const float ir = __builtin_amdgcn_rsqf(d2);  // d2 >= 1e-20: never denormal
+            const float t = q.w * ir;
+            const float f = t * (ir * ir);
+            ax = __builtin_fmaf(-f, dx, ax);
+            ay = __builtin_fmaf(-f, dy, ay);
+            az = __builtin_fmaf(-f, dz, az);
+            phi -= t;
+        }
+#pragma unroll 4
+        for (int e = e0; e < nf; e += kStride) {
+            const float4 q = L.leaf[e];
+            const float eij = halfEpsI + L.leafEps[e];
+            const float dx = pi.x - q.x, dy = pi.y - q.y, dz = pi.z - q.z;
+            const float e2 = __builtin_fmaf(eij, eij, 1.0e-20f);
+            const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, e2)));
+            const float ir = __builtin_amdgcn_rsqf(d2);
+            const float t = q.w * ir;
+            const float f = t * (ir * ir);
+            ax = __builtin_fmaf(-f, dx, ax);
+            ay = __builtin_fmaf(-f, dy, ay);
+            az = __builtin_fmaf(-f, dz, az);
+            phi -= t;
+        }
+        nn = 0;
+        nf = 0;
+        __builtin_amdgcn_wave_barrier();
+    };
     L.stack[0] = (int)off;
     __builtin_amdgcn_wave_barrier();
     for (;;) {
@@ -351,42 +388,11 @@ __global__ __launch_bounds__(W * kWave) void kGravity(DeviceCtx c) {
                 nf += __popcll(mLeaf);
                 sp = sp + nPush < SCAP ? sp + nPush : SCAP;
                 forced += (uint32_t)__popcll(ballot(forcedHere));
+                if (side + 1 < passes && (nn > kFlushAt || nf > kFlushAt)) evaluate();
             }
             __builtin_amdgcn_wave_barrier();
         }
-        // ---- evaluate the staged records ----
-        listed += (uint32_t)(nn + nf);
-#pragma unroll 4
-        for (int e = e0; e < nn; e += kStride) {
-            const float4 q = L.node[e];
-            const float dx = pi.x - q.x, dy = pi.y - q.y, dz = pi.z - q.z;
-            const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, e2Node)));
-            const float ir = __builtin_amdgcn_rsqf(d2);  // d2 >= 1e-20: never denormal
-            const float t = q.w * ir;
-            const float f = t * (ir * ir);
-            ax = __builtin_fmaf(-f, dx, ax);
-            ay = __builtin_fmaf(-f, dy, ay);
-            az = __builtin_fmaf(-f, dz, az);
-            phi -= t;
-        }
-#pragma unroll 4
-        for (int e = e0; e < nf; e += kStride) {
-            const float4 q = L.leaf[e];
-            const float eij = halfEpsI + L.leafEps[e];
-            const float dx = pi.x - q.x, dy = pi.y - q.y, dz = pi.z - q.z;
-            const float e2 = __builtin_fmaf(eij, eij, 1.0e-20f);
-            const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, e2)));
-            const float ir = __builtin_amdgcn_rsqf(d2);
-            const float t = q.w * ir;
-            const float f = t * (ir * ir);
-            ax = __builtin_fmaf(-f, dx, ax);
-            ay = __builtin_fmaf(-f, dy, ay);
-            az = __builtin_fmaf(-f, dz, az);
-            phi -= t;
-        }
-        nn = 0;
-        nf = 0;
-        __builtin_amdgcn_wave_barrier();
+        evaluate();
         if (sp == 0) break;
     }
     // the particle's own leaf: zero separation, so only its potential term is taken back
