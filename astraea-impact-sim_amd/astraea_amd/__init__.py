@@ -456,7 +456,7 @@ class Context:
         _check(_lib.astraea_autotune(self._h, float(max_dt), int(steps_per_variant), C.byref(t), ms))
         chosen = {k: int(getattr(t, k)) for k, _ in Tuning._fields_}
         walk = [f"{w}x{s}" for w, s in WALK_VARIANTS]
-        return dict(chosen=chosen, pass1_ms={128: ms[0], 256: ms[1]}, pass2_ms={64: ms[2], 128: ms[3]},
+        return dict(chosen=chosen, pass1_ms={128: ms[0], 256: ms[1]}, pass2_ms={128: ms[2], 160: ms[3]},
                     drift_ms={128: ms[4], 160: ms[5], 192: ms[6], 256: ms[7]}, sort_ms={8: ms[8], 9: ms[9]},
                     gravity_ms={v: ms[10 + k] for k, v in enumerate(walk)},
                     build_ms={v: ms[14 + k] for k, v in enumerate(walk)})

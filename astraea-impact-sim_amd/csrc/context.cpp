@@ -1534,7 +1534,7 @@ static bool validWalk(uint32_t waves, uint32_t stack) {
 }
 
 static bool validTuning(const astraea_tuning& t) {
-    return (t.pass1_round == 128u || t.pass1_round == 256u) && (t.pass2_round == 64u || t.pass2_round == 128u) &&
+    return (t.pass1_round == 128u || t.pass1_round == 256u) && (t.pass2_round == 128u || t.pass2_round == 160u) &&
            (t.drift_round == 128u || t.drift_round == 160u || t.drift_round == 192u || t.drift_round == 256u) &&
            (t.sort_bits == 8u || t.sort_bits == 9u) && validWalk(t.gravity_waves, t.gravity_stack) &&
            validWalk(t.build_waves, t.build_stack);
@@ -1558,7 +1558,7 @@ int astraea_set_tuning(astraea_ctx* c, const astraea_tuning* t) {
     if (!validTuning(*t))
         return fail(ASTRAEA_E_INVALID,
                     "tuning (%u, %u, %u, %u, gravity %u x %u, build %u x %u) outside pass 1 {128, 256}, pass 2 "
-                    "{64, 128}, drift {128, 160, 192, 256}, sort bits {8, 9}, (waves, stack) {(2, 1024), (4, 1024), "
+                    "{128, 160}, drift {128, 160, 192, 256}, sort bits {8, 9}, (waves, stack) {(2, 1024), (4, 1024), "
                     "(4, 512), (8, 512)}",
                     t->pass1_round, t->pass2_round, t->drift_round, t->sort_bits, t->gravity_waves, t->gravity_stack,
                     t->build_waves, t->build_stack);
@@ -1688,7 +1688,7 @@ static int tuneWalks(astraea_ctx* c, double gms[4], double bms[4], int* gBest, i
 int astraea_autotune(astraea_ctx* c, float maxDt, uint32_t stepsPerVariant, astraea_tuning* chosen, double* ms) {
     if (!c) return fail(ASTRAEA_E_INVALID, "null context");
     if (stepsPerVariant == 0) return fail(ASTRAEA_E_INVALID, "steps per variant must be > 0");
-    static constexpr uint32_t p1[2] = {128u, 256u}, p2[2] = {64u, 128u}, dr[4] = {128u, 160u, 192u, 256u},
+    static constexpr uint32_t p1[2] = {128u, 256u}, p2[2] = {128u, 160u}, dr[4] = {128u, 160u, 192u, 256u},
                               sb[2] = {8u, 9u};
     // eight runs in palindromic orders: every variant's runs sit on average at the middle of the
     // sequence, so a steady drift of the step's cost as the state evolves cancels out

@@ -417,7 +417,7 @@ void launchEos(const DeviceCtx& c, hipStream_t s) {
 }
 
 void launchPass2(const DeviceCtx& c, hipStream_t s) {
-    if (c.tune.pass2Round == 64u) pass2As<64>(c, s);
+    if (c.tune.pass2Round == 160u) pass2As<160>(c, s);
     else pass2As<128>(c, s);
 }
 

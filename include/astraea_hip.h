@@ -366,7 +366,7 @@ int astraea_stage_times(astraea_ctx* ctx, double* out_ms, uint64_t* out_samples)
  * are (2, 1024), (4, 1024), (4, 512) and (8, 512). */
 typedef struct astraea_tuning {
     uint32_t pass1_round;   /* 128 or 256 (default 256) */
-    uint32_t pass2_round;   /* 64 or 128 (default 128) */
+    uint32_t pass2_round;   /* 128 or 160 (default 128) */
     uint32_t drift_round;   /* 128, 160, 192 or 256 (default 128) */
     uint32_t sort_bits;     /* 8 (8 passes) or 9 (7 passes) (default 8) */
     uint32_t gravity_waves; /* wavefronts per block of the gravity walk (default 4) */
@@ -385,7 +385,7 @@ int astraea_set_tuning(astraea_ctx* ctx, const astraea_tuning* tuning);
  * twice timed in palindromic order, timed with events; those launches rewrite what the step computed, bit for
  * bit, and a variant whose stack filled (a forced acceptance, an incomplete table) is not chosen
  * (the table is then built again with the chosen variant).  `chosen` and `ms` (mean milliseconds:
- * pass 1 at 128 and 256, pass 2 at 64 and 128, drift at 128, 160, 192 and 256, the sort at 8 and 9
+ * pass 1 at 128 and 256, pass 2 at 128 and 160, drift at 128, 160, 192 and 256, the sort at 8 and 9
  * bits, then gravity and the build at (2, 1024), (4, 1024), (4, 512) and (8, 512); 18 values, a
  * variant whose stack filled reads -1) may be NULL.  On an error the previous tuning is
  * restored. */

@@ -277,7 +277,7 @@ def test_tuning_table_round_trip(lib, tmp_path):
 
         def autotune(self, max_dt, steps_per_variant):
             self.tuned += 1
-            return dict(chosen={"pass1_round": 128, "pass2_round": 64, "drift_round": 192, "sort_bits": 9,
+            return dict(chosen={"pass1_round": 128, "pass2_round": 160, "drift_round": 192, "sort_bits": 9,
                                 "gravity_waves": 2, "gravity_stack": 1024, "build_waves": 8, "build_stack": 512})
 
     path = tmp_path / "tuning.json"
@@ -288,7 +288,7 @@ def test_tuning_table_round_trip(lib, tmp_path):
     assert not first["cached"] and ctx.tuned == 1
     again = FakeContext()
     second = lib.tune_cached(again, 5e-3, path, key)
-    assert second["cached"] and again.tuned == 0 and again.applied == (128, 64, 192, 9, 2, 1024, 8, 512)
+    assert second["cached"] and again.tuned == 0 and again.applied == (128, 160, 192, 9, 2, 1024, 8, 512)
     other = FakeContext()
     assert not lib.tune_cached(other, 5e-3, path, lib.tuning_key(1_000_000, 1, False, False))["cached"]
     import json

@@ -41,8 +41,8 @@ def _state(ctx):
 def test_variants_are_bit_identical(lib, strength):
     st = perturbed_small_state(strength=strength)
     want = None
-    for t in [(256, 128, 128, 8, 4, 1024, 4, 1024), (128, 64, 160, 9, 2, 1024, 8, 512),
-              (256, 64, 192, 9, 8, 512, 2, 1024), (128, 128, 256, 8, 4, 512, 4, 512)]:
+    for t in [(256, 128, 128, 8, 4, 1024, 4, 1024), (128, 160, 160, 9, 2, 1024, 8, 512),
+              (256, 160, 192, 9, 8, 512, 2, 1024), (128, 128, 256, 8, 4, 512, 4, 512)]:
         ctx = _context(lib, st, strength)
         ctx.set_tuning(*t)
         assert tuple(ctx.tuning().values()) == t
@@ -59,7 +59,7 @@ def test_autotune_steps_are_ordinary_steps(lib):
     st = perturbed_small_state()
     tuned = _context(lib, st, False)
     r = tuned.autotune(MAX_DT, steps_per_variant=1)
-    assert r["chosen"]["pass1_round"] in (128, 256) and r["chosen"]["pass2_round"] in (64, 128)
+    assert r["chosen"]["pass1_round"] in (128, 256) and r["chosen"]["pass2_round"] in (128, 160)
     assert r["chosen"]["drift_round"] in (128, 160, 192, 256) and r["chosen"]["sort_bits"] in (8, 9)
     assert (r["chosen"]["gravity_waves"], r["chosen"]["gravity_stack"]) in lib.WALK_VARIANTS
     assert (r["chosen"]["build_waves"], r["chosen"]["build_stack"]) in lib.WALK_VARIANTS
