@@ -216,6 +216,35 @@ def run_single_plain(A, steps, warmup, n_target, n_impactor):
     return n * steps / time_steps(ctx, steps)
 
 
+def run_config(A, state, counts, params, steps, warmup):
+    """Throughput of another benchmark configuration in its own context: autotuned like the
+    headline run, then `warmup` untimed and `steps` timed graph steps, then the same number of
+    eager steps with stage events."""
+    n = int(sum(counts))
+    ctx = A.Context(capacity=n, max_segments=len(counts))
+    try:
+        ctx.set_params(params)
+        ctx.set_layout(counts)
+        ctx.load_state(state)
+        ctx.initial_forces()
+        tune = ctx.autotune(MAX_DT, steps_per_variant=1)
+        ctx.step(MAX_DT, warmup, graph=True)
+        ctx.counters()
+        wall = time_steps(ctx, steps)
+        cnt = ctx.counters()
+        ctx.enable_stage_timing(True)
+        ctx.step(MAX_DT, steps, graph=False)
+        ctx.synchronize()
+        stages = ctx.stage_times()
+        return {"particles": n, "scenarios": len(counts), "particle_steps_per_s": n * steps / wall,
+                "ms_per_step": 1e3 * wall / steps, "steps": steps, "warmup": warmup,
+                "gravity_forced_accepts": int(cnt["forced_accepts"]),
+                "stage_ms": {k: v[0] / max(v[1], 1) for k, v in stages.items()},
+                "tuning": tune["chosen"]}
+    finally:
+        ctx.close()
+
+
 def host_info():
     import os
     import oracle
@@ -399,6 +428,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP32 comparison runs")
     ap.add_argument("--drift-steps", type=int, default=1000,
                     help="steps of the energy-drift comparison with the FP64 oracle (0: skip)")
+    ap.add_argument("--no-other-configs", action="store_true",
+                    help="skip the strength (4M) and gas-giant (16 x 500k) throughput runs")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch variants instead of timing them before the warm-up")
     args = ap.parse_args()
@@ -499,6 +530,21 @@ def main():
         }
         out["speedup_vs_sequential_plain"] = value / seq
         out["speedup_vs_giant_impact_plain"] = value / single
+    if not args.no_other_configs:
+        # the other GPU configurations at their own sizes, reported beside the headline (not `value`)
+        log("[bench] precision form: 4,000,000 particles with strength and damage")
+        st4, p4 = S.config_strength()
+        other = {"strength_4m": run_config(A, st4, [len(st4["mass"])], p4, K, W)}
+        del st4
+        other["strength_4m"]["workload"] = ("4,000,000-particle impact (Fe/granite/ice, strength and damage on for "
+                                            "granite and ice), FP32 with FP64 leapfrog accumulators")
+        log("[bench] gas giants: 16 scenarios of 500,000 particles")
+        stg, cg = S.build_gas_giant_sweep(S.gas_giant_sweep(16, 500_000))
+        other["gas_giants_16x500k"] = run_config(A, stg, cg, S.gas_giant_params(), K, W)
+        del stg
+        other["gas_giants_16x500k"]["workload"] = ("16 gas-giant impacts of 500,000 particles (H/He envelope, "
+                                                   "cooling above 2,000 K), one batched step, FP64 accumulators")
+        out["other_configs"] = other
     if args.drift_steps > 0:
         log(f"[bench] energy drift vs the FP64 oracle, {args.drift_steps} steps")
         out["energy_drift_vs_fp64_oracle"] = drift_vs_oracle(
