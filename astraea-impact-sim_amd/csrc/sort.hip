@@ -146,9 +146,16 @@ __global__ __launch_bounds__(kBlock) void kScatter(const uint64_t* __restrict__ 
                                                    const uint32_t* __restrict__ segOff, const uint32_t* __restrict__ prefix,
                                                    const uint32_t* __restrict__ totals, uint32_t tilesCap, int shift) {
     constexpr int DIG = 1 << BITS;
-    __shared__ uint64_t sKey[kTile];
+    // the per-wave digit counts are dead once every element's destination is in registers, so
+    // they share their LDS with the staged keys
+    __shared__ union {
+        uint64_t key[kTile];
+        uint32_t wave[kWavesPerBlock][DIG];
+    } sKW;
+    static_assert(sizeof(uint32_t) * kWavesPerBlock * DIG <= sizeof(uint64_t) * kTile, "the counts fit the keys' space");
+    uint64_t* const sKey = sKW.key;
+    uint32_t (*const sWave)[DIG] = sKW.wave;
     __shared__ uint32_t sVal[kTile];
-    __shared__ uint32_t sWave[kWavesPerBlock][DIG];
     __shared__ uint32_t sLocal[DIG];
     __shared__ uint32_t sGlobal[DIG];
     __shared__ uint32_t scratch[kWavesPerBlock];
@@ -209,12 +216,16 @@ __global__ __launch_bounds__(kBlock) void kScatter(const uint64_t* __restrict__ 
     __syncthreads();
     digitExclusiveScan<DIG>(sLocal, sLocal, scratch);
     __syncthreads();
+    uint32_t pos[kIPT];
+#pragma unroll
+    for (int k = 0; k < kIPT; ++k)
+        pos[k] = dig[k] < (uint32_t)DIG ? sLocal[dig[k]] + sWave[wave][dig[k]] + rank[k] : 0u;
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < kIPT; ++k) {
         if (dig[k] < (uint32_t)DIG) {
-            const uint32_t p = sLocal[dig[k]] + sWave[wave][dig[k]] + rank[k];
-            sKey[p] = key[k];
-            sVal[p] = val[k];
+            sKey[pos[k]] = key[k];
+            sVal[pos[k]] = val[k];
         }
     }
     __syncthreads();
