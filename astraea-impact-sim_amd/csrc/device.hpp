@@ -131,7 +131,8 @@ __device__ __forceinline__ void waveSync() {
 // coalesced loads and finish(rec) derives what a record needs from them; each lane consumes its
 // entries that fall in the round, pair(rec) in table order, reading the next entry's record from
 // LDS before it runs pair on the current one.  Past the end of a bank a round stages copies of
-// its last slot (never read).
+// its last slot (never read).  S is a multiple of 32: a round is ceil(S / 64) wavefront loads, and
+// in the last one of a round of 160 only lanes 0-31 store their record.
 //
 // (Keeping the next round's records in flight while a round is evaluated does not pay here: the
 // wait for an entry chunk inside the pair loop is a wait for every older load, and the records
