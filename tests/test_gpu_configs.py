@@ -145,7 +145,7 @@ def test_giant_impact_1m_against_fp64(lib, orc):
     em = {name: rel_rms(*by_id(ctx, o, name, f64=True)) for name in ("rho", "u", "du_dt", "drho_dt")}
     _say(f"after one step: velocity {ev:.2e}, position {ex:.2e} of the extent,", {k: f"{v:.2e}" for k, v in em.items()})
     assert ev < 1e-6 and ex < 4e-6
-    assert all(v < 2e-5 for v in em.values()), em
+    assert all(v < 1e-5 for v in em.values()), em  # measured: 2e-8 (rho, u) to 1.7e-6 (drho/dt)
 
 
 # ---------------------------------------------------------------------------------------------

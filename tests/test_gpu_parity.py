@@ -452,11 +452,14 @@ def test_step_parity_one_step(lib, orc):
     assert np.max(np.abs(xg - xr)) < 1e-6 * np.max(np.abs(xr)) * 4
     vg, vr = vec_by_id(ctx, o, ["vel_x", "vel_y", "vel_z"])
     assert rel_rms(vg, vr) < 1e-6
-    for name in ("rho", "u", "du_dt", "drho_dt"):
-        g, r = by_id(ctx, o, name, f64=True)
-        assert rel_rms(g, r) < 2e-5, (name, rel_rms(g, r))
+    err = {name: rel_rms(*by_id(ctx, o, name, f64=True)) for name in ("rho", "u", "du_dt", "drho_dt")}
     ag, ar = vec_by_id(ctx, o, ["acc_grav_x", "acc_grav_y", "acc_grav_z"])
-    assert rel_rms(ag, ar) < 1e-4
+    err["acc_grav"] = rel_rms(ag, ar)
+    print("after one step", {k: f"{v:.2e}" for k, v in err.items()})
+    # the north-star bar; measured on MI355X: rho, u 3e-8, du/dt 2e-6, drho/dt 4e-6, gravity 6e-7
+    # (the forces are evaluated at positions that FP32 rounding moved by ~1e-7 of the extent)
+    for name, e in err.items():
+        assert e < 1e-5, (name, e)
 
 
 def test_graph_replay_matches_eager(lib):
