@@ -273,14 +273,14 @@ def test_force_parity_strength(lib, orc):
     st = perturbed_small_state(strength=True)
     p = S.plain_params(strength_mask=0x17)
     ctx, o = run_pair(lib, orc, st, [len(st["mass"])], p, [("initial_forces",)])
-    for name in ("ds_xx", "ds_xy", "ds_yz", "ddamage_cbrt_dt"):
-        g, r = by_id(ctx, o, name, f64=True)
-        assert rel_rms(g, r) < 1e-4, (name, rel_rms(g, r))
+    err = {name: rel_rms(*by_id(ctx, o, name, f64=True)) for name in ("ds_xx", "ds_xy", "ds_yz", "ddamage_cbrt_dt", "du_dt")}
     ag, ar = vec_by_id(ctx, o, ["acc_x", "acc_y", "acc_z"])
     gg, gr = vec_by_id(ctx, o, ["acc_grav_x", "acc_grav_y", "acc_grav_z"])
-    assert rel_rms(ag - gg, ar - gr) < 2e-5
-    g, r = by_id(ctx, o, "du_dt", f64=True)
-    assert rel_rms(g, r) < 1e-5
+    err["acc_sph"] = rel_rms(ag - gg, ar - gr)
+    print("strength initial forces", {k: f"{v:.2e}" for k, v in err.items()})
+    # the north-star bar for every field; measured on MI355X: 9e-8 (damage rate) to 4e-7 (a_sph)
+    for name, e in err.items():
+        assert e < 1e-5, (name, e)
 
 
 def test_eos_then_pass2_without_pass1(lib, orc):
@@ -303,10 +303,12 @@ def test_eos_then_pass2_without_pass1(lib, orc):
     ctx.synchronize()
     ag, ar = vec_by_id(ctx, o, ["acc_x", "acc_y", "acc_z"])
     gg, gr = vec_by_id(ctx, o, ["acc_grav_x", "acc_grav_y", "acc_grav_z"])
-    assert rel_rms(ag - gg, ar - gr) < 2e-5
-    for name, tol in (("du_dt", 1e-5), ("drho_dt", 1e-5), ("ds_xx", 1e-4), ("pressure", 1e-6)):
-        g, r = by_id(ctx, o, name, f64=True)
-        assert rel_rms(g, r) < tol, (name, rel_rms(g, r))
+    err = {"acc_sph": rel_rms(ag - gg, ar - gr)}
+    err.update({name: rel_rms(*by_id(ctx, o, name, f64=True)) for name in ("du_dt", "drho_dt", "ds_xx", "pressure")})
+    print("eos then pass 2", {k: f"{v:.2e}" for k, v in err.items()})
+    # measured on MI355X: 3e-8 (pressure) to 4e-7 (a_sph)
+    for name, tol in (("acc_sph", 1e-5), ("du_dt", 1e-5), ("drho_dt", 1e-5), ("ds_xx", 1e-5), ("pressure", 1e-6)):
+        assert err[name] < tol, (name, err[name])
 
 
 @pytest.mark.parametrize("theta", [0.45, 0.7])
