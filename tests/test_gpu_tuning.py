@@ -55,6 +55,30 @@ def test_variants_are_bit_identical(lib, strength):
             assert np.array_equal(got[k], want[k]), (t, k)
 
 
+def test_round_sizes_with_several_rounds_per_bank(lib):
+    """Smoothing lengths 1.7 times the state's (about 490 table entries a particle, four times as
+    many, so particles spill past the regular region too): a wavefront's union then spans several
+    rounds per bank at every round size, including the last, partly filled staging load of a
+    160-record round, and every round size gives the same state."""
+    st = perturbed_small_state(seed=3)
+    st["h"] = (st["h"] * 1.7).astype(np.float32)
+    want = None
+    for t in [(128, 128, 128, 8, 4, 1024, 4, 1024), (256, 160, 160, 8, 4, 1024, 4, 1024),
+              (256, 128, 192, 8, 4, 1024, 4, 1024), (128, 160, 256, 8, 4, 1024, 4, 1024)]:
+        ctx = _context(lib, st, False)
+        ctx.set_tuning(*t)
+        entries = ctx.counters()["nbr_entries"] / len(st["mass"])
+        print(t, "entries per particle", entries)
+        assert entries > 250.0, entries  # several hundred union members per bank
+        ctx.step(MAX_DT, 2, graph=False)
+        got = _state(ctx)
+        if want is None:
+            want = got
+            continue
+        for k in FIELDS:
+            assert np.array_equal(got[k], want[k]), (t, k)
+
+
 def test_autotune_steps_are_ordinary_steps(lib):
     st = perturbed_small_state()
     tuned = _context(lib, st, False)
