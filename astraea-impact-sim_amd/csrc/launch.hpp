@@ -34,8 +34,8 @@ struct SortBuffers {
 // use (and so occupancy), never results.
 struct Tuning {
     uint32_t pass1Round = 256;  // 128 or 256
-    uint32_t pass2Round = 128;  // 128 or 160
-    uint32_t driftRound = 128;  // 128, 160, 192 or 256
+    uint32_t pass2Round = 160;  // 128 or 160
+    uint32_t driftRound = 160;  // 128, 160, 192 or 256
     uint32_t sortBits = 8;      // radix sort digit: 8 (8 passes) or 9 (7 passes)
     // wavefronts per block and per-wave stack entries of the gravity walk and of the neighbour
     // build: (2, 1024), (4, 1024), (4, 512) or (8, 512)

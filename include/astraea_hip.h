@@ -366,8 +366,8 @@ int astraea_stage_times(astraea_ctx* ctx, double* out_ms, uint64_t* out_samples)
  * are (2, 1024), (4, 1024), (4, 512) and (8, 512). */
 typedef struct astraea_tuning {
     uint32_t pass1_round;   /* 128 or 256 (default 256) */
-    uint32_t pass2_round;   /* 128 or 160 (default 128) */
-    uint32_t drift_round;   /* 128, 160, 192 or 256 (default 128) */
+    uint32_t pass2_round;   /* 128 or 160 (default 160) */
+    uint32_t drift_round;   /* 128, 160, 192 or 256 (default 160) */
     uint32_t sort_bits;     /* 8 (8 passes) or 9 (7 passes) (default 8) */
     uint32_t gravity_waves; /* wavefronts per block of the gravity walk (default 4) */
     uint32_t gravity_stack; /* its per-wave stack entries (default 1024) */

@@ -111,5 +111,5 @@ def test_invalid_tuning_is_rejected(lib):
         ctx.set_tuning(256, 128, 128, 8, gravity_waves=8, gravity_stack=1024)  # 80 KB of LDS per block
     with pytest.raises(lib.AstraeaError):
         ctx.set_tuning(256, 128, 128, 8, build_waves=3)
-    assert ctx.tuning() == {"pass1_round": 256, "pass2_round": 128, "drift_round": 128, "sort_bits": 8,
+    assert ctx.tuning() == {"pass1_round": 256, "pass2_round": 160, "drift_round": 160, "sort_bits": 8,
                             "gravity_waves": 4, "gravity_stack": 1024, "build_waves": 4, "build_stack": 1024}
