@@ -44,6 +44,12 @@ MUTANTS = [
      "test_kick_von_mises_radial_return"),
     ("friction 0.5 P", "0.6 * std::max(0.0, s.P[i])", "0.5 * std::max(0.0, s.P[i])",
      "test_kick_von_mises_radial_return"),
+    ("thermal floor 2 c_v", "coldEnergy(mi, rhoN) + 3.0 * mp.cv", "coldEnergy(mi, rhoN) + 2.0 * mp.cv",
+     "test_kick_thermal_clamp"),
+    ("thermal floor at the old density", "coldEnergy(mi, rhoN) + 3.0 * mp.cv",
+     "coldEnergy(mi, rhoN - hdt * s.drho[i]) + 3.0 * mp.cv", "test_kick_thermal_clamp"),
+    ("no EOS after the clamp", "        if (clampThermal) {\n            const EosOut o",
+     "        if (false) {\n            const EosOut o", "test_kick_thermal_clamp"),
     ("living ambient 273 K", "? 255.0 : 3.0", "? 273.0 : 3.0", "test_cooling_ambient_and_threshold"),
     ("cooling threshold ignored", "if (tI > coolMin) {", "if (true) {", "test_cooling_ambient_and_threshold"),
     ("diffusion across materials", "if (s.mat[j] == matI) {", "if (true) {", "test_pass2_delta_sph_is_a_laplacian"),

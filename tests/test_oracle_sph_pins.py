@@ -20,6 +20,8 @@ mathematics fixes, independently of the oracle's code:
   the largest principal stress (numpy eigenvalues) over K + 4 mu/3 exceeds eps_fail, else 0;
 - the kick's yield (sph_gravity.cu:1342-1373): von Mises radial return to Y = (1 - D) Y0 (1 - T/Tm)
   or the friction limit D min(Y0, 0.6 P), by hand on diag(2, -1, -1) stresses;
+- the relaxation kicks' thermal floor (sph_gravity.cu:1317-1340): u' >= u_cold(rho') + 3 c_v at the
+  kicked density, i.e. the EOS's 3 K floor, with the EOS evaluated again;
 - the drift's XSPH (sph_gravity.cu:1452-1487): zero for a linear field on a lattice, and
   0.25 kappa H^2 / 15 for v_x = kappa y^2 (the kernel's second moment is H^2 / 15 per axis).
 
@@ -548,6 +550,47 @@ def test_kick_von_mises_radial_return(orc, case):
         assert abs(got[0] / S[0] - 0.5 * (1.0 - T / TMELT_GRANITE)) < 1e-6  # sigma_eq = 2 Y0
     if case == "elastic":
         assert np.array_equal(got, S)
+
+
+@pytest.mark.parametrize("case", ["cold", "warm"])
+def test_kick_thermal_clamp(orc, case):
+    """Relaxation kicks (sph_gravity.cu:1317-1323, 1334-1340) hold u at or above the energy of the
+    EOS's 3 K temperature floor at the kicked density, u_cold(rho') + 3 c_v, and evaluate the EOS
+    again at (rho', u').  Two granite particles closing at 200 m/s are compressed by the kick: one
+    that starts cold (u = 0) ends exactly on the floor, so its temperature reads 3 K and u' - u_cold
+    (rho') is 3 c_v, a property of the kicked density (u_cold(rho') > u_cold(rho)); one that starts at
+    c_v x 600 K keeps u + dt/2 du.  At 200 m/s the viscous heating of the cold pair stays below the
+floor (2 km/s would heat it past the floor).  The cold energy and the EOS are the ones pinned by the EOS
+    tests."""
+    cv = orc.material(1)["cv"] / 1e6  # c_v of granite in PSU energy per K
+    u0 = 0.0 if case == "cold" else cv * 600.0
+    st = _pair(0.06, (0.1, 0.0, 0.0), (-0.1, 0.0, 0.0), u=(u0, u0))
+    o = run(orc, st, PAIR_STAGES, orc.default_params(cooling_min_temp_k=1e12))
+    rho, drho = by_id(o, "rho"), by_id(o, "drho_dt")
+    u, du, P0 = by_id(o, "u"), by_id(o, "du_dt"), by_id(o, "pressure")
+    dt = 2e-3
+    o.set_dt([dt])
+    o.kick_half(True)
+    rho1, u1, T1, P1 = by_id(o, "rho"), by_id(o, "u"), by_id(o, "temp_k"), by_id(o, "pressure")
+    assert np.all(drho > 0.0) and np.all(rho1 > rho * (1 + 1e-6))  # compressed by the kick
+    assert np.allclose(rho1, rho + 0.5 * dt * drho, rtol=1e-15, atol=0.0)
+    for k in range(2):
+        floor = orc.cold_energy_psu(1, rho1[k]) + 3.0 * cv
+        assert floor > orc.cold_energy_psu(1, rho[k]) + 3.0 * cv * (1 + 1e-9)
+        if case == "cold":
+            assert abs(u1[k] - floor) <= 1e-13 * floor, (u1[k], floor)
+            assert abs(T1[k] - 3.0) <= 1e-6, T1[k]
+        else:
+            assert floor < u[k] + 0.5 * dt * du[k]
+            assert abs(u1[k] - (u[k] + 0.5 * dt * du[k])) <= 1e-14 * u1[k]
+            assert abs(T1[k] - 600.0) < 60.0
+        # the EOS of the kicked state (the FP32 evaluation: its inputs rounded to FP32), not the
+        # pressure before the kick
+        p, _, t, _ = orc.eos_eval(1, rho1[k], u1[k])
+        # (on the floor the pressure is a near cancellation, 1e-5 of the Tillotson terms)
+        tol, moved = (1e-3, 0.5) if case == "cold" else (1e-5, 5e-3)
+        assert abs(P1[k] - p) <= tol * abs(p) and abs(T1[k] - t) <= 1e-6 * t, (P1[k], p, T1[k], t)
+        assert abs(P1[k] - P0[k]) > moved * abs(P1[k])
 
 
 # ---------------------------------------------------------------------------------------------
