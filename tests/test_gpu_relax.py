@@ -5,7 +5,9 @@ Bars: force-free particles end at 0.85^steps of their v_rms about the mean (1e-5
 velocities).  Two bodies relaxed as two segments for 6 steps (each step's corrections change u,
 stress and velocities, so the forces after them test the EOS -> pass 2 path): positions (relative
 to the body radius) and u (relative) differ from the FP64 oracle by at most 1e-6 RMS, velocities
-by at most 3e-6 of v_esc, and the returned v_rms / v_esc by at most 1e-6.
+by at most 3e-6 of v_esc, and the returned v_rms / v_esc by at most 1e-6.  Bodies that start cold
+sit on the 3 K floor, where u follows the FP32 density: u within 5e-7 of the floor of the GPU's
+own density and within 1e-5 of the oracle.
 """
 import numpy as np
 import pytest
@@ -49,10 +51,14 @@ def _two_bodies():
 
 
 @pytest.mark.parametrize("precision", [PREC_FP32, PREC_FP64_ACCUM])
-@pytest.mark.parametrize("anchored", [False, True])
-def test_relax_two_bodies_matches_oracle(lib, orc, precision, anchored):
+@pytest.mark.parametrize("anchored,cold", [(False, False), (True, False), (False, True)])
+def test_relax_two_bodies_matches_oracle(lib, orc, precision, anchored, cold):
+    """cold: the bodies start at u = 0, below the 3 K floor, so both kicks of every relaxation step
+    raise u to u_cold(rho') + 3 c_v (sph_gravity.cu:1317-1323) and evaluate the EOS there."""
     st, counts, radii = _two_bodies()
     n = sum(counts)
+    if cold:
+        st["u"] = np.zeros(n, np.float32)
     p = default_params(precision=precision)
     anchor = (400.0 + 3.0 * (np.arange(n) % 11)) if anchored else None
     ctx = _ctx(lib, st, counts, p)
@@ -84,10 +90,30 @@ def test_relax_two_bodies_matches_oracle(lib, orc, precision, anchored):
                       u=np.sqrt(((ug[sel] - ur[sel]) ** 2).mean() / (ur[sel] ** 2).mean()),
                       ratio=abs(g[s] - r[s]))
     print("ratio gpu", g, "fp64", r, "errors", err)
+    u_bar = 1e-6
+    if cold:  # the floor binds: most particles sit at 3 K after the last kick, on the GPU as in the oracle
+        tg, tr = field("temp_k")
+        print("at the 3 K floor: gpu", np.mean(tg <= 3.001), "fp64", np.mean(tr <= 3.001))
+        assert np.mean(tr <= 3.001) > 0.5 and np.mean(tg <= 3.001) > 0.5
+        # On the floor u is a function of rho: u_cold(rho) + 3 c_v, whose slope P_cold / rho^2 turns
+        # the FP32 density's rounding (1e-7) into a few 1e-6 of u, as in the reference's FP32 kick.
+        # So the GPU's u is checked against the floor of its own density (FP32 rounding of the
+        # FP64 floor), and against the oracle at 1e-5.
+        ug = field("u")[0]
+        rg = field("rho")[0]
+        mat = np.empty(n, np.int64)
+        mat[ig] = ctx.download("material")
+        at = np.where(tg <= 3.001)[0]
+        want = np.array([orc.cold_energy_psu(int(mat[k]), float(rg[k])) + 3.0 * orc.material(int(mat[k]))["cv"] / 1e6
+                         for k in at])
+        rel = np.abs(ug[at] - want) / want
+        print("floor of the GPU's own density: max rel", rel.max())
+        assert rel.max() < 5e-7
+        u_bar = 1e-5
     assert (r > 0).all() and (r < 0.2).all()
     for s in range(2):
         e = err[s]
-        assert e["x"] < 1e-6 and e["v"] < 3e-6 and e["u"] < 1e-6 and e["ratio"] < 1e-6, (s, e)
+        assert e["x"] < 1e-6 and e["v"] < 3e-6 and e["u"] < u_bar and e["ratio"] < 1e-6, (s, e)
     # the corrections left no stress or damage
     for k in ("sxx", "sxy", "damage", "damage_cbrt"):
         assert not np.any(ctx.download(k)), k
